@@ -1,0 +1,153 @@
+"""Benchmark: sheared Heun particle-steps/s on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload shear_large]
+
+A step is one Heun sub-step (predictor force pass + corrector force pass with
+Lees-Edwards shear and cell re-binning) over the whole packing; inputs are
+resident in HBM before the timed region.  Prints one JSON line.
+
+Workloads (seeded synthetic packings, spherocyls_amd/configs.py):
+  shear_large  N=262144, R uniform [0.5, 0.7], aspect 1.5, phi=0.845,
+               strain rate 1e-6, dt 0.01  (default; largest single-GPU case)
+  compress     N=16384 bidisperse aspect 2, phi=0.80, strain rate 1e-5
+  shear_small  N=512 bidisperse aspect 2, phi=0.85, strain rate 1e-5
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from spherocyls_amd import configs, native  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle and launch (FP64 SoA, compulsory arrays once):
+#   predictor: read x,y,phi,R,A (40) + write x~,y~,phi~ (24) + write v0 (24)      =  88
+#   corrector: read x~ (24) + R,A (16) + v0 (24) + x (24), write x (24) + moved (32) = 144
+BYTES_PRED = 88
+BYTES_CORR = 144
+
+WORKLOADS = {
+    "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
+    "compress": (lambda: configs.compress_start(1), 1e-5, 0.01),
+    "shear_small": (lambda: configs.shear_small(1), 1e-5, 0.01),
+}
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(pk, strain_rate, dt, budget_s=15.0):
+    """Reference CPU path (oracle/_ref, else the oracle restatement), 1 thread,
+    on a bounded number of steps of the same packing."""
+    import oracle
+
+    kind = "reference" if oracle.ref_available() else "port"
+    cls = oracle.RefBox if kind == "reference" else oracle.OracleBox
+    b = cls.from_packing(pk, 0.1)
+    b.set_params(0.0, 0.0, strain_rate, 0.0, dt)
+    b.calc_forces()
+    b.run_steps(1, 0)
+    steps, elapsed = 0, 0.0
+    chunk = 1
+    while elapsed < budget_s:
+        t0 = time.perf_counter()
+        b.run_steps(chunk, 0)
+        elapsed += time.perf_counter() - t0
+        steps += chunk
+        chunk = min(chunk * 2, 64)
+    return {"value": pk.n * steps / elapsed, "unit": "particle-steps/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{steps} Heun steps of the same N={pk.n} packing on 1 core "
+                      f"({cpu_model()}, nproc={os.cpu_count()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="shear_large", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(packings and slabs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    make, strain_rate, dt = WORKLOADS[args.workload]
+    pk = make()
+    s = native.System(pk, pads=0.1)
+    s.set_dynamics(strain_rate=strain_rate, step=dt)
+    if args.precision == "mixed":
+        s.set_precision("mixed")
+    if args.tile and args.block:
+        s.set_tiling(args.tile, args.block)
+    tile, block = s.tiling()
+
+    s.strain_steps(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.strain_steps(args.steps)  # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = pk.n * args.steps / elapsed
+
+    # kernel-level figures: same steps again with HIP events on the system's stream
+    s.timing(True)
+    s.timing_reset()
+    s.strain_steps(args.steps)
+    pred_ms, pred_n = s.kernel_time(native.SC_KERNEL_PREDICT)
+    corr_ms, corr_n = s.kernel_time(native.SC_KERNEL_CORRECT)
+    s.timing(False)
+    s.close()
+    corr_avg = corr_ms / max(corr_n, 1) * 1e-3
+    pred_avg = pred_ms / max(pred_n, 1) * 1e-3
+    achieved = BYTES_CORR * pk.n / corr_avg / 1e9
+
+    out = {
+        "metric": "particle-steps/s (sheared Heun)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-sum",
+        "data": "synthetic (seeded non-crossing random packing)",
+        "config": {"workload": pk.name, "n_particles": pk.n, "strain_rate": strain_rate, "dt": dt,
+                   "cell_pad": 0.1, "tile_cols": tile, "block_threads": block},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tile<corrector>", "avg_us": corr_avg * 1e6,
+                     "predictor_avg_us": pred_avg * 1e6,
+                     "bytes_per_particle": BYTES_CORR},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pk, strain_rate, dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * spherocyls.h -- C ABI of libspherocyls.so, the MI355X (gfx950) spherocylinder
+ * dynamics library.
+ *
+ * The reference (tmarschall/spherocyls) has no FFI: its boundary is the C++ class
+ * SpherocylBox (src/cpp/spherocyl_box.h:85-120), whose methods the entry points
+ * below replace one by one (each cites the method it stands for).  The C++ host
+ * class in spherocyls_amd/host/spherocyl_box.h keeps that class's names and
+ * semantics on top of this ABI.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (SC_OK = 0); nothing calls exit().
+ *   - All arrays are caller-owned host memory, copied in and out; particle arrays
+ *     are in the caller's particle order (the library sorts particles by cell on
+ *     the device and undoes the permutation on every copy-out).
+ *   - A system holds one or more independent packings ("a batch"); packing k owns
+ *     the particles offsets[k] .. offsets[k+1]-1 of the concatenated arrays.  The
+ *     reference's single box is a system with one packing.
+ *   - One system per host thread; calls on one system are serialised on its own
+ *     HIP stream.  No global mutable state is shared between systems.
+ */
+#ifndef SPHEROCYLS_H
+#define SPHEROCYLS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SC_OK 0
+#define SC_ERR_ARG 1            /* invalid argument */
+#define SC_ERR_HIP 2            /* HIP runtime failure (message in sc_last_error) */
+#define SC_ERR_NO_DEVICE 3      /* no usable GPU */
+#define SC_ERR_CELL_OVERFLOW 4  /* more particles in one cell than the bucket holds */
+#define SC_ERR_STALE_CELLS 5    /* a predictor step moved a particle more than pad/2 */
+#define SC_ERR_PLACEMENT 6      /* random placement gave up */
+#define SC_ERR_LIST_OVERFLOW 7  /* more than the per-particle candidate list holds */
+#define SC_ERR_NOT_CONVERGED 8  /* minimiser hit its iteration limit */
+#define SC_ERR_STATE 9          /* call not valid in the current state */
+
+/* precision modes (sc_set_precision) */
+#define SC_PREC_FP64 0  /* all pair math in FP64 */
+#define SC_PREC_MIXED 1 /* FP32 pair math on FP64 relative displacements; FP64 sums,
+                           stress, energy and residual */
+
+/* sc_get_box fields (double out[SC_BOX_NFIELDS]) */
+#define SC_BOX_LX 0
+#define SC_BOX_LY 1
+#define SC_BOX_GAMMA 2
+#define SC_BOX_GAMMA_TOTAL 3
+#define SC_BOX_E 4     /* energy, sum over contacts of (1-d/sigma)^2 / (2N) */
+#define SC_BOX_PXX 5   /* stress with the reference's own formulas            */
+#define SC_BOX_PYY 6   /*   (src/cpp/spherocyl_box.cpp:483-495)                */
+#define SC_BOX_PXY 7
+#define SC_BOX_CPXX 8  /* corrected virial P_ab = sum F_b (CA_a - CB_a) / area */
+#define SC_BOX_CPYY 9
+#define SC_BOX_CPXY 10
+#define SC_BOX_CPYX 11
+#define SC_BOX_NCONTACTS 12
+#define SC_BOX_FRMS 13 /* sqrt(sum(Fx^2+Fy^2+tau^2)/(3N)) of the last force pass */
+#define SC_BOX_PACKING 14
+#define SC_BOX_ROWS 15
+#define SC_BOX_COLS 16
+#define SC_BOX_PAD 17
+#define SC_BOX_ITERS 18 /* minimiser iterations of the last sc_fire_minimize */
+#define SC_BOX_NFIELDS 19
+
+/* sc_strain_steps / sc_resize_steps record layout: per recorded sub-step and
+   packing, SC_SE_NFIELDS doubles: E Pxx Pyy Pxy (reference formulas),
+   cPxx cPyy cPxy cPyx (corrected), contacts */
+#define SC_SE_NFIELDS 9
+
+typedef struct sc_system sc_system;
+
+int sc_version(void);
+const char* sc_status_string(int status);
+/* Number of visible GPUs (0 on a host without one; never fails). */
+int sc_device_count(void);
+
+/* SpherocylBox(dL, N, pdX, pdY, pdPhi, pdA, pdR, dPadding)
+ *   src/cpp/spherocyl_box.cpp:152-173 -- copies the arrays, wraps positions and
+ *   angles as set_particle does (:188-221), sizes cells as configure_cells
+ *   (:223-268) and bins particles (find_neighbors, :320-355).
+ * Batched form: npack packings, offsets[npack+1], L[npack], pad[npack]. */
+int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
+                     const double* L, const double* x, const double* y, const double* phi,
+                     const double* A, const double* R, const double* pad);
+/* ~SpherocylBox()  src/cpp/spherocyl_box.cpp:174-186 */
+int sc_system_destroy(sc_system* s);
+/* Text of the last error on this system ("" if none). */
+const char* sc_last_error(const sc_system* s);
+int sc_num_particles(const sc_system* s, int* n_total, int* npack);
+
+/* set_strain_rate / set_resize_rate / set_step  src/cpp/spherocyl_box.h:112-114
+   (k = -1 applies to every packing) */
+int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate, double step);
+/* set_gamma / set_total_gamma  src/cpp/spherocyl_box.h:118-119 */
+int sc_set_gamma(sc_system* s, int k, double gamma, double gamma_total);
+int sc_set_precision(sc_system* s, int mode);
+/* Tile shape: columns per tile and threads per workgroup (the autotuner's knobs). */
+int sc_set_tiling(sc_system* s, int tile_cols, int block_threads);
+int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads);
+
+/* set_particle for every particle (src/cpp/spherocyl_box.cpp:188-221): new
+   positions / angles, wrapped, then cells rebuilt. */
+int sc_set_positions(sc_system* s, const double* x, const double* y, const double* phi);
+int sc_get_positions(sc_system* s, double* x, double* y, double* phi);
+/* Per-particle force and torque of the last force pass (the reference keeps them
+   in Spherocyl::m_dFx/m_dFy/m_dTau, src/cpp/spherocyl_box.h:17-19). */
+int sc_get_forces(sc_system* s, double* fx, double* fy, double* tau);
+int sc_get_box(sc_system* s, int k, double* out);
+
+/* calc_forces  src/cpp/strain_dynamics.cpp:137-153 */
+int sc_calc_forces(sc_system* s);
+/* calc_se  src/cpp/spherocyl_box.cpp:502-522 (forces, energy, stress) */
+int sc_calc_se(sc_system* s);
+
+/* nsteps Heun shear steps: the inner loop of run_strain
+ * (src/cpp/strain_dynamics.cpp:285-296, strain_step :179-235, Lees-Edwards remap
+ * set_back_gamma src/cpp/spherocyl_box.cpp:524-545).  Energy and stress are
+ * recorded after every se_every-th sub-step (0: never) into
+ * se_out[rec][pack][SC_SE_NFIELDS]; forces are left at the final state. */
+int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out);
+/* nsteps Heun resize steps (resize_step, src/cpp/resize_dynamics.cpp:164-243):
+ * shrink != 0 compresses.  corrected_image != 0 uses the rescaled box length in
+ * the predictor pass (the reference uses the old one, :101). */
+int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, int se_every,
+                    double* se_out);
+/* gradient_descent_minimize loop (src/cpp/relaxation_dynamics.cpp:33-85,
+ * gradient_relax_step :11-31): at most nmax steps, stopping when E <= emin. */
+int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done);
+/* Affine rescale of packing k (k = -1: all): x, y, L multiplied by scale[k]
+ * (the quasi-static compression step of src/cuda/strain_dynamics.cu:1111-1128). */
+int sc_affine_resize(sc_system* s, const double* scale);
+
+/* FIRE minimisation of every packing to F_rms < ftol (no reference counterpart;
+ * specification in oracle/sc_oracle.cpp).  params[10]:
+ *   dt0 dt_max n_min f_inc f_dec alpha0 f_alpha max_move ftol max_iter
+ * In SC_PREC_MIXED mode the pair math runs in FP32 until F_rms < switch_frms,
+ * then in FP64 to the end.  iters[npack] and frms[npack] may be NULL. */
+int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int* iters,
+                     double* frms);
+
+/* Neighbour set of packing k under the reference's criterion (check_neighbors
+ * src/cpp/spherocyl_box.cpp:303-318 over the own + 8 adjacent cells), as
+ * half-list pairs (p < q) in caller particle indices.  *count gets the total;
+ * only min(count, cap) pairs are written. */
+int sc_get_neighbors(sc_system* s, int k, int* p, int* q, int cap, int* count);
+/* Contacts (|D| < R_p + R_q) of packing k, p < q. */
+int sc_get_contacts(sc_system* s, int k, int* p, int* q, int cap, int* count);
+
+/* Per-kernel timing with HIP events on the system's stream. */
+#define SC_KERNEL_PREDICT 0 /* Heun predictor force pass */
+#define SC_KERNEL_CORRECT 1 /* Heun corrector force pass + cell binning */
+#define SC_KERNEL_FORCE 2   /* plain force / stress pass */
+#define SC_KERNEL_FIRE_FORCE 3
+#define SC_KERNEL_FIRE_UPDATE 4
+#define SC_KERNEL_SORT 5
+#define SC_KERNEL_COUNT 6
+int sc_timing_enable(sc_system* s, int enable);
+int sc_timing_get(sc_system* s, int kernel, double* ms_total, long long* launches);
+int sc_timing_reset(sc_system* s);
+int sc_synchronize(sc_system* s);
+
+/* Seeded random placement (counterpart of SpherocylBox::random_configuration,
+ * src/cpp/spherocyl_box.cpp:694-790, RANDOM mode): uniform centres in [0,L)^2
+ * (sheared coordinates at strain gamma) and angles in [0,pi), redrawing any
+ * particle whose segment comes within sqrt(1e-5)(R_p+R_q) of an earlier one
+ * under the Lees-Edwards pair separation.  Host only; needs no GPU. */
+int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
+                      unsigned long long seed, int max_tries, double* x, double* y, double* phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHEROCYLS_H */

@@ -1,0 +1,735 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU restatement of the spherocylinder dynamics of tmarschall/spherocyls
+// (src/cpp), written from the equations and operation order of that code so
+// that it agrees with the compiled reference to rounding.  It is the checker for
+// the HIP path and the timed CPU baseline ("port"); nothing on the product path
+// may link or call it.
+//
+// Pinning (what each part is checked against, see tests/):
+//   pair force / torque / E / stress .. golden vectors from the compiled
+//       reference (tests/golden), two-body closed forms, sum(F)=0,
+//       F = -N dE/dx by finite differences
+//   cells / neighbour list ............ golden neighbour sets, O(N^2) brute force
+//   Heun / resize / gradient steps .... golden one-step and trajectory states
+//   FIRE .............................. no reference counterpart: pinned by its
+//       defining properties (residual reaches tolerance, energy never rises on a
+//       step that keeps the velocity, reduces to gradient descent for alpha=1)
+#include "sc_oracle.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// The reference uses a truncated pi everywhere (src/cpp/spherocyl_box.cpp:12).
+constexpr double kPi = 3.14159265358979;
+
+struct Cells {
+  int rows = 1, cols = 1;
+  double w = 0, h = 0;
+  std::vector<int> adj;                 // 8 per cell
+  std::vector<std::vector<int>> members;
+};
+
+}  // namespace
+
+struct orc_box {
+  int n = 0;
+  double Lx = 0, Ly = 0, gamma = 0, gamma_total = 0, strain_rate = 0, resize_rate = 0;
+  double dt = 0.05, pad = 0, packing = 0, Rmax = 0, Amax = 0;
+  std::vector<double> x, y, phi, R, A, I, RC, fx, fy, tau, xm, ym;
+  std::vector<double> tx, ty, tphi, tfx, tfy, ttau;  // predictor copies
+  std::vector<int> cell;
+  Cells cg;
+  std::vector<std::vector<int>> nbr;  // half lists (q > p), duplicates kept
+  double E = 0, Pxx = 0, Pyy = 0, Pxy = 0;           // reference formulas
+  double cPxx = 0, cPyy = 0, cPxy = 0, cPyx = 0;     // corrected virial
+  int ncontacts = 0;
+};
+
+namespace {
+
+// set_particle's rotational constants, spherocyl_box.cpp:198-202
+void rot_consts(double R, double A, double* I, double* RC) {
+  double al = A / R;
+  double C = (3 * kPi + 24 * al + 6 * kPi * al * al + 8 * al * al * al);
+  double area = R * (kPi * R + 4 * A);
+  *I = R * R * R * R * C / (6 * area);
+  *RC = (8 * al + 6 * kPi * al * al + 8 * al * al * al) / C;
+}
+
+// Wrap rule of set_particle / find_cells (spherocyl_box.cpp:204-215, 279-290):
+// strict comparisons, so a coordinate equal to L stays at L.
+inline void wrap_pos(double& v, double L) {
+  while (v > L) v -= L;
+  while (v < 0.0) v += L;
+}
+inline void wrap_angle(double& p) {
+  while (p > kPi) p -= 2 * kPi;
+  while (p < -kPi) p += 2 * kPi;
+}
+
+// configure_cells, spherocyl_box.cpp:223-268
+void configure_cells(orc_box* b) {
+  double wmin = 2.24 * (b->Rmax + b->Amax) + b->pad;
+  double hmin = 2 * (b->Rmax + b->Amax) + b->pad;
+  Cells& c = b->cg;
+  c.rows = std::max(static_cast<int>(b->Ly / hmin), 1);
+  c.cols = std::max(static_cast<int>(b->Lx / wmin), 1);
+  c.w = b->Lx / c.cols;
+  c.h = b->Ly / c.rows;
+  if (b->pad == 0) b->pad = std::max(c.w - wmin, c.h - hmin);
+  int nc = c.rows * c.cols;
+  c.members.assign(nc, {});
+  c.adj.assign(8 * nc, 0);
+  for (int k = 0; k < nc; ++k) {
+    int r = k / c.cols, col = k % c.cols;
+    int cr = (col + 1) % c.cols, cl = (c.cols + col - 1) % c.cols;
+    int up = (r + 1) % c.rows, dn = (c.rows + r - 1) % c.rows;
+    int* a = &c.adj[8 * k];
+    a[0] = r * c.cols + cr;
+    a[1] = r * c.cols + cl;
+    a[2] = up * c.cols + col;
+    a[3] = up * c.cols + cr;
+    a[4] = up * c.cols + cl;
+    a[5] = dn * c.cols + col;
+    a[6] = dn * c.cols + cr;
+    a[7] = dn * c.cols + cl;
+  }
+}
+
+// find_cells, spherocyl_box.cpp:270-301
+void bin_particles(orc_box* b) {
+  Cells& c = b->cg;
+  for (auto& m : c.members) m.clear();
+  for (int p = 0; p < b->n; ++p) {
+    wrap_pos(b->x[p], b->Lx);
+    wrap_pos(b->y[p], b->Ly);
+    wrap_angle(b->phi[p]);
+    int col = (int)(b->x[p] / c.w);
+    int row = (int)(b->y[p] / c.h);
+    int id = col + row * c.cols;
+    b->cell[p] = id;
+    c.members[id].push_back(p);
+  }
+}
+
+// check_neighbors, spherocyl_box.cpp:303-318
+bool candidate(const orc_box* b, int p, int q) {
+  double reach = b->R[p] + b->A[p] + b->R[q] + b->A[q];
+  double dy = b->y[p] - b->y[q];
+  dy += b->Ly * ((dy < -0.5 * b->Ly) - (dy > 0.5 * b->Ly));
+  if (std::fabs(dy) < reach + b->pad) {
+    double dx = b->x[p] - b->x[q];
+    dx += b->Lx * ((dx < -0.5 * b->Lx) - (dx > 0.5 * b->Lx));
+    double dx_max_shear = dx + 0.5 * dy;
+    dx += b->gamma * dy;
+    if (std::fabs(dx) < reach + b->pad || std::fabs(dx_max_shear) < reach + b->pad) return true;
+  }
+  return false;
+}
+
+// find_neighbors, spherocyl_box.cpp:320-355
+void build_neighbors(orc_box* b) {
+  bin_particles(b);
+  for (int p = 0; p < b->n; ++p) {
+    b->nbr[p].clear();
+    b->xm[p] = 0.0;
+    b->ym[p] = 0.0;
+  }
+  const Cells& c = b->cg;
+  for (int p = 0; p < b->n - 1; ++p) {
+    int home = b->cell[p];
+    for (int q : c.members[home])
+      if (q > p && candidate(b, p, q)) b->nbr[p].push_back(q);
+    for (int k = 0; k < 8; ++k)
+      for (int q : c.members[c.adj[8 * home + k]])
+        if (q > p && candidate(b, p, q)) b->nbr[p].push_back(q);
+  }
+}
+
+// Result of one segment-segment closest-approach evaluation.
+struct PairOut {
+  bool contact;
+  double fx, fy, tau_p, tau_q;   // force on p (q gets -f) and torques
+  double s, t, dx, dy;           // contact parameters and D vector
+  double nxA, nyA, nxB, nyB;
+  double dist, sigma, dV;
+};
+
+// calc_force_pair / calc_temp_force_pair / calc_se_force_pair geometry,
+// strain_dynamics.cpp:17-75 (operation order kept for rounding agreement).
+inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap, double xq,
+                         double yq, double phq, double Rq, double Aq, double Lx, double Ly,
+                         double gam) {
+  PairOut o;
+  double ddx = xp - xq;
+  double ddy = yp - yq;
+  o.sigma = Rp + Rq;
+  ddx += Lx * ((ddx < -0.5 * Lx) - (ddx > 0.5 * Lx));
+  ddy += Ly * ((ddy < -0.5 * Ly) - (ddy > 0.5 * Ly));
+  ddx += gam * ddy;
+  o.nxA = Ap * std::cos(php);
+  o.nyA = Ap * std::sin(php);
+  o.nxB = Aq * std::cos(phq);
+  o.nyB = Aq * std::sin(phq);
+  double a = Ap * Ap;
+  double bb = -(o.nxA * o.nxB + o.nyA * o.nyB);
+  double c = Aq * Aq;
+  double d = o.nxA * ddx + o.nyA * ddy;
+  double e = -o.nxB * ddx - o.nyB * ddy;
+  double det = a * c - bb * bb;
+  double t = std::fmin(std::fmax((bb * d - a * e) / det, -1.), 1.);
+  double s = -(bb * t + d) / a;
+  double sabs = std::fabs(s);
+  s = std::fmin(std::fmax(s, -1.), 1.);
+  if (sabs > 1) t = std::fmin(std::fmax(-(bb * s + e) / c, -1.), 1.);
+  o.s = s;
+  o.t = t;
+  o.dx = ddx + s * o.nxA - t * o.nxB;
+  o.dy = ddy + s * o.nyA - t * o.nyB;
+  double d2 = o.dx * o.dx + o.dy * o.dy;
+  o.contact = d2 < o.sigma * o.sigma;
+  if (o.contact) {
+    o.dist = std::sqrt(d2);
+    o.dV = (1.0 - o.dist / o.sigma) / o.sigma;
+    o.fx = o.dx * o.dV / o.dist;
+    o.fy = o.dy * o.dV / o.dist;
+    o.tau_p = s * o.nxA * o.fy - s * o.nyA * o.fx;
+    o.tau_q = t * o.nyB * o.fx - t * o.nxB * o.fy;
+  } else {
+    o.dist = o.dV = o.fx = o.fy = o.tau_p = o.tau_q = 0;
+  }
+  return o;
+}
+
+inline PairOut pair_main(const orc_box* b, int p, int q, double gam, double Lx, double Ly) {
+  return pair_eval(b->x[p], b->y[p], b->phi[p], b->R[p], b->A[p], b->x[q], b->y[q], b->phi[q],
+                   b->R[q], b->A[q], Lx, Ly, gam);
+}
+inline PairOut pair_temp(const orc_box* b, int p, int q, double gam, double Lx, double Ly) {
+  return pair_eval(b->tx[p], b->ty[p], b->tphi[p], b->R[p], b->A[p], b->tx[q], b->ty[q],
+                   b->tphi[q], b->R[q], b->A[q], Lx, Ly, gam);
+}
+
+// calc_forces (strain_dynamics.cpp:137-153): zero, then half list with q > p.
+void forces_main(orc_box* b) {
+  for (int i = 0; i < b->n; ++i) b->fx[i] = b->fy[i] = b->tau[i] = 0.0;
+  for (int p = 0; p < b->n; ++p)
+    for (int q : b->nbr[p]) {
+      if (q <= p) continue;
+      PairOut o = pair_main(b, p, q, b->gamma, b->Lx, b->Ly);
+      if (!o.contact) continue;
+      b->fx[p] += o.fx;
+      b->fy[p] += o.fy;
+      b->tau[p] += o.tau_p;
+      b->fx[q] -= o.fx;
+      b->fy[q] -= o.fy;
+      b->tau[q] += o.tau_q;
+    }
+}
+
+// calc_temp_forces (strain_dynamics.cpp:156-172), with the predicted state and
+// gamma + dt*gamma_dot; Lx/Ly are passed so the resize predictor can choose.
+void forces_temp(orc_box* b, double Lx, double Ly) {
+  for (int i = 0; i < b->n; ++i) b->tfx[i] = b->tfy[i] = b->ttau[i] = 0.0;
+  double gam = b->gamma + b->dt * b->strain_rate;
+  for (int p = 0; p < b->n; ++p)
+    for (int q : b->nbr[p]) {
+      if (q <= p) continue;
+      PairOut o = pair_temp(b, p, q, gam, Lx, Ly);
+      if (!o.contact) continue;
+      b->tfx[p] += o.fx;
+      b->tfy[p] += o.fy;
+      b->ttau[p] += o.tau_p;
+      b->tfx[q] -= o.fx;
+      b->tfy[q] -= o.fy;
+      b->ttau[q] += o.tau_q;
+    }
+}
+
+// calc_se (spherocyl_box.cpp:425-522): forces plus energy and stress.  The
+// reference's stress carries two slips (spherocyl_box.cpp:486, 493); E/Pxx/Pyy/Pxy
+// reproduce them, cP** hold the corrected virial P_ab = sum F_b (CA_a - CB_a) / area.
+void forces_se(orc_box* b) {
+  for (int i = 0; i < b->n; ++i) b->fx[i] = b->fy[i] = b->tau[i] = 0.0;
+  b->E = b->Pxx = b->Pyy = b->Pxy = 0.0;
+  b->cPxx = b->cPyy = b->cPxy = b->cPyx = 0.0;
+  b->ncontacts = 0;
+  double area = b->Lx * b->Ly;
+  for (int p = 0; p < b->n; ++p)
+    for (int q : b->nbr[p]) {
+      if (q <= p) continue;
+      PairOut o = pair_main(b, p, q, b->gamma, b->Lx, b->Ly);
+      if (!o.contact) continue;
+      b->ncontacts++;
+      b->fx[p] += o.fx;
+      b->fy[p] += o.fy;
+      b->tau[p] += o.tau_p;
+      b->fx[q] -= o.fx;
+      b->fy[q] -= o.fy;
+      b->tau[q] += o.tau_q;
+      double cxa = o.s * o.nxA - 0.5 * o.dx;
+      double cya = o.s * o.nyA - 0.5 * o.dy;
+      double cxb = o.t * o.nxB + 0.5 * o.dx;
+      double cyb_ref = o.t * o.nxB + 0.5 * o.dy;  // reference slip (uses nxB)
+      double cyb = o.t * o.nyB + 0.5 * o.dy;
+      b->E += o.dV * o.sigma * (1.0 - o.dist / o.sigma) / (2.0 * b->n);
+      b->Pxx += o.fx * cxa / area;
+      b->Pxx -= o.fx * cxb / area;
+      b->Pyy += o.fy * cya / area;
+      b->Pyy -= o.fy * cyb_ref / area;
+      b->Pxy += o.fy * cxa / area;
+      b->Pxx -= o.fy * cxb / area;  // reference slip: should update Pxy
+      b->cPxx += o.fx * (cxa - cxb) / area;
+      b->cPyy += o.fy * (cya - cyb) / area;
+      b->cPxy += o.fy * (cxa - cxb) / area;
+      b->cPyx += o.fx * (cya - cyb) / area;
+    }
+}
+
+inline double iso_rot(double RC, double phi) { return 0.5 * (1 - RC * std::cos(2 * phi)); }
+
+inline bool moved_too_far(const orc_box* b, int i) {
+  return b->xm[i] > 0.5 * b->pad || b->xm[i] < -0.5 * b->pad || b->ym[i] > 0.5 * b->pad ||
+         b->ym[i] < -0.5 * b->pad;
+}
+
+// set_back_gamma, spherocyl_box.cpp:524-545 (Lx = Ly as the constructors enforce)
+void lees_edwards_remap(orc_box* b) {
+  for (int i = 0; i < b->n; ++i) {
+    if (b->y[i] < 0) b->y[i] += b->Ly;
+    else if (b->y[i] > b->Ly) b->y[i] -= b->Ly;
+    b->x[i] += b->y[i];
+    if (b->x[i] < 0) b->x[i] += b->Lx;
+    else
+      while (b->x[i] > b->Lx) b->x[i] -= b->Lx;
+  }
+  b->gamma -= 1.0;
+  b->gamma_total = int(b->gamma_total + 1) + b->gamma;
+  build_neighbors(b);
+}
+
+// strain_step, strain_dynamics.cpp:179-235 (Heun; rebuild inside the loop as the
+// reference does).
+void heun_step(orc_box* b) {
+  int n = b->n;
+  std::vector<double> v0x(n), v0y(n), v0t(n);
+  for (int i = 0; i < n; ++i) {
+    v0x[i] = b->fx[i] - b->gamma * b->fy[i];
+    v0y[i] = b->fy[i];
+    v0t[i] = b->tau[i] / b->I[i] - b->strain_rate * iso_rot(b->RC[i], b->phi[i]);
+    b->tx[i] = b->x[i] + b->dt * v0x[i];
+    b->ty[i] = b->y[i] + b->dt * v0y[i];
+    b->tphi[i] = b->phi[i] + b->dt * v0t[i];
+  }
+  forces_temp(b, b->Lx, b->Ly);
+  double gam1 = b->gamma + b->dt * b->strain_rate;
+  for (int i = 0; i < n; ++i) {
+    double v1x = b->tfx[i] - gam1 * b->tfy[i];
+    double v1y = b->tfy[i];
+    double v1t = b->ttau[i] / b->I[i] - b->strain_rate * iso_rot(b->RC[i], b->tphi[i]);
+    double ddx = 0.5 * b->dt * (v0x[i] + v1x);
+    double ddy = 0.5 * b->dt * (v0y[i] + v1y);
+    double ddp = 0.5 * b->dt * (v0t[i] + v1t);
+    b->x[i] += ddx;
+    b->y[i] += ddy;
+    b->phi[i] += ddp;
+    b->xm[i] += ddx;
+    b->ym[i] += ddy;
+    if (moved_too_far(b, i)) build_neighbors(b);
+  }
+  b->gamma += b->strain_rate * b->dt;
+  b->gamma_total += b->strain_rate * b->dt;
+  if (b->gamma >= 0.5) lees_edwards_remap(b);
+}
+
+// reconfigure_cells, resize_dynamics.cpp:106-121 (the reference compares rows
+// with Lx/hmin and cols with Ly/wmin; identical for Lx = Ly).
+void reconfigure_cells(orc_box* b) {
+  double wmin = 2.24 * (b->Rmax + b->Amax) + b->pad;
+  double hmin = 2 * (b->Rmax + b->Amax) + b->pad;
+  int rows = std::max(static_cast<int>(b->Lx / hmin), 1);
+  int cols = std::max(static_cast<int>(b->Ly / wmin), 1);
+  if (rows != b->cg.rows || cols != b->cg.cols) {
+    configure_cells(b);
+    build_neighbors(b);
+  } else {
+    b->cg.w = b->Lx / b->cg.cols;
+    b->cg.h = b->Ly / b->cg.rows;
+  }
+}
+
+// resize_step, resize_dynamics.cpp:164-243.  The reference's predictor force
+// pass uses the pre-step box length for the minimum image (corrected_image=0);
+// corrected_image=1 uses the rescaled length instead.
+void resize_step(orc_box* b, bool shrink, bool corrected_image) {
+  int n = b->n;
+  std::vector<double> f0x(n), f0y(n), t0(n);
+  double scale = std::pow(1.0 - b->resize_rate, b->dt);
+  for (int i = 0; i < n; ++i) {
+    f0x[i] = b->fx[i];
+    f0y[i] = b->fy[i];
+    t0[i] = b->tau[i];
+    b->tx[i] = b->x[i] + b->dt * f0x[i];
+    b->ty[i] = b->y[i] + b->dt * f0y[i];
+    b->tphi[i] = b->phi[i] + b->dt * t0[i] / b->I[i];
+    if (shrink) {
+      b->tx[i] *= std::pow(1.0 - b->resize_rate, b->dt);
+      b->ty[i] *= std::pow(1.0 - b->resize_rate, b->dt);
+    } else {
+      b->tx[i] /= std::pow(1.0 - b->resize_rate, b->dt);
+      b->ty[i] /= std::pow(1.0 - b->resize_rate, b->dt);
+    }
+  }
+  if (corrected_image) {
+    double Lx = shrink ? b->Lx * scale : b->Lx / scale;
+    double Ly = shrink ? b->Ly * scale : b->Ly / scale;
+    forces_temp(b, Lx, Ly);
+  } else {
+    forces_temp(b, b->Lx, b->Ly);
+  }
+  for (int i = 0; i < n; ++i) {
+    double ddx = 0.5 * b->dt * (f0x[i] + b->tfx[i]);
+    double ddy = 0.5 * b->dt * (f0y[i] + b->tfy[i]);
+    double ddp = 0.5 * b->dt * (t0[i] + b->ttau[i]) / b->I[i];
+    double x0 = b->x[i], y0 = b->y[i];
+    b->x[i] += ddx;
+    b->y[i] += ddy;
+    b->phi[i] += ddp;
+    if (shrink) {
+      b->x[i] *= std::pow(1.0 - b->resize_rate, b->dt);
+      b->y[i] *= std::pow(1.0 - b->resize_rate, b->dt);
+    } else {
+      b->x[i] /= std::pow(1.0 - b->resize_rate, b->dt);
+      b->y[i] /= std::pow(1.0 - b->resize_rate, b->dt);
+    }
+    b->xm[i] += b->x[i] - x0;
+    b->ym[i] += b->y[i] - y0;
+    reconfigure_cells(b);
+    if (moved_too_far(b, i)) build_neighbors(b);
+  }
+  if (shrink) {
+    b->Lx *= std::pow(1.0 - b->resize_rate, b->dt);
+    b->Ly *= std::pow(1.0 - b->resize_rate, b->dt);
+  } else {
+    b->Lx /= std::pow(1.0 - b->resize_rate, b->dt);
+    b->Ly /= std::pow(1.0 - b->resize_rate, b->dt);
+  }
+}
+
+// gradient_relax_step, relaxation_dynamics.cpp:11-31
+void gd_step(orc_box* b) {
+  for (int i = 0; i < b->n; ++i) {
+    double ddx = b->dt * b->fx[i];
+    double ddy = b->dt * b->fy[i];
+    b->x[i] += ddx;
+    b->y[i] += ddy;
+    b->phi[i] += b->dt * b->tau[i] / b->I[i];
+    b->xm[i] += ddx;
+    b->ym[i] += ddy;
+    if (moved_too_far(b, i)) build_neighbors(b);
+  }
+}
+
+double residual_rms(const orc_box* b) {
+  double s = 0;
+  for (int i = 0; i < b->n; ++i) s += b->fx[i] * b->fx[i] + b->fy[i] * b->fy[i] + b->tau[i] * b->tau[i];
+  return std::sqrt(s / (3.0 * b->n));
+}
+
+// ---------------------------------------------------------------------------
+// FIRE -- the project's specification (no reference counterpart; Bitzek et al.,
+// PRL 97, 170201 (2006), with the half-step retreat of FIRE 2.0, Guenole et al.,
+// Comput. Mater. Sci. 175, 109584 (2020)).  Generalised coordinates per particle
+// are (x, y, psi = phi*sqrt(I)) so that all three have unit mass; the generalised
+// force is g = (Fx, Fy, tau/sqrt(I)).  One iteration from (x_k, v_k):
+//   1. forces at x_k; stop if sqrt(sum(Fx^2+Fy^2+tau^2)/(3N)) < ftol
+//   2. P = g.v, G = g.g, V = v.v (sums over all particles)
+//   3. P >= 0: v <- (1-alpha) v + alpha sqrt(V/G) g; npos += 1;
+//             if npos > n_min: dt <- min(f_inc dt, dt_max), alpha <- f_alpha alpha
+//      else : x <- x - dt v / 2; v <- 0; dt <- f_dec dt; alpha <- alpha0; npos <- 0
+//   4. v <- v + dt g; displacement = dt v, its (x, y) part scaled down to length
+//      max_move if longer; x <- x + displacement
+// The sums are taken in FP64.
+struct FireParams {
+  double dt0, dt_max, n_min, f_inc, f_dec, alpha0, f_alpha, max_move, ftol, max_iter;
+};
+
+FireParams fire_params(const double* p) {
+  return FireParams{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]};
+}
+
+// Returns 1 when converged at entry (state untouched), 0 otherwise.
+int fire_iteration(orc_box* b, const FireParams& fp, double* vx, double* vy, double* vp,
+                   double* st) {
+  forces_main(b);
+  if (residual_rms(b) < fp.ftol) {
+    st[3] = 1;
+    return 1;
+  }
+  double P = 0, G = 0, V = 0;
+  for (int i = 0; i < b->n; ++i) {
+    double sq = std::sqrt(b->I[i]);
+    double gx = b->fx[i], gy = b->fy[i], gp = b->tau[i] / sq;
+    P += gx * vx[i] + gy * vy[i] + gp * vp[i];
+    G += gx * gx + gy * gy + gp * gp;
+    V += vx[i] * vx[i] + vy[i] * vy[i] + vp[i] * vp[i];
+  }
+  double dt = st[0], alpha = st[1], npos = st[2];
+  bool mix = P >= 0;  // v = 0 at the start gives P = 0: keep going, do not shrink dt
+  double c_keep = 0, c_force = 0, dt_new;
+  if (mix) {
+    c_keep = 1.0 - alpha;
+    c_force = (G > 0) ? alpha * std::sqrt(V / G) : 0.0;
+    npos += 1;
+    if (npos > fp.n_min) {
+      dt_new = std::fmin(dt * fp.f_inc, fp.dt_max);
+      alpha = alpha * fp.f_alpha;
+    } else {
+      dt_new = dt;
+    }
+  } else {
+    dt_new = dt * fp.f_dec;
+    alpha = fp.alpha0;
+    npos = 0;
+  }
+  for (int i = 0; i < b->n; ++i) {
+    double sq = std::sqrt(b->I[i]);
+    double gx = b->fx[i], gy = b->fy[i], gp = b->tau[i] / sq;
+    double ux, uy, up;
+    if (mix) {
+      ux = c_keep * vx[i] + c_force * gx;
+      uy = c_keep * vy[i] + c_force * gy;
+      up = c_keep * vp[i] + c_force * gp;
+    } else {
+      b->x[i] -= 0.5 * dt * vx[i];
+      b->y[i] -= 0.5 * dt * vy[i];
+      b->phi[i] -= 0.5 * dt * vp[i] / sq;
+      b->xm[i] -= 0.5 * dt * vx[i];
+      b->ym[i] -= 0.5 * dt * vy[i];
+      ux = uy = up = 0.0;
+    }
+    ux += dt_new * gx;
+    uy += dt_new * gy;
+    up += dt_new * gp;
+    vx[i] = ux;
+    vy[i] = uy;
+    vp[i] = up;
+    double mx = dt_new * ux, my = dt_new * uy;
+    double m2 = mx * mx + my * my;
+    if (m2 > fp.max_move * fp.max_move) {
+      double sc = fp.max_move / std::sqrt(m2);
+      mx *= sc;
+      my *= sc;
+    }
+    b->x[i] += mx;
+    b->y[i] += my;
+    b->phi[i] += dt_new * up / sq;
+    b->xm[i] += mx;
+    b->ym[i] += my;
+  }
+  for (int i = 0; i < b->n; ++i)
+    if (moved_too_far(b, i)) {
+      build_neighbors(b);
+      break;
+    }
+  st[0] = dt_new;
+  st[1] = alpha;
+  st[2] = npos;
+  st[3] = 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_box* orc_new(double L, int n, const double* x, const double* y, const double* phi,
+                 const double* A, const double* R, double pad) {
+  orc_box* b = new orc_box();
+  b->n = n;
+  b->Lx = b->Ly = L;
+  auto sz = [n](std::vector<double>& v) { v.assign(n, 0.0); };
+  for (auto* v : {&b->x, &b->y, &b->phi, &b->R, &b->A, &b->I, &b->RC, &b->fx, &b->fy, &b->tau,
+                  &b->xm, &b->ym, &b->tx, &b->ty, &b->tphi, &b->tfx, &b->tfy, &b->ttau})
+    sz(*v);
+  b->cell.assign(n, 0);
+  b->nbr.assign(n, {});
+  for (int p = 0; p < n; ++p) {
+    b->x[p] = x[p];
+    b->y[p] = y[p];
+    b->phi[p] = phi[p];
+    b->A[p] = A[p];
+    b->R[p] = R[p];
+    rot_consts(R[p], A[p], &b->I[p], &b->RC[p]);
+    wrap_pos(b->x[p], b->Lx);
+    wrap_pos(b->y[p], b->Ly);
+    wrap_angle(b->phi[p]);
+    if (A[p] > b->Amax) b->Amax = A[p];
+    if (R[p] > b->Rmax) b->Rmax = R[p];
+  }
+  double filled = 0.0;  // calculate_packing_fraction, spherocyl_box.cpp:357-366
+  for (int p = 0; p < n; ++p) filled += (kPi * b->R[p] + 4.0 * b->A[p]) * b->R[p];
+  b->packing = filled / (b->Lx * b->Ly);
+  b->pad = pad;
+  configure_cells(b);
+  build_neighbors(b);
+  return b;
+}
+
+void orc_free(orc_box* b) { delete b; }
+
+void orc_set_params(orc_box* b, double gamma, double gamma_total, double strain_rate,
+                    double resize_rate, double step) {
+  b->gamma = gamma;
+  b->gamma_total = gamma_total;
+  b->strain_rate = strain_rate;
+  b->resize_rate = resize_rate;
+  b->dt = step;
+}
+
+void orc_set_positions(orc_box* b, const double* x, const double* y, const double* phi) {
+  for (int i = 0; i < b->n; ++i) {
+    if (x) b->x[i] = x[i];
+    if (y) b->y[i] = y[i];
+    if (phi) b->phi[i] = phi[i];
+  }
+}
+
+void orc_find_neighbors(orc_box* b) { build_neighbors(b); }
+void orc_calc_forces(orc_box* b) { forces_main(b); }
+void orc_calc_se(orc_box* b) { forces_se(b); }
+void orc_strain_step(orc_box* b) { heun_step(b); }
+void orc_resize_step(orc_box* b, int shrink, int corrected_image) {
+  resize_step(b, shrink != 0, corrected_image != 0);
+}
+void orc_gd_step(orc_box* b) { gd_step(b); }
+void orc_set_back_gamma(orc_box* b) { lees_edwards_remap(b); }
+
+int orc_run_steps(orc_box* b, int nsteps, int se_every, double* se_out) {
+  int rec = 0;
+  for (int s = 1; s <= nsteps; ++s) {
+    heun_step(b);
+    if (se_every > 0 && s % se_every == 0) {
+      forces_se(b);
+      if (se_out) {
+        se_out[4 * rec + 0] = b->E;
+        se_out[4 * rec + 1] = b->Pxx;
+        se_out[4 * rec + 2] = b->Pyy;
+        se_out[4 * rec + 3] = b->Pxy;
+      }
+      ++rec;
+    } else {
+      forces_main(b);
+    }
+  }
+  return rec;
+}
+
+void orc_get_particles(const orc_box* b, double* x, double* y, double* phi, double* R, double* A,
+                       double* fx, double* fy, double* tau, double* I, double* RC) {
+  size_t bytes = sizeof(double) * b->n;
+  if (x) std::memcpy(x, b->x.data(), bytes);
+  if (y) std::memcpy(y, b->y.data(), bytes);
+  if (phi) std::memcpy(phi, b->phi.data(), bytes);
+  if (R) std::memcpy(R, b->R.data(), bytes);
+  if (A) std::memcpy(A, b->A.data(), bytes);
+  if (fx) std::memcpy(fx, b->fx.data(), bytes);
+  if (fy) std::memcpy(fy, b->fy.data(), bytes);
+  if (tau) std::memcpy(tau, b->tau.data(), bytes);
+  if (I) std::memcpy(I, b->I.data(), bytes);
+  if (RC) std::memcpy(RC, b->RC.data(), bytes);
+}
+
+void orc_force_scale(orc_box* b, double* sx, double* sy, double* st) {
+  for (int i = 0; i < b->n; ++i) sx[i] = sy[i] = st[i] = 0.0;
+  for (int p = 0; p < b->n; ++p)
+    for (int q : b->nbr[p]) {
+      if (q <= p) continue;
+      PairOut o = pair_main(b, p, q, b->gamma, b->Lx, b->Ly);
+      if (!o.contact) continue;
+      sx[p] += std::fabs(o.fx);
+      sy[p] += std::fabs(o.fy);
+      st[p] += std::fabs(o.tau_p);
+      sx[q] += std::fabs(o.fx);
+      sy[q] += std::fabs(o.fy);
+      st[q] += std::fabs(o.tau_q);
+    }
+}
+
+void orc_get_scalars(const orc_box* b, double* out) {
+  double v[20] = {b->Lx,  b->Ly,  b->gamma, b->gamma_total, b->strain_rate, b->resize_rate,
+                  b->dt,  b->packing, b->pad, b->E, b->Pxx, b->Pyy, b->Pxy, b->cPxx, b->cPyy,
+                  b->cPxy, b->cPyx, (double)b->cg.rows, (double)b->cg.cols, (double)b->ncontacts};
+  std::memcpy(out, v, sizeof(v));
+}
+
+int orc_get_neighbors(const orc_box* b, int* p, int* q, int cap) {
+  int k = 0;
+  for (int i = 0; i < b->n; ++i)
+    for (int j : b->nbr[i]) {
+      if (k < cap) {
+        p[k] = i;
+        q[k] = j;
+      }
+      ++k;
+    }
+  return k;
+}
+
+int orc_get_contacts(orc_box* b, int* p, int* q, int cap) {
+  int k = 0;
+  for (int i = 0; i < b->n; ++i)
+    for (int j : b->nbr[i]) {
+      if (j <= i) continue;
+      if (!pair_main(b, i, j, b->gamma, b->Lx, b->Ly).contact) continue;
+      if (k < cap) {
+        p[k] = i;
+        q[k] = j;
+      }
+      ++k;
+    }
+  return k;
+}
+
+int orc_brute_contacts(orc_box* b, int* p, int* q, int cap) {
+  int k = 0;
+  for (int i = 0; i < b->n; ++i)
+    for (int j = i + 1; j < b->n; ++j) {
+      if (!pair_main(b, i, j, b->gamma, b->Lx, b->Ly).contact) continue;
+      if (k < cap) {
+        p[k] = i;
+        q[k] = j;
+      }
+      ++k;
+    }
+  return k;
+}
+
+double orc_frms(orc_box* b) {
+  forces_main(b);
+  return residual_rms(b);
+}
+
+void orc_fire_iter(orc_box* b, const double* params, double* vx, double* vy, double* vpsi,
+                   double* fire_state) {
+  FireParams fp = fire_params(params);
+  fire_iteration(b, fp, vx, vy, vpsi, fire_state);
+}
+
+int orc_fire(orc_box* b, const double* params, double* frms_out) {
+  FireParams fp = fire_params(params);
+  std::vector<double> vx(b->n, 0.0), vy(b->n, 0.0), vp(b->n, 0.0);
+  double st[4] = {fp.dt0, fp.alpha0, 0.0, 0.0};
+  int it = 0;
+  for (; it < (int)fp.max_iter; ++it)
+    if (fire_iteration(b, fp, vx.data(), vy.data(), vp.data(), st)) break;
+  forces_main(b);
+  if (frms_out) *frms_out = residual_rms(b);
+  return it;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Seeded random packing generator (host code, C ABI).
+//
+// Counterpart of SpherocylBox::random_configuration (src/cpp/spherocyl_box.cpp:694-790):
+// particles are placed one after another at uniform random centres and angles,
+// and a placement is redrawn while it "crosses" an already placed particle, i.e.
+// while the squared closest distance between the two segments is below
+// 1e-5 * (R_p + R_q)^2 or is NaN (the reference's RANDOM mode, check_particle_cross
+// at :597-645).  Overlaps are allowed, exact crossings are not: a crossing gives a
+// zero-length contact vector, whose force direction is undefined (0/0 in the
+// pair force).  Unlike the reference (srand(time(0)), :739) the stream is seeded.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/spherocyls.h"
+
+namespace {
+
+// SplitMix64-seeded xoshiro256** generator (Blackman & Vigna, 2018).
+struct Rng {
+  uint64_t s[4];
+  static uint64_t splitmix(uint64_t& z) {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+    r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+    return r ^ (r >> 31);
+  }
+  explicit Rng(uint64_t seed) {
+    uint64_t z = seed;
+    for (auto& v : s) v = splitmix(z);
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    uint64_t r = rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
+};
+
+// Squared closest distance between two segments (same closest-approach
+// parametrisation as the pair force) for positions in sheared coordinates at
+// strain gamma: minimum image in each coordinate, then dx += gamma dy, the pair
+// separation of src/cpp/strain_dynamics.cpp:27-31.
+double seg_dist2(double xp, double yp, double pp, double Ap, double xq, double yq, double pq,
+                 double Aq, double L, double gamma) {
+  double dx = xp - xq, dy = yp - yq;
+  dx += L * ((dx < -0.5 * L) - (dx > 0.5 * L));
+  dy += L * ((dy < -0.5 * L) - (dy > 0.5 * L));
+  dx += gamma * dy;
+  double nxa = Ap * std::cos(pp), nya = Ap * std::sin(pp);
+  double nxb = Aq * std::cos(pq), nyb = Aq * std::sin(pq);
+  double a = Ap * Ap, b = -(nxa * nxb + nya * nyb), c = Aq * Aq;
+  double d = nxa * dx + nya * dy, e = -nxb * dx - nyb * dy;
+  double det = a * c - b * b;
+  double t = std::fmin(std::fmax((b * d - a * e) / det, -1.), 1.);
+  double s = -(b * t + d) / a;
+  double sa = std::fabs(s);
+  s = std::fmin(std::fmax(s, -1.), 1.);
+  if (sa > 1) t = std::fmin(std::fmax(-(b * s + e) / c, -1.), 1.);
+  double ex = dx + s * nxa - t * nxb, ey = dy + s * nya - t * nyb;
+  return ex * ex + ey * ey;
+}
+
+}  // namespace
+
+extern "C" int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
+                                 unsigned long long seed, int max_tries, double* x, double* y,
+                                 double* phi) {
+  if (n <= 0 || !(L > 0) || !(std::fabs(gamma) <= 0.5) || !A || !R || !x || !y || !phi)
+    return SC_ERR_ARG;
+  double reach = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!(R[i] > 0) || !(A[i] >= 0)) return SC_ERR_ARG;
+    reach = std::fmax(reach, 2 * (R[i] + A[i]));
+  }
+  // Uniform grid in sheared coordinates; with |gamma| <= 0.5 a pair within one
+  // reach (lab frame) is at most 1.12 reach apart in sheared x.
+  int nc = std::max(1, (int)(L / (1.12 * reach)));
+  double w = L / nc;
+  std::vector<std::vector<int>> grid((size_t)nc * nc);
+  Rng rng(seed);
+  const double kPi = 3.14159265358979323846;
+  for (int p = 0; p < n; ++p) {
+    int tries = 0;
+    for (;;) {
+      if (++tries > max_tries) return SC_ERR_PLACEMENT;
+      double px = L * rng.uniform(), py = L * rng.uniform(), pp = kPi * rng.uniform();
+      int cx = std::min(nc - 1, (int)(px / w)), cy = std::min(nc - 1, (int)(py / w));
+      bool bad = false;
+      int span = nc < 3 ? 0 : 1;
+      for (int oy = -span; oy <= span && !bad; ++oy)
+        for (int ox = -span; ox <= span && !bad; ++ox) {
+          int gx = (cx + ox + nc) % nc, gy = (cy + oy + nc) % nc;
+          for (int q : grid[(size_t)gy * nc + gx]) {
+            double sig = R[p] + R[q];
+            double d2 = seg_dist2(px, py, pp, A[p], x[q], y[q], phi[q], A[q], L, gamma);
+            if (d2 < 1e-5 * sig * sig || d2 != d2) {
+              bad = true;
+              break;
+            }
+          }
+        }
+      if (nc < 3 && !bad) {  // small box: compare with everybody
+        for (int q = 0; q < p && !bad; ++q) {
+          double sig = R[p] + R[q];
+          double d2 = seg_dist2(px, py, pp, A[p], x[q], y[q], phi[q], A[q], L, gamma);
+          if (d2 < 1e-5 * sig * sig || d2 != d2) bad = true;
+        }
+      }
+      if (bad) continue;
+      x[p] = px;
+      y[p] = py;
+      phi[p] = pp;
+      grid[(size_t)cy * nc + cx].push_back(p);
+      break;
+    }
+  }
+  return SC_OK;
+}

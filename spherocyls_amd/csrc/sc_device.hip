@@ -1,0 +1,1019 @@
+// Device kernels for gfx950 (MI355X).
+//
+// Work decomposition
+//   * Particles of each packing are stored sorted by cell (counting sort, see
+//     k_scan_cells / k_sort_cells / k_gather).  Cell membership is kept exact
+//     every step in fixed-capacity buckets (cnt[c], slot[c*kCap+k]) that the
+//     position-update kernels refill with one atomic per particle, so no
+//     neighbour-list rebuild trigger and no host round trip is needed.
+//   * A workgroup owns a tile: a run of cells of one cell row.  It stages the
+//     particles of the 3 x (width+2) cells around the tile into LDS (position,
+//     segment vector A(cos phi, sin phi), R, A, id), ordered by particle id within
+//     each cell so every sum below has a fixed order.
+//   * L lanes cooperate on one own particle.  Phase A tests all candidates of the
+//     9 surrounding cells with a cheap centre-distance bound and compacts the
+//     survivors into a per-particle LDS list (ballot + popcount, order kept);
+//     phase B runs the segment-segment closest-approach force on the list, so
+//     the expensive math is spent on real near pairs only.  The L partial sums
+//     are combined with xor shuffles.
+//   * Every pair is evaluated in the reference's orientation (lower particle id
+//     first, src/cpp/spherocyl_box.cpp:514-520), so both particles of a pair see
+//     bit-identical pair results.
+//   * Energy / stress / FIRE sums reduce per tile, then the last tile of each
+//     packing (ticket counter) reduces the tile partials in tile order.
+#include <hip/hip_runtime.h>
+
+#include "sc_internal.h"
+
+namespace sc {
+
+// ---------------------------------------------------------------------------
+// pair math
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PairOut {
+  T fx, fy, tp, tq, s, t, dx, dy, dist, dV;
+};
+
+// Segment-segment closest approach and harmonic contact force, following
+// calc_force_pair (src/cpp/strain_dynamics.cpp:38-64) term by term.  Returns
+// false when |D| >= sigma (or D is NaN).  FP64: no contraction, so the rounding
+// matches the reference's x86-64 build.
+__device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double nxA, double nyA,
+                                                 double nxB, double nyB, double Ap, double Aq,
+                                                 double sig, PairOut<double>& o) {
+#pragma clang fp contract(off)
+  double a = Ap * Ap;
+  double b = -(nxA * nxB + nyA * nyB);
+  double c = Aq * Aq;
+  double d = nxA * ddx + nyA * ddy;
+  double e = -nxB * ddx - nyB * ddy;
+  double det = a * c - b * b;
+  double t = fmin(fmax((b * d - a * e) / det, -1.0), 1.0);
+  double s = -(b * t + d) / a;
+  double sabs = fabs(s);
+  s = fmin(fmax(s, -1.0), 1.0);
+  if (sabs > 1.0) t = fmin(fmax(-(b * s + e) / c, -1.0), 1.0);
+  double Dx = ddx + s * nxA - t * nxB;
+  double Dy = ddy + s * nyA - t * nyB;
+  double d2 = Dx * Dx + Dy * Dy;
+  if (!(d2 < sig * sig)) return false;
+  double dist = sqrt(d2);
+  double dV = (1.0 - dist / sig) / sig;
+  double fx = Dx * dV / dist;
+  double fy = Dy * dV / dist;
+  o.fx = fx;
+  o.fy = fy;
+  o.tp = s * nxA * fy - s * nyA * fx;
+  o.tq = t * nyB * fx - t * nxB * fy;
+  o.s = s;
+  o.t = t;
+  o.dx = Dx;
+  o.dy = Dy;
+  o.dist = dist;
+  o.dV = dV;
+  return true;
+}
+
+// The same closest-approach force in FP32 (mixed mode), contraction allowed.
+__device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA, float nyA,
+                                                 float nxB, float nyB, float Ap, float Aq,
+                                                 float sig, PairOut<float>& o) {
+#pragma clang fp contract(fast)
+  float a = Ap * Ap;
+  float b = -(nxA * nxB + nyA * nyB);
+  float c = Aq * Aq;
+  float d = nxA * ddx + nyA * ddy;
+  float e = -nxB * ddx - nyB * ddy;
+  float det = a * c - b * b;
+  float t = fminf(fmaxf((b * d - a * e) / det, -1.0f), 1.0f);
+  float s = -(b * t + d) / a;
+  float sabs = fabsf(s);
+  s = fminf(fmaxf(s, -1.0f), 1.0f);
+  if (sabs > 1.0f) t = fminf(fmaxf(-(b * s + e) / c, -1.0f), 1.0f);
+  float Dx = ddx + s * nxA - t * nxB;
+  float Dy = ddy + s * nyA - t * nyB;
+  float d2 = Dx * Dx + Dy * Dy;
+  if (!(d2 < sig * sig)) return false;
+  float dist = sqrtf(d2);
+  float dV = (1.0f - dist / sig) / sig;
+  float fx = Dx * dV / dist;
+  float fy = Dy * dV / dist;
+  o.fx = fx;
+  o.fy = fy;
+  o.tp = s * nxA * fy - s * nyA * fx;
+  o.tq = t * nyB * fx - t * nxB * fy;
+  o.s = s;
+  o.t = t;
+  o.dx = Dx;
+  o.dy = Dy;
+  o.dist = dist;
+  o.dV = dV;
+  return true;
+}
+
+// Minimum image in sheared coordinates (src/cpp/strain_dynamics.cpp:28-29).
+__device__ __forceinline__ double min_image(double d, double L) {
+#pragma clang fp contract(off)
+  return d + L * (double)((d < -0.5 * L) - (d > 0.5 * L));
+}
+
+// Wrap rule of find_cells (src/cpp/spherocyl_box.cpp:279-286); bounded so a
+// runaway coordinate cannot spin.  Returns false for a non-finite coordinate.
+__device__ __forceinline__ bool wrap_coord(double& v, double L) {
+  if (!isfinite(v)) return false;
+  int guard = 0;
+  while (v > L && guard < 8) { v -= L; ++guard; }
+  while (v < 0.0 && guard < 16) { v += L; ++guard; }
+  if (guard >= 8) v = v - L * floor(v / L);
+  return true;
+}
+
+__device__ __forceinline__ void wrap_angle(double& p) {
+  if (!isfinite(p)) return;
+  int guard = 0;
+  while (p > kRefPi && guard < 8) { p -= 2 * kRefPi; ++guard; }
+  while (p < -kRefPi && guard < 16) { p += 2 * kRefPi; ++guard; }
+}
+
+// Bin one particle into bucket buffer (cnt, slot); the cell rule is find_cells'
+// (src/cpp/spherocyl_box.cpp:293-295), clamped so x == L stays in range.
+__device__ __forceinline__ void bin_into(int g, double xw, double yw, double Lx, double Ly,
+                                         const PackInfo& P, int* cnt, int* slot, int* err) {
+  double w = Lx / P.cols, h = Ly / P.rows;
+  int col = (int)(xw / w);
+  int row = (int)(yw / h);
+  col = min(max(col, 0), P.cols - 1);
+  row = min(max(row, 0), P.rows - 1);
+  int c = P.coff + row * P.cols + col;
+  int k = atomicAdd(&cnt[c], 1);
+  if (k < kCap) slot[c * kCap + k] = g;
+  else atomicOr(err, kErrCellOverflow);
+}
+
+// isolated_rotation, src/cpp/strain_dynamics.cpp:174-177
+__device__ __forceinline__ double iso_rot(double RC, double phi) {
+#pragma clang fp contract(off)
+  return 0.5 * (1 - RC * cos(2 * phi));
+}
+
+// ---------------------------------------------------------------------------
+// block reduction helpers (fixed order -> deterministic)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red, int nthr) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[i] = x;
+  }
+  const int w = tid >> 6, nw = nthr >> 6;
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s_red[w * NV + i] = v[i];
+  __syncthreads();
+  if (tid < NV) {
+    double x = 0;
+    for (int k = 0; k < nw; ++k) x += s_red[k * NV + tid];
+    s_red[tid] = x;  // each slot tid is read (k = 0) before this write by the same thread
+  }
+  __syncthreads();
+}
+
+// LDS carve-up of the tile kernel (sizes in entries).
+struct TileLds {
+  int* cs;             // staged cell prefix [128]
+  double *x, *y, *nx, *ny, *R, *A;
+  int *id, *g;
+  unsigned char* col;
+  unsigned short* list;
+  double* red;
+};
+
+__device__ __forceinline__ TileLds carve(unsigned char* base, int cap, int groups) {
+  TileLds t;
+  size_t off = 0;
+  t.cs = reinterpret_cast<int*>(base + off);
+  off += 128 * sizeof(int);
+  t.x = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.y = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.nx = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.ny = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.R = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.A = reinterpret_cast<double*>(base + off);
+  off += (size_t)cap * 8;
+  t.id = reinterpret_cast<int*>(base + off);
+  off += (size_t)cap * 4;
+  t.g = reinterpret_cast<int*>(base + off);
+  off += (size_t)cap * 4;
+  t.col = base + off;
+  off += ((size_t)cap + 15) / 16 * 16;
+  t.list = reinterpret_cast<unsigned short*>(base + off);
+  off += (size_t)groups * kListLen * 2;
+  off = (off + 15) / 16 * 16;
+  t.red = reinterpret_cast<double*>(base + off);
+  return t;
+}
+
+// Host mirror of carve(): bytes of dynamic LDS for a launch.
+size_t tile_lds_bytes(int cap, int groups, int nthr) {
+  size_t off = 128 * sizeof(int) + (size_t)cap * (6 * 8 + 4 + 4);
+  off += ((size_t)cap + 15) / 16 * 16;
+  off += (size_t)groups * kListLen * 2;
+  off = (off + 15) / 16 * 16;
+  off += (size_t)(nthr / 64) * kNSum * 8;
+  return off;
+}
+
+// Last-tile finalisation of the reduced sums of packing pk (thread 0 only).
+__device__ void finalize_pack(const TileArgs& a, int pk, const PackInfo& P, const PackScal& S,
+                              const double* tot, bool do_se) {
+#pragma clang fp contract(off)
+  double* ps = a.pack_sum + (size_t)pk * kNSum;
+  double area = S.Lx * S.Ly;
+  double frms = sqrt(tot[kSumR2] / (3.0 * P.n));
+  if (do_se) {
+    ps[kSeE] = tot[kSeE] * P.inv_2n;
+    for (int i = kSePxx; i <= kSeCPyx; ++i) ps[i] = tot[i] / area;
+    ps[kSeNc] = tot[kSeNc];
+    if (a.se_ring && a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0) {
+      int rec = S.step / a.se_every - 1;
+      double* r = a.se_ring + ((size_t)rec * a.npack + pk) * kNSe;
+      for (int i = 0; i < kNSe; ++i) r[i] = ps[i];
+    }
+  }
+  ps[kSumP] = tot[kSumP];
+  ps[kSumG] = tot[kSumG];
+  ps[kSumV] = tot[kSumV];
+  ps[kSumR2] = tot[kSumR2];
+  if (a.do_fire) {
+    const FireParams& fp = a.fire;
+    PackScal S2 = S;
+    S2.frms = frms;
+    if (frms < fp.ftol) {
+      S2.converged = 1;
+      S2.iters = S.step;
+    } else if (S.step >= (int)fp.max_iter) {
+      S2.converged = 2;
+      S2.iters = S.step;
+    } else {
+      double P_ = tot[kSumP], G_ = tot[kSumG], V_ = tot[kSumV];
+      double dt = S.fire_dt, alpha = S.fire_alpha, npos = S.fire_npos;
+      S2.fire_dt_old = dt;
+      if (P_ >= 0) {
+        S2.fire_mix = 1;
+        S2.fire_c_keep = 1.0 - alpha;
+        S2.fire_c_force = (G_ > 0) ? alpha * sqrt(V_ / G_) : 0.0;
+        npos += 1;
+        if (npos > fp.n_min) {
+          dt = fmin(dt * fp.f_inc, fp.dt_max);
+          alpha = alpha * fp.f_alpha;
+        }
+      } else {
+        S2.fire_mix = 0;
+        S2.fire_c_keep = 0;
+        S2.fire_c_force = 0;
+        dt = dt * fp.f_dec;
+        alpha = fp.alpha0;
+        npos = 0;
+      }
+      S2.fire_dt = dt;
+      S2.fire_dt_new = dt;
+      S2.fire_alpha = alpha;
+      S2.fire_npos = npos;
+      S2.step = S.step + 1;
+      if (S.prec == kFP32 && frms < fp.switch_frms) S2.prec = kFP64;
+    }
+    a.scal_next[pk] = S2;
+  } else if (a.do_gd) {
+    PackScal S2 = S;
+    S2.frms = frms;
+    if (ps[kSeE] <= a.fire.emin || S.step >= (int)a.fire.max_iter) {
+      S2.converged = 1;
+      S2.iters = S.step;
+    } else {
+      S2.step = S.step + 1;
+    }
+    a.scal_next[pk] = S2;
+  } else {
+    // plain force pass: keep the residual for sc_get_box
+    a.pack_sum[(size_t)pk * kNSum + kSumR2] = tot[kSumR2];
+  }
+  a.pack_done[pk] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// the tile force kernel
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(256) k_tile(TileArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int s_last;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+
+  if (a.cnt_clear)
+    for (int i = blockIdx.x * nthr + tid; i < a.ncells_total; i += gridDim.x * nthr)
+      a.cnt_clear[i] = 0;
+
+  const int4 tl = a.tiles[blockIdx.x];
+  const int pk = tl.x, row = tl.y, c0 = tl.z, ntw = tl.w;
+  const PackInfo P = a.packs[pk];
+  const PackScal S = a.scal[pk];
+
+  if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) {
+    if (blockIdx.x == P.toff && tid == 0) a.scal_next[pk] = S;
+    return;
+  }
+
+  const bool do_se =
+      (MODE != kModeCorr) &&
+      (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
+  const bool do_sum = do_se || (MODE == kModeForces && (a.do_fire || a.do_gd || a.do_resid));
+
+  TileLds L = carve(lds_raw, a.stage_cap, a.list_groups);
+
+  // ---- the staged 3 x (ntw + 2) block of cells
+  const int rows = P.rows, cols = P.cols;
+  int nr, ci, r0, r1, r2;
+  if (rows >= 3) {
+    nr = 3;
+    ci = 1;
+    r0 = (row + rows - 1) % rows;
+    r1 = row;
+    r2 = (row + 1) % rows;
+  } else if (rows == 2) {
+    nr = 2;
+    ci = 0;
+    r0 = row;
+    r1 = 1 - row;
+    r2 = row;
+  } else {
+    nr = 1;
+    ci = 0;
+    r0 = row;
+    r1 = row;
+    r2 = row;
+  }
+  const bool full = cols < ntw + 2;  // narrow box: the whole cell row is staged once
+  const int nc = full ? cols : ntw + 2;
+  const int ncell = nr * nc;
+
+  for (int k = tid; k < ncell; k += nthr) {
+    int rr = k / nc, jj = k - rr * nc;
+    int r = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
+    int cc = full ? jj : (c0 - 1 + jj + cols) % cols;
+    int cell = P.coff + r * cols + cc;
+    L.cs[k + 1] = min(a.cnt[cell], kCap);
+  }
+  __syncthreads();
+  if (tid < 64) {  // exclusive prefix of <= 128 cell counts in wave 0
+    int i0 = 2 * tid + 1, i1 = 2 * tid + 2;
+    int v0 = i0 <= ncell ? L.cs[i0] : 0;
+    int v1 = i1 <= ncell ? L.cs[i1] : 0;
+    int sum = v0 + v1;
+    for (int off = 1; off < 64; off <<= 1) {
+      int t = __shfl_up(sum, off);
+      if (tid >= off) sum += t;
+    }
+    int excl = sum - v0 - v1;
+    if (i0 <= ncell) L.cs[i0] = excl + v0;
+    if (i1 <= ncell) L.cs[i1] = excl + v0 + v1;
+    if (tid == 0) L.cs[0] = 0;
+  }
+  __syncthreads();
+  const int total = L.cs[ncell];
+
+  // ---- stage particles, ordered by id inside each cell
+  for (int e = tid; e < total; e += nthr) {
+    int lo = 0, hi = ncell;  // find k: cs[k] <= e < cs[k+1]
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (L.cs[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const int k = lo;
+    int rr = k / nc, jj = k - rr * nc;
+    int r = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
+    int cc = full ? jj : (c0 - 1 + jj + cols) % cols;
+    int cell = P.coff + r * cols + cc;
+    int m = L.cs[k + 1] - L.cs[k];
+    int j = e - L.cs[k];
+    const int* sl = a.slot + (size_t)cell * kCap;
+    int g = sl[j];
+    int myid = a.b.id[g];
+    int rank = 0;
+    for (int q = 0; q < m; ++q) rank += (a.b.id[sl[q]] < myid);
+    int dst = L.cs[k] + rank;
+    double ph = a.sphi[g];
+    double Ai = a.b.A[g];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    L.x[dst] = a.sx[g];
+    L.y[dst] = a.sy[g];
+    L.nx[dst] = Ai * cs;
+    L.ny[dst] = Ai * sn;
+    L.R[dst] = a.b.R[g];
+    L.A[dst] = Ai;
+    L.id[dst] = myid;
+    L.g[dst] = g;
+    L.col[dst] = (unsigned char)jj;
+  }
+  __syncthreads();
+
+  const int own_lo = L.cs[ci * nc + (full ? 0 : 1)];
+  const int own_hi = L.cs[ci * nc + (full ? nc : 1 + ntw)];
+  const int n_own = own_hi - own_lo;
+
+  // box scalars for this pass
+  const double dt = P.dt, gd = P.strain_rate;
+  double Lx = S.Lx, Ly = S.Ly;
+  double gam = S.gamma;
+  double L_new_x = Lx, L_new_y = Ly;
+  if (MODE != kModeForces && a.resize) {
+    L_new_x = a.shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
+    L_new_y = a.shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
+  }
+  if (MODE == kModeCorr) {
+    gam = S.gamma + dt * gd;
+    if (a.resize && a.corrected_image) {
+      Lx = L_new_x;
+      Ly = L_new_y;
+    }
+  }
+  const bool fp64 = S.prec == kFP64;
+
+  // lanes per particle
+  int LN = 1;
+  while (LN < 8 && LN * 2 * n_own <= nthr) LN *= 2;
+  while (nthr / LN > a.list_groups) LN *= 2;
+  const int G = nthr / LN;
+  const int grp = tid / LN, lane = tid - grp * LN;
+  const int grp_shift = (tid & 63) - lane;
+  const unsigned long long gmask = LN >= 64 ? ~0ull : ((1ull << LN) - 1ull);
+  unsigned short* my_list = L.list + grp * kListLen;
+
+  double acc[kNSum];
+#pragma unroll
+  for (int i = 0; i < kNSum; ++i) acc[i] = 0.0;
+
+  for (int base_o = own_lo; base_o < own_hi; base_o += G) {
+    const int o = base_o + grp;
+    if (o >= own_hi) continue;
+    const double xi = L.x[o], yi = L.y[o], Ri = L.R[o], Ai = L.A[o];
+    const int idi = L.id[o];
+    const int jc = L.col[o];
+    int lo_[3], n_[3];
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      if (rr < nr) {
+        int k0 = rr * nc + (full ? 0 : jc - 1), k1 = rr * nc + (full ? nc : jc + 2);
+        lo_[rr] = L.cs[k0];
+        n_[rr] = L.cs[k1] - lo_[rr];
+      } else {
+        lo_[rr] = 0;
+        n_[rr] = 0;
+      }
+    }
+    const int ntot = n_[0] + n_[1] + n_[2];
+    const double reach_i = Ri + Ai;
+
+    // phase A: centre-distance bound, compaction into my_list (order kept)
+    int nlist = 0;
+    for (int base = 0; base < ntot; base += LN) {
+      const int k = base + lane;
+      bool pass = false;
+      int e = 0;
+      if (k < ntot) {
+        e = k < n_[0] ? lo_[0] + k : (k < n_[0] + n_[1] ? lo_[1] + (k - n_[0]) : lo_[2] + (k - n_[0] - n_[1]));
+        if (L.id[e] != idi) {
+          double dx = min_image(xi - L.x[e], Lx);
+          double dy = min_image(yi - L.y[e], Ly);
+          dx += gam * dy;
+          double reach = reach_i + L.R[e] + L.A[e];
+          pass = dx * dx + dy * dy < reach * reach * (1.0 + 1e-10);
+        }
+      }
+      unsigned long long m = (__ballot(pass) >> grp_shift) & gmask;
+      if (pass) {
+        int pos = nlist + __popcll(m & ((1ull << lane) - 1ull));
+        if (pos < kListLen) my_list[pos] = (unsigned short)e;
+      }
+      nlist += __popcll(m);
+    }
+    if (nlist > kListLen) {
+      if (lane == 0) atomicOr(a.err, kErrList);
+      nlist = kListLen;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // phase B: closest-approach force on the compacted list
+    double Fx = 0.0, Fy = 0.0, Tq = 0.0;
+    for (int k = lane; k < nlist; k += LN) {
+      const int e = my_list[k];
+      const bool iP = idi < L.id[e];
+      const int p = iP ? o : e, q = iP ? e : o;
+      double ddx = min_image(L.x[p] - L.x[q], Lx);
+      double ddy = min_image(L.y[p] - L.y[q], Ly);
+      ddx += gam * ddy;
+      const double sig = L.R[p] + L.R[q];
+      PairOut<double> r;
+      bool hit;
+      if (fp64) {
+        hit = pair_contact_f64(ddx, ddy, L.nx[p], L.ny[p], L.nx[q], L.ny[q], L.A[p], L.A[q], sig, r);
+      } else {
+        PairOut<float> rf;
+        hit = pair_contact_f32((float)ddx, (float)ddy, (float)L.nx[p], (float)L.ny[p],
+                               (float)L.nx[q], (float)L.ny[q], (float)L.A[p], (float)L.A[q],
+                               (float)sig, rf);
+        r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
+        r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
+      }
+      if (!hit) continue;
+      if (iP) {
+        Fx += r.fx;
+        Fy += r.fy;
+        Tq += r.tp;
+        if (do_se) {
+          // contact arms, src/cpp/spherocyl_box.cpp:483-486
+          double nxA = L.nx[p], nyA = L.ny[p], nxB = L.nx[q], nyB = L.ny[q];
+          double cxa = r.s * nxA - 0.5 * r.dx;
+          double cya = r.s * nyA - 0.5 * r.dy;
+          double cxb = r.t * nxB + 0.5 * r.dx;
+          double cyb_ref = r.t * nxB + 0.5 * r.dy;
+          double cyb = r.t * nyB + 0.5 * r.dy;
+          acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
+          acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
+          acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
+          acc[kSePxy] += r.fy * cxa;
+          acc[kSeCPxx] += r.fx * (cxa - cxb);
+          acc[kSeCPyy] += r.fy * (cya - cyb);
+          acc[kSeCPxy] += r.fy * (cxa - cxb);
+          acc[kSeCPyx] += r.fx * (cya - cyb);
+          acc[kSeNc] += 1.0;
+        }
+      } else {
+        Fx -= r.fx;
+        Fy -= r.fy;
+        Tq += r.tq;
+      }
+    }
+    for (int off = LN >> 1; off > 0; off >>= 1) {
+      Fx += __shfl_xor(Fx, off);
+      Fy += __shfl_xor(Fy, off);
+      Tq += __shfl_xor(Tq, off);
+    }
+    if (lane != 0) continue;
+
+    // ---- per-particle epilogue
+    const int g = L.g[o];
+    if (MODE == kModeForces) {
+      a.b.fx[g] = Fx;
+      a.b.fy[g] = Fy;
+      a.b.tau[g] = Tq;
+      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
+      if (a.do_fire) {
+        double sq = sqrt(a.b.I[g]);
+        double gp = Tq / sq;
+        double vx = a.b.vx[g], vy = a.b.vy[g], vp = a.b.vpsi[g];
+        acc[kSumP] += Fx * vx + Fy * vy + gp * vp;
+        acc[kSumG] += Fx * Fx + Fy * Fy + gp * gp;
+        acc[kSumV] += vx * vx + vy * vy + vp * vp;
+      }
+    } else if (MODE == kModePred) {
+      const double x0 = xi, y0 = yi, p0 = a.b.phi[g];
+      double v0x, v0y, v0t, qx, qy, qp;
+      if (!a.resize) {
+        // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
+        v0x = Fx - S.gamma * Fy;
+        v0y = Fy;
+        v0t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], p0);
+        qx = x0 + dt * v0x;
+        qy = y0 + dt * v0y;
+        qp = p0 + dt * v0t;
+      } else {
+        // resize_step predictor, src/cpp/resize_dynamics.cpp:174-190
+        v0x = Fx;
+        v0y = Fy;
+        v0t = Tq;
+        qx = x0 + dt * v0x;
+        qy = y0 + dt * v0y;
+        qp = p0 + dt * v0t / a.b.I[g];
+        if (a.shrink) {
+          qx *= a.resize_scale;
+          qy *= a.resize_scale;
+        } else {
+          qx /= a.resize_scale;
+          qy /= a.resize_scale;
+        }
+      }
+      a.px[g] = qx;
+      a.py[g] = qy;
+      a.pphi[g] = qp;
+      a.v0x[g] = v0x;
+      a.v0y[g] = v0y;
+      a.v0t[g] = v0t;
+      if (fabs(dt * v0x) >= 0.5 * P.pad || fabs(dt * v0y) >= 0.5 * P.pad) atomicOr(a.err, kErrStale);
+      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
+    } else {  // kModeCorr
+      double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
+      double Lbx = Lx, Lby = Ly;
+      if (!a.resize) {
+        // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
+        double v1x = Fx - gam * Fy;
+        double v1y = Fy;
+        double v1t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], a.pphi[g]);
+        double ddx = 0.5 * dt * (a.v0x[g] + v1x);
+        double ddy = 0.5 * dt * (a.v0y[g] + v1y);
+        double ddp = 0.5 * dt * (a.v0t[g] + v1t);
+        x1 += ddx;
+        y1 += ddy;
+        p1 += ddp;
+        double gnew = S.gamma + gd * dt;
+        if (gnew >= 0.5) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
+          if (y1 < 0) y1 += S.Ly;
+          else if (y1 > S.Ly) y1 -= S.Ly;
+          x1 += y1;
+          if (x1 < 0) x1 += S.Lx;
+          else {
+            int guard = 0;
+            while (x1 > S.Lx && guard++ < 8) x1 -= S.Lx;
+          }
+        }
+        Lbx = S.Lx;
+        Lby = S.Ly;
+      } else {
+        // resize_step corrector, src/cpp/resize_dynamics.cpp:196-215
+        double ddx = 0.5 * dt * (a.v0x[g] + Fx);
+        double ddy = 0.5 * dt * (a.v0y[g] + Fy);
+        double ddp = 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
+        x1 += ddx;
+        y1 += ddy;
+        p1 += ddp;
+        if (a.shrink) {
+          x1 *= a.resize_scale;
+          y1 *= a.resize_scale;
+        } else {
+          x1 /= a.resize_scale;
+          y1 /= a.resize_scale;
+        }
+        Lbx = L_new_x;
+        Lby = L_new_y;
+      }
+      bool okx = wrap_coord(x1, Lbx), oky = wrap_coord(y1, Lby);
+      if (!okx || !oky || !isfinite(p1)) atomicOr(a.err, kErrNonFinite);
+      a.b.x[g] = x1;
+      a.b.y[g] = y1;
+      a.b.phi[g] = p1;
+      bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
+    }
+  }
+
+  // ---- scalar advance of this packing (corrector: one thread per packing)
+  if (MODE == kModeCorr && blockIdx.x == P.toff && tid == 0) {
+    PackScal S2 = S;
+    S2.step = S.step + 1;
+    if (!a.resize) {
+      S2.gamma = S.gamma + gd * dt;
+      S2.gamma_total = S.gamma_total + gd * dt;
+      if (S2.gamma >= 0.5) {
+        S2.gamma -= 1.0;
+        S2.gamma_total = (double)(int)(S2.gamma_total + 1) + S2.gamma;
+      }
+    } else {
+      S2.Lx = L_new_x;
+      S2.Ly = L_new_y;
+    }
+    a.scal_next[pk] = S2;
+  }
+
+  if (!do_sum) return;
+  block_sum<kNSum>(acc, L.red, nthr);
+  if (tid < kNSum) a.tile_part[(size_t)blockIdx.x * kNSum + tid] = L.red[tid];
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    unsigned t = atomicAdd(&a.pack_done[pk], 1u);
+    s_last = (t == (unsigned)P.ntiles - 1u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double tot[kNSum];
+#pragma unroll
+  for (int i = 0; i < kNSum; ++i) tot[i] = 0.0;
+  for (int t = P.toff + tid; t < P.toff + P.ntiles; t += nthr) {
+    const double* tp = a.tile_part + (size_t)t * kNSum;
+#pragma unroll
+    for (int i = 0; i < kNSum; ++i) tot[i] += __builtin_nontemporal_load(tp + i);
+  }
+  __syncthreads();
+  block_sum<kNSum>(tot, L.red, nthr);
+  if (tid == 0) {
+    double fin[kNSum];
+    for (int i = 0; i < kNSum; ++i) fin[i] = L.red[i];
+    finalize_pack(a, pk, P, S, fin, do_se);
+  }
+}
+
+template __global__ void k_tile<kModeForces>(TileArgs);
+template __global__ void k_tile<kModePred>(TileArgs);
+template __global__ void k_tile<kModeCorr>(TileArgs);
+
+// ---------------------------------------------------------------------------
+// particle-parallel kernels
+// ---------------------------------------------------------------------------
+
+// Wrap every particle (set_particle / find_cells rules) and bin it.
+__global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
+                          int* cnt, int* slot, int* err, int ntot, int wrap_angles) {
+#pragma clang fp contract(off)
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  int pk = pack_of[g];
+  const PackInfo P = packs[pk];
+  const PackScal S = scal[pk];
+  double x = b.x[g], y = b.y[g];
+  bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
+  bool ok = okx && oky;
+  if (!ok) atomicOr(err, kErrNonFinite);
+  b.x[g] = x;
+  b.y[g] = y;
+  if (wrap_angles) {
+    double p = b.phi[g];
+    wrap_angle(p);
+    b.phi[g] = p;
+  }
+  bin_into(g, x, y, S.Lx, S.Ly, P, cnt, slot, err);
+}
+
+// Gradient-descent / FIRE position update (one thread per particle) + binning.
+// mode 0: gradient_relax_step (src/cpp/relaxation_dynamics.cpp:15-19)
+// mode 1: FIRE update (specification in oracle/sc_oracle.cpp)
+__global__ void k_update(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
+                         int* cnt, int* slot, int* cnt_clear, int ncells_total, int* err, int ntot,
+                         int mode, double max_move) {
+#pragma clang fp contract(off)
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cnt_clear)
+    for (int i = g; i < ncells_total; i += gridDim.x * blockDim.x) cnt_clear[i] = 0;
+  if (g >= ntot) return;
+  int pk = pack_of[g];
+  const PackInfo P = packs[pk];
+  const PackScal S = scal[pk];
+  double x = b.x[g], y = b.y[g], p = b.phi[g];
+  if (!S.converged) {
+    if (mode == 0) {
+      double dt = P.dt;
+      x += dt * b.fx[g];
+      y += dt * b.fy[g];
+      p += dt * b.tau[g] / b.I[g];
+    } else {
+      double sq = sqrt(b.I[g]);
+      double gx = b.fx[g], gy = b.fy[g], gp = b.tau[g] / sq;
+      double vx = b.vx[g], vy = b.vy[g], vp = b.vpsi[g];
+      double ux, uy, up;
+      if (S.fire_mix) {
+        ux = S.fire_c_keep * vx + S.fire_c_force * gx;
+        uy = S.fire_c_keep * vy + S.fire_c_force * gy;
+        up = S.fire_c_keep * vp + S.fire_c_force * gp;
+      } else {
+        x -= 0.5 * S.fire_dt_old * vx;
+        y -= 0.5 * S.fire_dt_old * vy;
+        p -= 0.5 * S.fire_dt_old * vp / sq;
+        ux = uy = up = 0.0;
+      }
+      double dtn = S.fire_dt_new;
+      ux += dtn * gx;
+      uy += dtn * gy;
+      up += dtn * gp;
+      b.vx[g] = ux;
+      b.vy[g] = uy;
+      b.vpsi[g] = up;
+      double mx = dtn * ux, my = dtn * uy;
+      double m2 = mx * mx + my * my;
+      if (m2 > max_move * max_move) {
+        double sc = max_move / sqrt(m2);
+        mx *= sc;
+        my *= sc;
+      }
+      x += mx;
+      y += my;
+      p += dtn * up / sq;
+    }
+  }
+  bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
+  bool ok = okx && oky;
+  if (!ok || !isfinite(p)) atomicOr(err, kErrNonFinite);
+  b.x[g] = x;
+  b.y[g] = y;
+  b.phi[g] = p;
+  bin_into(g, x, y, S.Lx, S.Ly, P, cnt, slot, err);
+}
+
+// Affine rescale (x, y) *= scale[pack] (quasi-static compression step).
+__global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  double s = scale[pack_of[g]];
+  b.x[g] *= s;
+  b.y[g] *= s;
+}
+
+// ---- counting sort by cell ------------------------------------------------
+
+// Exclusive scan of bucket counts over the cells of each packing (one
+// workgroup per packing, chunks of blockDim cells).
+__global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart) {
+  __shared__ int s_w[16];
+  __shared__ int s_carry;
+  const PackInfo P = packs[blockIdx.x];
+  const int ncell = P.rows * P.cols;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (tid == 0) s_carry = P.poff;
+  __syncthreads();
+  for (int base = 0; base < ncell; base += nthr) {
+    int c = base + tid;
+    int v = c < ncell ? min(cnt[P.coff + c], kCap) : 0;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      int t = __shfl_up(incl, off);
+      if ((tid & 63) >= off) incl += t;
+    }
+    if ((tid & 63) == 63) s_w[tid >> 6] = incl;
+    __syncthreads();
+    int wpre = 0;
+    for (int w = 0; w < (tid >> 6); ++w) wpre += s_w[w];
+    int carry = s_carry;
+    if (c < ncell) cstart[P.coff + c] = carry + wpre + incl - v;
+    __syncthreads();
+    if (tid == nthr - 1) s_carry = carry + wpre + incl;
+    __syncthreads();
+  }
+}
+
+// Per cell: order members by particle id, record the permutation and rewrite
+// the bucket with the new (sorted) storage indices.
+__global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
+                             int ncells_total) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells_total) return;
+  int m = min(cnt[c], kCap);
+  int mem[kCap], key[kCap];
+  for (int j = 0; j < m; ++j) {
+    mem[j] = slot[c * kCap + j];
+    key[j] = id[mem[j]];
+  }
+  for (int j = 1; j < m; ++j) {  // insertion sort by id
+    int km = key[j], mm = mem[j], i = j - 1;
+    while (i >= 0 && key[i] > km) {
+      key[i + 1] = key[i];
+      mem[i + 1] = mem[i];
+      --i;
+    }
+    key[i + 1] = km;
+    mem[i + 1] = mm;
+  }
+  int base = cstart[c];
+  for (int j = 0; j < m; ++j) {
+    perm[base + j] = mem[j];
+    slot[c * kCap + j] = base + j;
+  }
+  cnt[c] = m;
+}
+
+// Coalesced gather into the other bank: dst[d] = src[perm[d]].
+__global__ void k_gather(Bank src, Bank dst, const int* perm, int ntot, int with_velocity) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ntot) return;
+  int s = perm[d];
+  // after a bucket overflow the permutation has holes (the overflow is reported
+  // to the host); keep every read in bounds regardless
+  if (s < 0 || s >= ntot) s = d;
+  dst.x[d] = src.x[s];
+  dst.y[d] = src.y[s];
+  dst.phi[d] = src.phi[s];
+  dst.R[d] = src.R[s];
+  dst.A[d] = src.A[s];
+  dst.I[d] = src.I[s];
+  dst.RC[d] = src.RC[s];
+  dst.fx[d] = src.fx[s];
+  dst.fy[d] = src.fy[s];
+  dst.tau[d] = src.tau[s];
+  dst.id[d] = src.id[s];
+  if (with_velocity) {
+    dst.vx[d] = src.vx[s];
+    dst.vy[d] = src.vy[s];
+    dst.vpsi[d] = src.vpsi[s];
+  }
+}
+
+// ---- neighbour / contact export (diagnostics) ------------------------------
+
+// For every particle of packing pk: the reference's neighbour criterion
+// (check_neighbors, src/cpp/spherocyl_box.cpp:303-318) or the contact test over
+// the own + 8 adjacent cells (deduplicated), partners with a larger id only.
+// Pairs go to a per-particle region of width kExport.
+constexpr int kExport = 64;
+__global__ void k_export(Bank b, const PackInfo* packs, const PackScal* scal, const int* cnt,
+                         const int* slot, int pk, int contacts, int* out_q, int* out_n) {
+#pragma clang fp contract(off)
+  const PackInfo P = packs[pk];
+  const PackScal S = scal[pk];
+  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= P.n) return;
+  int g = P.poff + li;
+  double w = S.Lx / P.cols, h = S.Ly / P.rows;
+  int col = min(max((int)(b.x[g] / w), 0), P.cols - 1);
+  int row = min(max((int)(b.y[g] / h), 0), P.rows - 1);
+  int rs[3], cs[3], nr = 0, ncl = 0;
+  for (int d = -1; d <= 1; ++d) {
+    int r = (row + d + P.rows) % P.rows;
+    bool dup = false;
+    for (int k = 0; k < nr; ++k) dup |= rs[k] == r;
+    if (!dup) rs[nr++] = r;
+    int c = (col + d + P.cols) % P.cols;
+    dup = false;
+    for (int k = 0; k < ncl; ++k) dup |= cs[k] == c;
+    if (!dup) cs[ncl++] = c;
+  }
+  int idp = b.id[g];
+  int count = 0;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < ncl; ++j) {
+      int cell = P.coff + rs[i] * P.cols + cs[j];
+      int m = min(cnt[cell], kCap);
+      for (int k = 0; k < m; ++k) {
+        int q = slot[cell * kCap + k];
+        int idq = b.id[q];
+        if (idq <= idp) continue;
+        bool hit;
+        if (!contacts) {
+          double reach = b.R[g] + b.A[g] + b.R[q] + b.A[q];
+          double dy = min_image(b.y[g] - b.y[q], S.Ly);
+          hit = false;
+          if (fabs(dy) < reach + P.pad) {
+            double dx = min_image(b.x[g] - b.x[q], S.Lx);
+            double dxp = dx + 0.5 * dy;
+            dx += S.gamma * dy;
+            hit = fabs(dx) < reach + P.pad || fabs(dxp) < reach + P.pad;
+          }
+        } else {
+          double ddx = min_image(b.x[g] - b.x[q], S.Lx);
+          double ddy = min_image(b.y[g] - b.y[q], S.Ly);
+          ddx += S.gamma * ddy;
+          PairOut<double> r;
+          hit = pair_contact_f64(ddx, ddy, b.A[g] * cos(b.phi[g]), b.A[g] * sin(b.phi[g]),
+                                 b.A[q] * cos(b.phi[q]), b.A[q] * sin(b.phi[q]), b.A[g], b.A[q],
+                                 b.R[g] + b.R[q], r);
+        }
+        if (hit) {
+          if (count < kExport) out_q[(size_t)li * kExport + count] = idq;
+          ++count;
+        }
+      }
+    }
+  out_n[li] = count;
+}
+
+// ---- caller-order transfer -------------------------------------------------
+
+// out[poff + id[g]] = field(g) for the selected fields (positions or forces).
+__global__ void k_to_caller(Bank b, const int* pack_of, const PackInfo* packs, int what,
+                            double* o0, double* o1, double* o2, int ntot) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  int dst = packs[pack_of[g]].poff + b.id[g];
+  if (what == 0) {
+    o0[dst] = b.x[g];
+    o1[dst] = b.y[g];
+    o2[dst] = b.phi[g];
+  } else {
+    o0[dst] = b.fx[g];
+    o1[dst] = b.fy[g];
+    o2[dst] = b.tau[g];
+  }
+}
+
+// field(g) = in[poff + id[g]] for positions.
+__global__ void k_from_caller(Bank b, const int* pack_of, const PackInfo* packs, const double* i0,
+                              const double* i1, const double* i2, int ntot) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  int src = packs[pack_of[g]].poff + b.id[g];
+  b.x[g] = i0[src];
+  b.y[g] = i1[src];
+  b.phi[g] = i2[src];
+}
+
+}  // namespace sc

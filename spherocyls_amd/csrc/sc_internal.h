@@ -1,0 +1,113 @@
+// Internal definitions shared by the device kernels (sc_device.hip) and the host
+// side of the C ABI (sc_api.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sc {
+
+// Particles a cell bucket holds.  Cells are at least 2.24 (Rmax + Amax) + pad
+// wide, so a bucket holds several times the mean occupancy of a dense packing.
+constexpr int kCap = 16;
+// Columns of cells per tile (upper bound; the autotuner picks the actual width).
+constexpr int kMaxTileCols = 32;
+// Passing candidates kept per particle in the LDS contact list.
+constexpr int kListLen = 48;
+// Particles processed concurrently per workgroup (LDS list slots).
+constexpr int kMaxGroups = 128;
+// Reduction slots: 9 stress/energy sums, then FIRE's P G V and the residual R2.
+constexpr int kSeE = 0, kSePxx = 1, kSePyy = 2, kSePxy = 3, kSeCPxx = 4, kSeCPyy = 5,
+              kSeCPxy = 6, kSeCPyx = 7, kSeNc = 8, kSumP = 9, kSumG = 10, kSumV = 11,
+              kSumR2 = 12;
+constexpr int kNSum = 13;
+constexpr int kNSe = 9;
+
+// The reference's truncated pi (src/cpp/spherocyl_box.cpp:12).
+constexpr double kRefPi = 3.14159265358979;
+
+enum Prec : int { kFP64 = 0, kFP32 = 1 };
+
+// Static description of one packing.
+struct PackInfo {
+  int n, poff;           // particles poff .. poff+n-1
+  int rows, cols, coff;  // cell grid, first global cell index
+  int toff, ntiles;      // tile range
+  int tile_cols;
+  double pad;            // cell padding
+  double dt, strain_rate, resize_rate;
+  double inv_2n;         // 1 / (2N), energy normalisation
+};
+
+// Mutable per-packing scalars, double-buffered: a kernel that advances a
+// packing reads scal[cur] and one designated thread writes scal[next].
+struct PackScal {
+  double Lx, Ly, gamma, gamma_total;
+  double fire_dt, fire_alpha, fire_npos;
+  double fire_c_keep, fire_c_force, fire_dt_new, fire_dt_old;
+  double frms;           // residual of the last reduced force pass
+  int step;              // sub-steps / iterations since the counter was reset
+  int fire_mix;          // 1: keep-and-mix velocities, 0: retreat and zero them
+  int converged;         // minimiser finished (FIRE: F_rms < ftol, GD: E <= Emin)
+  int prec;              // Prec of the pair math for this packing
+  int iters;             // minimiser iterations at convergence
+  int pad_;
+};
+
+struct FireParams {
+  double dt0, dt_max, n_min, f_inc, f_dec, alpha0, f_alpha, max_move, ftol, max_iter;
+  double switch_frms;    // mixed mode: FP32 pair math while frms >= switch_frms
+  double emin;           // gradient descent: stop when E <= emin
+};
+
+enum ErrBit : int { kErrCellOverflow = 1, kErrStale = 2, kErrList = 4, kErrNonFinite = 8 };
+
+// Per-particle arrays of one storage bank (sorted by cell within a packing).
+struct Bank {
+  double *x, *y, *phi, *R, *A, *I, *RC, *fx, *fy, *tau, *vx, *vy, *vpsi;
+  int* id;
+};
+
+enum TileMode : int { kModeForces = 0, kModePred = 1, kModeCorr = 2 };
+
+// Arguments of the tile force kernel.
+struct TileArgs {
+  const PackInfo* packs;
+  const PackScal* scal;   // scalars of the state being evaluated
+  PackScal* scal_next;    // advanced scalars (pred/corr/fire)
+  const int4* tiles;      // (packing, row, first column, columns)
+  const int* cnt;         // bucket counts / slots of the staged state's cells
+  const int* slot;
+  int* cnt_fill;          // buckets filled by the corrector
+  int* slot_fill;
+  int* cnt_clear;         // bucket counts zeroed by this launch (or null)
+  int ncells_total;
+  Bank b;                 // current bank (own state, static particle data)
+  const double* sx;       // staged positions (current or predicted)
+  const double* sy;
+  const double* sphi;
+  double *px, *py, *pphi; // predicted positions (pred writes, corr reads)
+  double *v0x, *v0y, *v0t;
+  double* tile_part;      // [tiles][kNSum]
+  double* pack_sum;       // [npack][kNSum] latest reduced sums
+  unsigned* pack_done;    // [npack] last-block tickets
+  double* se_ring;        // recorded stress/energy [rec][npack][kNSe]
+  int* err;
+  FireParams fire;
+  double resize_scale;    // (1 - rate)^dt of the resize step
+  int se_every;           // pred/forces: record when step % se_every == 0 (0: never)
+  int se_always;          // forces: reduce stress/energy unconditionally
+  int do_fire;            // forces: FIRE sums + decisions in the last block
+  int do_gd;              // forces: gradient-descent stop test in the last block
+  int do_resid;           // forces: reduce the residual sum(F^2) for F_rms
+  int write_forces;
+  int resize;             // pred/corr: resize dynamics instead of shear
+  int shrink;
+  int corrected_image;
+  int npack;
+  int stage_cap;          // LDS staging capacity (entries)
+  int list_groups;        // LDS contact-list slots
+};
+
+}  // namespace sc

@@ -1,0 +1,269 @@
+"""ctypes binding of the C ABI in include/spherocyls.h (libspherocyls.so).
+
+The library is loaded from the package directory (built in-tree by
+``python -m spherocyls_amd.build``).  There is no fallback: if the library is
+missing, or no GPU is visible when a GPU call is made, these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libspherocyls.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "spherocyls.h")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+def _constants():
+    out = {}
+    with open(HEADER) as f:
+        for m in re.finditer(r"#define\s+(SC_\w+)\s+(-?\d+)", f.read()):
+            out[m.group(1)] = int(m.group(2))
+    return out
+
+
+CONST = _constants()
+globals().update(CONST)
+
+
+class SpherocylsError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        name = next((k for k, v in CONST.items() if k.startswith("SC_ERR") and v == status), str(status))
+        super().__init__(f"{name}: {msg}" if msg else name)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built (python -m spherocyls_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    sysp = C.c_void_p
+    sig = {
+        "sc_version": ([], C.c_int),
+        "sc_status_string": ([C.c_int], C.c_char_p),
+        "sc_device_count": ([], C.c_int),
+        "sc_system_create": ([C.POINTER(sysp), C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                              _dp, _dp], C.c_int),
+        "sc_system_destroy": ([sysp], C.c_int),
+        "sc_last_error": ([sysp], C.c_char_p),
+        "sc_num_particles": ([sysp, _ip, _ip], C.c_int),
+        "sc_set_dynamics": ([sysp, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+        "sc_set_gamma": ([sysp, C.c_int, C.c_double, C.c_double], C.c_int),
+        "sc_set_precision": ([sysp, C.c_int], C.c_int),
+        "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
+        "sc_get_tiling": ([sysp, _ip, _ip], C.c_int),
+        "sc_set_positions": ([sysp, _dp, _dp, _dp], C.c_int),
+        "sc_get_positions": ([sysp, _dp, _dp, _dp], C.c_int),
+        "sc_get_forces": ([sysp, _dp, _dp, _dp], C.c_int),
+        "sc_get_box": ([sysp, C.c_int, _dp], C.c_int),
+        "sc_calc_forces": ([sysp], C.c_int),
+        "sc_calc_se": ([sysp], C.c_int),
+        "sc_strain_steps": ([sysp, C.c_int, C.c_int, _dp], C.c_int),
+        "sc_resize_steps": ([sysp, C.c_int, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+        "sc_gd_steps": ([sysp, C.c_int, C.c_double, _ip], C.c_int),
+        "sc_affine_resize": ([sysp, _dp], C.c_int),
+        "sc_fire_minimize": ([sysp, _dp, C.c_double, _ip, _dp], C.c_int),
+        "sc_get_neighbors": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
+        "sc_get_contacts": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
+        "sc_timing_enable": ([sysp, C.c_int], C.c_int),
+        "sc_timing_get": ([sysp, C.c_int, _dp, C.POINTER(C.c_longlong)], C.c_int),
+        "sc_timing_reset": ([sysp], C.c_int),
+        "sc_synchronize": ([sysp], C.c_int),
+        "sc_random_packing": ([C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_ulonglong, C.c_int,
+                               _dp, _dp, _dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def random_packing(L, A, R, seed, gamma=0.0, max_tries=100000):
+    """Seeded non-crossing random placement in sheared coordinates at strain
+    gamma (host code, no GPU needed)."""
+    lib = load()
+    A, R = _f64(A), _f64(R)
+    n = A.shape[0]
+    x, y, phi = np.empty(n), np.empty(n), np.empty(n)
+    st = lib.sc_random_packing(n, float(L), float(gamma), _p(A), _p(R), C.c_ulonglong(seed),
+                               max_tries, _p(x), _p(y), _p(phi))
+    if st != 0:
+        raise SpherocylsError(st, "random placement")
+    return x, y, phi
+
+
+FIRE_DEFAULTS = dict(dt0=0.05, dt_max=0.5, n_min=5, f_inc=1.1, f_dec=0.5, alpha0=0.1,
+                     f_alpha=0.99, max_move=0.02, ftol=1e-10, max_iter=200000)
+
+
+def fire_params(**kw):
+    p = dict(FIRE_DEFAULTS)
+    p.update(kw)
+    return np.array([p[k] for k in ("dt0", "dt_max", "n_min", "f_inc", "f_dec", "alpha0",
+                                     "f_alpha", "max_move", "ftol", "max_iter")], dtype=np.float64)
+
+
+BOX_FIELDS = ("Lx", "Ly", "gamma", "gamma_total", "E", "Pxx", "Pyy", "Pxy", "cPxx", "cPyy",
+              "cPxy", "cPyx", "ncontacts", "frms", "packing", "rows", "cols", "pad", "iters")
+SE_FIELDS = ("E", "Pxx", "Pyy", "Pxy", "cPxx", "cPyy", "cPxy", "cPyx", "ncontacts")
+
+
+class System:
+    """One or more packings on one GPU (thin wrapper over sc_system)."""
+
+    def __init__(self, packings, pads=0.1, device=0):
+        lib = load()
+        self.lib = lib
+        if not isinstance(packings, (list, tuple)):
+            packings = [packings]
+        self.npack = len(packings)
+        self.sizes = [int(p.x.shape[0]) for p in packings]
+        self.offsets = np.zeros(self.npack + 1, dtype=np.int32)
+        self.offsets[1:] = np.cumsum(self.sizes)
+        cat = lambda k: _f64(np.concatenate([getattr(p, k) for p in packings]))
+        x, y, phi, A, R = cat("x"), cat("y"), cat("phi"), cat("A"), cat("R")
+        L = _f64([p.L for p in packings])
+        pad = _f64(np.broadcast_to(np.asarray(pads, dtype=np.float64), (self.npack,)))
+        self.ntot = int(self.offsets[-1])
+        h = C.c_void_p()
+        st = lib.sc_system_create(C.byref(h), device, self.npack, self.offsets.ctypes.data_as(_ip),
+                                  _p(L), _p(x), _p(y), _p(phi), _p(A), _p(R), _p(pad))
+        if st != 0:
+            raise SpherocylsError(st, "sc_system_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sc_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, st):
+        if st != 0:
+            msg = self.lib.sc_last_error(self.h)
+            raise SpherocylsError(st, msg.decode() if msg else "")
+        return st
+
+    def set_dynamics(self, strain_rate=0.0, resize_rate=0.0, step=0.05, k=-1):
+        self._ck(self.lib.sc_set_dynamics(self.h, k, strain_rate, resize_rate, step))
+
+    def set_gamma(self, gamma, gamma_total=0.0, k=-1):
+        self._ck(self.lib.sc_set_gamma(self.h, k, gamma, gamma_total))
+
+    def set_precision(self, mode):
+        self._ck(self.lib.sc_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)))
+
+    def set_tiling(self, tile_cols, block):
+        self._ck(self.lib.sc_set_tiling(self.h, tile_cols, block))
+
+    def tiling(self):
+        a, b = C.c_int(), C.c_int()
+        self._ck(self.lib.sc_get_tiling(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_positions(self, x, y, phi):
+        x, y, phi = _f64(x), _f64(y), _f64(phi)
+        self._ck(self.lib.sc_set_positions(self.h, _p(x), _p(y), _p(phi)))
+
+    def positions(self):
+        x, y, phi = np.empty(self.ntot), np.empty(self.ntot), np.empty(self.ntot)
+        self._ck(self.lib.sc_get_positions(self.h, _p(x), _p(y), _p(phi)))
+        return x, y, phi
+
+    def forces(self):
+        x, y, t = np.empty(self.ntot), np.empty(self.ntot), np.empty(self.ntot)
+        self._ck(self.lib.sc_get_forces(self.h, _p(x), _p(y), _p(t)))
+        return x, y, t
+
+    def box(self, k=0):
+        out = np.zeros(CONST["SC_BOX_NFIELDS"])
+        self._ck(self.lib.sc_get_box(self.h, k, _p(out)))
+        return dict(zip(BOX_FIELDS, out.tolist()))
+
+    def calc_forces(self):
+        self._ck(self.lib.sc_calc_forces(self.h))
+
+    def calc_se(self):
+        self._ck(self.lib.sc_calc_se(self.h))
+
+    def strain_steps(self, nsteps, se_every=0):
+        nrec = nsteps // se_every if se_every > 0 else 0
+        out = np.zeros(max(1, nrec * self.npack * 9))
+        self._ck(self.lib.sc_strain_steps(self.h, nsteps, se_every, _p(out)))
+        return out[:nrec * self.npack * 9].reshape(nrec, self.npack, 9)
+
+    def resize_steps(self, nsteps, shrink=True, corrected_image=False, se_every=0):
+        nrec = nsteps // se_every if se_every > 0 else 0
+        out = np.zeros(max(1, nrec * self.npack * 9))
+        self._ck(self.lib.sc_resize_steps(self.h, nsteps, int(shrink), int(corrected_image),
+                                          se_every, _p(out)))
+        return out[:nrec * self.npack * 9].reshape(nrec, self.npack, 9)
+
+    def gd_steps(self, nmax, emin=0.0):
+        done = C.c_int()
+        self._ck(self.lib.sc_gd_steps(self.h, nmax, emin, C.byref(done)))
+        return done.value
+
+    def affine_resize(self, scale):
+        s = _f64(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.npack,)))
+        self._ck(self.lib.sc_affine_resize(self.h, _p(s)))
+
+    def fire(self, params=None, switch_frms=0.0, check=True):
+        p = fire_params() if params is None else _f64(params)
+        it = np.zeros(self.npack, dtype=np.int32)
+        fr = np.zeros(self.npack)
+        st = self.lib.sc_fire_minimize(self.h, _p(p), switch_frms, it.ctypes.data_as(_ip), _p(fr))
+        if check:
+            self._ck(st)
+        return it, fr, st
+
+    def _pairs(self, fn, k):
+        cnt = C.c_int()
+        self._ck(fn(self.h, k, None, None, 0, C.byref(cnt)))
+        n = cnt.value
+        p = np.empty(max(n, 1), dtype=np.int32)
+        q = np.empty(max(n, 1), dtype=np.int32)
+        self._ck(fn(self.h, k, p.ctypes.data_as(_ip), q.ctypes.data_as(_ip), n, C.byref(cnt)))
+        return p[:n], q[:n]
+
+    def neighbors(self, k=0):
+        return self._pairs(self.lib.sc_get_neighbors, k)
+
+    def contacts(self, k=0):
+        return self._pairs(self.lib.sc_get_contacts, k)
+
+    def timing(self, enable=True):
+        self._ck(self.lib.sc_timing_enable(self.h, int(enable)))
+
+    def kernel_time(self, kernel):
+        ms, n = C.c_double(), C.c_longlong()
+        self._ck(self.lib.sc_timing_get(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_reset(self):
+        self._ck(self.lib.sc_timing_reset(self.h))
+
+    def synchronize(self):
+        self._ck(self.lib.sc_synchronize(self.h))

@@ -1,0 +1,297 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and the CPU oracle on the same seeded inputs.
+
+Tolerances (FP64 mode):
+  forces / torques  |F - F_ref| <= 1e-10 * sum_j |f_ij|  (per particle)
+  energy, stress    relative 1e-10 (absolute 1e-10 * scale near zero)
+  contact and neighbour sets: exact
+Mixed mode is reported against the FP64 oracle per quantity.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden, golden_names, wrap_diff
+from spherocyls_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def system_from_golden(g):
+    pk = configs.Packing(float(g["L"][0]), g["x"], g["y"], g["phi"], g["A"], g["R"], 0.0)
+    s = native.System(pk, pads=float(g["pad"][0]))
+    s.set_gamma(float(g["gamma"][0]), 0.0)
+    return s, pk
+
+
+def oracle_at(pk, gamma, pad):
+    b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, pad)
+    b.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    b.find_neighbors()
+    b.calc_se()
+    return b
+
+
+def assert_forces(got, ref, scale, rel=TOL):
+    for name, a, b, sc in zip(("fx", "fy", "tau"), got, ref, scale):
+        err = np.abs(a - b)
+        bound = rel * sc + 1e-15
+        bad = np.nonzero(err > bound)[0]
+        assert bad.size == 0, f"{name}: {bad.size} particles off, worst {np.max(err / (sc + 1e-300)):.3g}"
+
+
+def pairset(p, q):
+    return set(zip(np.minimum(p, q).tolist(), np.maximum(p, q).tolist()))
+
+
+@pytest.mark.parametrize("name", golden_names("static_"))
+def test_static_forces_stress_sets(name):
+    g = golden(name)
+    s, pk = system_from_golden(g)
+    try:
+        s.calc_se()
+        fx, fy, tau = s.forces()
+        ob = oracle_at(pk, float(g["gamma"][0]), float(g["pad"][0]))
+        sc = ob.force_scale()
+        assert_forces((fx, fy, tau), (g["fx"], g["fy"], g["tau"]), sc)
+        bx = s.box()
+        ref = dict(zip(("E", "Pxx", "Pyy", "Pxy"), g["se"]))
+        o = ob.scalars()
+        for k in ("E", "Pxx", "Pyy", "Pxy"):
+            assert bx[k] == pytest.approx(ref[k], rel=TOL, abs=TOL * abs(o["E"]) + 1e-16), k
+        for k in ("cPxx", "cPyy", "cPxy", "cPyx"):
+            assert bx[k] == pytest.approx(o[k], rel=TOL, abs=TOL * abs(o["E"]) + 1e-16), k
+        assert bx["ncontacts"] == o["ncontacts"]
+        assert (bx["rows"], bx["cols"]) == (g["rows"][0], g["cols"][0])
+        # neighbour set under the reference's criterion, and the contact set
+        assert pairset(*s.neighbors()) == pairset(g["nbr_p"], g["nbr_q"])
+        assert pairset(*s.contacts()) == pairset(*ob.contacts())
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("name", golden_names("heun_"))
+def test_heun_trajectory(name):
+    g = golden(name)
+    s, pk = system_from_golden(g)
+    L = pk.L
+    try:
+        s.set_dynamics(strain_rate=float(g["strain_rate"][0]), step=float(g["dt"][0]))
+        s.strain_steps(1)
+        x, y, phi = s.positions()
+        assert np.max(np.abs(wrap_diff(x, g["x1"], L))) < 1e-12
+        assert np.max(np.abs(wrap_diff(y, g["y1"], L))) < 1e-12
+        assert np.max(np.abs(phi - g["phi1"])) < 1e-12
+        assert s.box()["gamma"] == pytest.approx(g["gamma1"][0], abs=1e-15)
+        n, every = int(g["nsteps"][0]), int(g["se_every"][0])
+        se = s.strain_steps(n, every)[:, 0, :4]
+        np.testing.assert_allclose(se, g["se"], rtol=1e-9, atol=1e-13)
+        x, y, phi = s.positions()
+        assert np.max(np.abs(wrap_diff(x, g["xN"], L))) < 1e-10
+        assert np.max(np.abs(wrap_diff(y, g["yN"], L))) < 1e-10
+        assert np.max(np.abs(phi - g["phiN"])) < 1e-10
+        fx, fy, tau = s.forces()
+        ob = oracle.OracleBox(L, g["xN"], g["yN"], g["phiN"], g["A"], g["R"], float(g["pad"][0]))
+        ob.set_params(float(g["gammaN"][0]), 0.0, 0.0, 0.0, 0.05)
+        ob.find_neighbors()
+        sc = ob.force_scale()
+        assert_forces((fx, fy, tau), (g["fxN"], g["fyN"], g["tauN"]), sc, rel=1e-8)
+        bx = s.box()
+        assert bx["gamma"] == pytest.approx(g["gammaN"][0], abs=1e-14)
+        assert bx["gamma_total"] == pytest.approx(g["gamma_totalN"][0], abs=1e-14)
+    finally:
+        s.close()
+
+
+def test_resize_steps():
+    g = golden("resize128")
+    s, pk = system_from_golden(g)
+    try:
+        s.set_dynamics(resize_rate=float(g["rate"][0]), step=float(g["dt"][0]))
+        s.resize_steps(int(g["nsteps"][0]), shrink=True)
+        bx = s.box()
+        assert bx["Lx"] == pytest.approx(g["LN"][0], rel=1e-15)
+        x, y, phi = s.positions()
+        L = g["LN"][0]
+        assert np.max(np.abs(wrap_diff(x, g["xN"], L))) < 1e-11
+        assert np.max(np.abs(wrap_diff(y, g["yN"], L))) < 1e-11
+        assert np.max(np.abs(phi - g["phiN"])) < 1e-11
+        fx, fy, tau = s.forces()
+        np.testing.assert_allclose(fx, g["fxN"], atol=1e-10)
+        np.testing.assert_allclose(tau, g["tauN"], atol=1e-10)
+    finally:
+        s.close()
+
+
+def test_gradient_descent():
+    g = golden("gd128")
+    s, pk = system_from_golden(g)
+    try:
+        s.set_dynamics(step=float(g["dt"][0]))
+        n = int(g["nsteps"][0])
+        done = s.gd_steps(n, emin=-1.0)
+        assert done == n
+        x, y, phi = s.positions()
+        L = pk.L
+        assert np.max(np.abs(wrap_diff(x, g["xN"], L))) < 1e-11
+        assert np.max(np.abs(phi - g["phiN"])) < 1e-11
+        assert s.box()["E"] == pytest.approx(g["se"][0], rel=1e-9)
+    finally:
+        s.close()
+
+
+FIRE = dict(dt0=0.05, dt_max=0.5, n_min=5, f_inc=1.1, f_dec=0.5, alpha0=0.1, f_alpha=0.99,
+            max_move=0.02, ftol=1e-10)
+
+
+def test_fire_first_iterations_match_oracle():
+    R, A = configs.bidisperse(400)
+    pk = configs.make_packing(R, A, 0.80, 51)
+    k = 40
+    params = native.fire_params(**FIRE, max_iter=k)
+    s = native.System(pk, pads=0.1)
+    try:
+        it, fr, st = s.fire(params, check=False)
+        assert st == native.SC_ERR_NOT_CONVERGED and it[0] == k
+        x, y, phi = s.positions()
+    finally:
+        s.close()
+    ob = oracle.OracleBox.from_packing(pk, 0.1)
+    ob.fire(params)
+    p = ob.particles()
+    assert np.max(np.abs(wrap_diff(x, p["x"], pk.L))) < 1e-9
+    assert np.max(np.abs(phi - p["phi"])) < 1e-9
+
+
+def test_fire_converges_and_oracle_agrees():
+    R, A = configs.bidisperse(2048)
+    pk = configs.make_packing(R, A, 0.78, 53)
+    s = native.System(pk, pads=0.1)
+    try:
+        it, fr, st = s.fire(native.fire_params(**FIRE, max_iter=20000))
+        assert st == 0 and fr[0] < 1e-10
+        x, y, phi = s.positions()
+        bx = s.box()
+    finally:
+        s.close()
+    # the oracle is the judge of the residual at the GPU's final state
+    ob = oracle.OracleBox(pk.L, x, y, phi, pk.A, pk.R, 0.1)
+    assert ob.frms() < 1e-10
+    ob.calc_se()
+    assert bx["E"] == pytest.approx(ob.scalars()["E"], rel=1e-8, abs=1e-25)
+
+
+def test_mixed_precision_error_report():
+    R, A = configs.bidisperse(4096)
+    pk = configs.make_packing(R, A, 0.85, 55)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.calc_se()
+        f64 = s.forces()
+        b64 = s.box()
+        s.set_precision("mixed")
+        s.calc_se()
+        f32 = s.forces()
+        b32 = s.box()
+    finally:
+        s.close()
+    ob = oracle_at(pk, 0.0, 0.1)
+    sc = ob.force_scale()
+    report = {}
+    for name, a, b, c in zip(("fx", "fy", "tau"), f32, f64, sc):
+        report[name] = float(np.max(np.abs(a - b) / (c + 1e-300)))
+    for k in ("E", "Pxx", "Pyy", "Pxy"):
+        report[k] = abs(b32[k] - b64[k]) / abs(b64[k])
+    print("mixed-vs-FP64 relative error:", report)
+    assert max(report.values()) < 1e-4
+
+
+def test_batch_equals_single_packings():
+    packs = configs.ensemble(count=6, seed=5, n_max=1500)
+    s = native.System(packs, pads=0.1)
+    try:
+        s.set_gamma(0.0)
+        s.calc_se()
+        fb = s.forces()
+        eb = [s.box(k)["E"] for k in range(len(packs))]
+    finally:
+        s.close()
+    off = np.concatenate([[0], np.cumsum([p.n for p in packs])])
+    for k, pk in enumerate(packs):
+        s1 = native.System(pk, pads=0.1)
+        try:
+            s1.calc_se()
+            f1 = s1.forces()
+            e1 = s1.box()["E"]
+        finally:
+            s1.close()
+        for a, b in zip(fb, f1):
+            np.testing.assert_array_equal(a[off[k]:off[k + 1]], b)
+        assert eb[k] == pytest.approx(e1, rel=1e-13)
+
+
+def test_tiling_shapes_give_identical_forces():
+    R, A = configs.bidisperse(3000)
+    pk = configs.make_packing(R, A, 0.85, 57, gamma=0.2)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_gamma(0.2)
+        ref = None
+        for tc, blk in ((1, 64), (4, 128), (8, 128), (12, 256), (24, 256)):
+            s.set_tiling(tc, blk)
+            s.calc_forces()
+            f = s.forces()
+            if ref is None:
+                ref = f
+            else:
+                for a, b in zip(f, ref):
+                    np.testing.assert_array_equal(a, b)
+    finally:
+        s.close()
+
+
+def test_large_system_force_parity():
+    pk = configs.shear_large(1)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.calc_se()
+        f = s.forces()
+        bx = s.box()
+    finally:
+        s.close()
+    ob = oracle_at(pk, 0.0, 0.1)
+    p = ob.particles()
+    sc = ob.force_scale()
+    assert_forces(f, (p["fx"], p["fy"], p["tau"]), sc)
+    o = ob.scalars()
+    assert bx["ncontacts"] == o["ncontacts"]
+    for k in ("E", "Pxx", "Pyy", "Pxy"):
+        assert bx[k] == pytest.approx(o[k], rel=TOL), k
+    # Newton's third law over the whole box
+    assert abs(f[0].sum()) < 1e-12 * sc[0].sum()
+
+
+def test_error_paths():
+    R, A = configs.bidisperse(256)
+    pk = configs.make_packing(R, A, 0.8, 59)
+    s = native.System(pk, pads=0.1)
+    try:
+        with pytest.raises(native.SpherocylsError) as e:
+            s.set_tiling(0, 128)
+        assert e.value.status == native.SC_ERR_ARG
+        with pytest.raises(native.SpherocylsError) as e:
+            s.fire(native.fire_params(max_move=0.06))  # max_move must stay below pad/2
+        assert e.value.status == native.SC_ERR_ARG
+        x, y, phi = s.positions()
+        x[3] = np.nan
+        with pytest.raises(native.SpherocylsError):
+            s.set_positions(x, y, phi)
+        # a predictor step larger than pad/2 is reported, not silently wrong
+        s.set_dynamics(strain_rate=0.0, step=50.0)
+        with pytest.raises(native.SpherocylsError) as e:
+            s.strain_steps(1)
+        assert e.value.status == native.SC_ERR_STALE_CELLS
+    finally:
+        s.close()

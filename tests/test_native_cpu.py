@@ -1,0 +1,96 @@
+"""CPU-only checks of the native library: it loads, exports every entry point
+include/spherocyls.h declares, and its host-only parts (status strings, the
+seeded placement generator) behave.  No GPU compute is called here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from spherocyls_amd import configs, native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "spherocyls.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(native.LIB_PATH)
+    out = subprocess.run(["nm", "-D", "--defined-only", native.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (sc_\w+)$", out, flags=re.M))
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing, missing
+    assert len(declared_functions()) >= 30
+
+
+def test_library_loads_and_describes_status():
+    lib = native.load()
+    assert lib.sc_version() >= 10000
+    assert lib.sc_status_string(native.SC_OK) == b"ok"
+    assert lib.sc_status_string(native.SC_ERR_CELL_OVERFLOW) == b"cell bucket overflow"
+    assert lib.sc_device_count() >= 0
+
+
+def test_create_without_gpu_is_an_error_not_a_fallback():
+    lib = native.load()
+    if lib.sc_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    pk = configs.shear_small(1)
+    with pytest.raises(native.SpherocylsError) as e:
+        native.System(pk)
+    assert e.value.status == native.SC_ERR_NO_DEVICE
+
+
+def test_random_packing_seeded_and_crossing_free():
+    R, A = configs.bidisperse(300, aspect=3.0)
+    L = configs.box_side(R, A, 0.85)
+    a = native.random_packing(L, A, R, 7, gamma=0.3)
+    b = native.random_packing(L, A, R, 7, gamma=0.3)
+    c = native.random_packing(L, A, R, 8, gamma=0.3)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    assert not np.array_equal(a[0], c[0])
+    x, y, phi = a
+    assert np.all((x >= 0) & (x < L) & (y >= 0) & (y < L) & (phi >= 0) & (phi < np.pi))
+    # no crossing pair: every contact force is finite at the strain it was made for
+    box = oracle.OracleBox(L, x, y, phi, A, R, 0.1)
+    box.set_params(0.3, 0.0, 0.0, 0.0, 0.05)
+    box.find_neighbors()
+    box.calc_se()
+    p = box.particles()
+    assert np.all(np.isfinite(p["fx"])) and np.all(np.isfinite(p["tau"]))
+    assert box.scalars()["ncontacts"] > 0  # overlaps are allowed, crossings are not
+
+
+def test_random_packing_rejects_bad_arguments():
+    with pytest.raises(native.SpherocylsError):
+        native.random_packing(10.0, np.array([0.5]), np.array([-1.0]), 1)
+    with pytest.raises(native.SpherocylsError):
+        native.random_packing(10.0, np.array([0.5]), np.array([0.5]), 1, gamma=0.7)
+    # an impossible density gives up instead of looping forever
+    R, A = configs.bidisperse(400)
+    with pytest.raises(native.SpherocylsError) as e:
+        native.random_packing(5.0, A, R, 1, max_tries=50)
+    assert e.value.status == native.SC_ERR_PLACEMENT
+
+
+def test_workload_generators():
+    pk = configs.shear_small(1)
+    assert pk.n == 512
+    np.testing.assert_array_equal(pk.R[:4], [0.5, 0.7, 0.5, 0.7])
+    np.testing.assert_array_equal(pk.A, pk.R)
+    area = np.sum((configs.REF_PI * pk.R + 4 * pk.A) * pk.R)
+    assert area / pk.L ** 2 == pytest.approx(0.85, rel=1e-12)
+    ens = configs.ensemble(count=24, seed=3)
+    ns = [p.n for p in ens]
+    assert min(ns) >= 64 and max(ns) <= 4096
+    aspects = {round(float((p.A[0] + p.R[0]) / p.R[0]), 6) for p in ens}
+    assert aspects <= {1.2, 2.0, 4.0}
+    assert all(0.80 <= p.packing_fraction <= 0.86 for p in ens)
