@@ -32,7 +32,6 @@ extern "C" {
 #define SC_ERR_HIP 2            /* HIP runtime failure (message in sc_last_error) */
 #define SC_ERR_NO_DEVICE 3      /* no usable GPU */
 #define SC_ERR_CELL_OVERFLOW 4  /* more particles in one cell than the bucket holds */
-#define SC_ERR_STALE_CELLS 5    /* a predictor step moved a particle more than pad/2 */
 #define SC_ERR_PLACEMENT 6      /* random placement gave up */
 #define SC_ERR_LIST_OVERFLOW 7  /* more than the per-particle candidate list holds */
 #define SC_ERR_NOT_CONVERGED 8  /* minimiser hit its iteration limit */

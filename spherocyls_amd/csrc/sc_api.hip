@@ -81,6 +81,7 @@ struct sc_system {
   std::vector<PackScal> scal;  // host mirror of d_scal[cur_scal] (valid after sync_scal)
   std::vector<double> Rmax, Amax, packing, filled;
   int tile_cols = 8, block = 128;
+  int lds_max = 64 * 1024;  // LDS bytes a workgroup may use on this device
   int prec = SC_PREC_FP64;
   int ncells = 0, ntiles = 0;
   std::vector<int4> tiles;
@@ -287,9 +288,6 @@ int check_errors(sc_system* s) {
   if (s->timing) resolve_events(s);
   if (!e) return SC_OK;
   HIPCK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
-  // report the root cause first: a stale predictor step can overfill cells
-  if (e & kErrStale)
-    return fail(s, SC_ERR_STALE_CELLS, "a predictor step moved a particle by more than pad/2");
   if (e & kErrCellOverflow)
     return fail(s, SC_ERR_CELL_OVERFLOW, "a cell holds more particles than its bucket (16)");
   if (e & kErrList)
@@ -347,7 +345,7 @@ TileArgs base_args(sc_system* s) {
   a.se_ring = s->d_se_ring;
   a.err = s->d_err;
   a.npack = s->npack;
-  a.list_groups = std::min(s->block, kMaxGroups);
+  a.list_groups = s->block / kLanes;
   a.stage_cap = 3 * (s->tile_cols + 2) * kCap;
   return a;
 }
@@ -355,13 +353,11 @@ TileArgs base_args(sc_system* s) {
 template <int MODE>
 void launch_tile(sc_system* s, const TileArgs& a, int kernel_id) {
   size_t lds = tile_lds_bytes(a.stage_cap, a.list_groups, s->block);
-  static bool lds_raised = false;  // wide tiles need more than the default 64 KiB
-  if (!lds_raised) {
+  if (lds > (size_t)s->lds_max) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
+  if (lds > 64 * 1024) {  // wide tiles need more than the default dynamic LDS
     HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    lds_raised = true;
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  if (lds > 160 * 1024) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
   KernelTimer t(s, kernel_id);
   hipLaunchKernelGGL(k_tile<MODE>, dim3(s->ntiles), dim3(s->block), lds, s->stream, a);
 }
@@ -413,7 +409,6 @@ const char* sc_status_string(int status) {
     case SC_ERR_HIP: return "HIP runtime error";
     case SC_ERR_NO_DEVICE: return "no GPU device";
     case SC_ERR_CELL_OVERFLOW: return "cell bucket overflow";
-    case SC_ERR_STALE_CELLS: return "predictor moved a particle more than pad/2";
     case SC_ERR_PLACEMENT: return "random placement failed";
     case SC_ERR_LIST_OVERFLOW: return "candidate list overflow";
     case SC_ERR_NOT_CONVERGED: return "minimiser did not converge";
@@ -459,6 +454,7 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
   try {
     s->device = device;
     HIPCK(hipSetDevice(device));
+    HIPCK(hipDeviceGetAttribute(&s->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->npack = npack;
     s->ntot = ntot;
@@ -642,6 +638,9 @@ int sc_set_precision(sc_system* s, int mode) {
 int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
   if (!s || tile_cols < 1 || tile_cols > kMaxTileCols) return SC_ERR_ARG;
   if (block_threads != 64 && block_threads != 128 && block_threads != 256) return SC_ERR_ARG;
+  if (tile_lds_bytes(3 * (tile_cols + 2) * kCap, block_threads / kLanes, block_threads) >
+      (size_t)s->lds_max)
+    return fail(s, SC_ERR_ARG, "tile too wide for the workgroup's LDS");
   API_BEGIN
   HIPCK(hipStreamSynchronize(s->stream));
   s->tile_cols = tile_cols;
@@ -755,30 +754,38 @@ int sc_calc_se(sc_system* s) {
   API_END
 }
 
-// One Heun sub-step (shear or resize) on every packing.
+// One Heun sub-step (shear or resize) on every packing.  Bucket buffers
+// rotate: c holds the cells of x, the predictor bins x~ into c+1 and zeroes
+// c+2, the corrector reads c+1, bins x' into c+2 and zeroes c (ready to take
+// the next step's predicted cells).
 static void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every,
                          double scale) {
-  int b1 = (s->cur_buf + 1) % 3;
+  const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
   TileArgs a = base_args(s);
   a.resize = resize;
   a.shrink = shrink;
   a.corrected_image = corrected;
   a.resize_scale = scale;
   a.se_every = se_every;
-  a.cnt_clear = s->d_cnt[b1];
+  a.cnt_fill = s->d_cnt[c1];
+  a.slot_fill = s->d_slot[c1];
+  a.cnt_clear = s->d_cnt[c2];
   launch_tile<kModePred>(s, a, SC_KERNEL_PREDICT);
   TileArgs c = base_args(s);
   c.resize = resize;
   c.shrink = shrink;
   c.corrected_image = corrected;
   c.resize_scale = scale;
+  c.cnt = s->d_cnt[c1];
+  c.slot = s->d_slot[c1];
   c.sx = s->d_px;
   c.sy = s->d_py;
   c.sphi = s->d_pphi;
-  c.cnt_fill = s->d_cnt[b1];
-  c.slot_fill = s->d_slot[b1];
+  c.cnt_fill = s->d_cnt[c2];
+  c.slot_fill = s->d_slot[c2];
+  c.cnt_clear = s->d_cnt[c0];
   launch_tile<kModeCorr>(s, c, SC_KERNEL_CORRECT);
-  s->cur_buf = b1;
+  s->cur_buf = c2;
   s->cur_scal = 1 - s->cur_scal;
 }
 

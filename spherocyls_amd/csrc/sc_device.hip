@@ -451,14 +451,13 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
   }
   const bool fp64 = S.prec == kFP64;
 
-  // lanes per particle
-  int LN = 1;
-  while (LN < 8 && LN * 2 * n_own <= nthr) LN *= 2;
-  while (nthr / LN > a.list_groups) LN *= 2;
+  // kLanes lanes per particle, fixed so that every per-particle sum has the same
+  // order whatever the tile shape or block size
+  constexpr int LN = kLanes;
   const int G = nthr / LN;
   const int grp = tid / LN, lane = tid - grp * LN;
   const int grp_shift = (tid & 63) - lane;
-  const unsigned long long gmask = LN >= 64 ? ~0ull : ((1ull << LN) - 1ull);
+  const unsigned long long gmask = (1ull << LN) - 1ull;
   unsigned short* my_list = L.list + grp * kListLen;
 
   double acc[kNSum];
@@ -623,7 +622,12 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
       a.v0x[g] = v0x;
       a.v0y[g] = v0y;
       a.v0t[g] = v0t;
-      if (fabs(dt * v0x) >= 0.5 * P.pad || fabs(dt * v0y) >= 0.5 * P.pad) atomicOr(a.err, kErrStale);
+      // the corrector's force pass runs on the cells of the predicted state
+      double Lbx = a.resize ? L_new_x : Lx, Lby = a.resize ? L_new_y : Ly;
+      double wx = qx, wy = qy;
+      bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
+      if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
+      bin_into(g, wx, wy, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
       if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
     } else {  // kModeCorr
       double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];

@@ -15,8 +15,10 @@ constexpr int kCap = 16;
 constexpr int kMaxTileCols = 32;
 // Passing candidates kept per particle in the LDS contact list.
 constexpr int kListLen = 48;
+// Lanes that cooperate on one particle (fixed: it sets the summation order).
+constexpr int kLanes = 8;
 // Particles processed concurrently per workgroup (LDS list slots).
-constexpr int kMaxGroups = 128;
+constexpr int kMaxGroups = 256 / kLanes;
 // Reduction slots: 9 stress/energy sums, then FIRE's P G V and the residual R2.
 constexpr int kSeE = 0, kSePxx = 1, kSePyy = 2, kSePxy = 3, kSeCPxx = 4, kSeCPyy = 5,
               kSeCPxy = 6, kSeCPyx = 7, kSeNc = 8, kSumP = 9, kSumG = 10, kSumV = 11,
@@ -61,7 +63,7 @@ struct FireParams {
   double emin;           // gradient descent: stop when E <= emin
 };
 
-enum ErrBit : int { kErrCellOverflow = 1, kErrStale = 2, kErrList = 4, kErrNonFinite = 8 };
+enum ErrBit : int { kErrCellOverflow = 1, kErrList = 4, kErrNonFinite = 8 };
 
 // Per-particle arrays of one storage bank (sorted by cell within a packing).
 struct Bank {
@@ -79,7 +81,7 @@ struct TileArgs {
   const int4* tiles;      // (packing, row, first column, columns)
   const int* cnt;         // bucket counts / slots of the staged state's cells
   const int* slot;
-  int* cnt_fill;          // buckets filled by the corrector
+  int* cnt_fill;          // buckets filled with the new positions (pred: x~, corr: x')
   int* slot_fill;
   int* cnt_clear;         // bucket counts zeroed by this launch (or null)
   int ncells_total;
