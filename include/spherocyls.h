@@ -96,9 +96,27 @@ int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate,
 /* set_gamma / set_total_gamma  src/cpp/spherocyl_box.h:118-119 */
 int sc_set_gamma(sc_system* s, int k, double gamma, double gamma_total);
 int sc_set_precision(sc_system* s, int mode);
-/* Tile shape: columns per tile and threads per workgroup (the autotuner's knobs). */
+/* Force-pass implementation (no reference counterpart; both give the same
+ * forces to rounding):
+ *   SC_PATH_LIST  one thread per particle over a per-particle partner list kept
+ *                 with a skin (the cell padding) and rebuilt on the device when a
+ *                 particle has moved pad/2 or the strain left the list's window;
+ *   SC_PATH_TILE  one workgroup per run of cells, the 3 x (width+2) surrounding
+ *                 cells staged in LDS, 8 lanes per particle with xor-shuffle sums. */
+#define SC_PATH_TILE 0
+#define SC_PATH_LIST 1
+int sc_set_path(sc_system* s, int path);
+int sc_get_path(const sc_system* s, int* path);
+/* Tile path shape: columns per tile and threads per workgroup. */
 int sc_set_tiling(sc_system* s, int tile_cols, int block_threads);
+/* Returns the tile width and the workgroup size of the active path. */
 int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads);
+/* List path workgroup size (64, 128 or 256). */
+int sc_set_list_block(sc_system* s, int block_threads);
+/* Autotuner: times reps force passes of this system for each candidate path and
+ * launch shape and keeps the fastest (outputs may be NULL). */
+int sc_autotune(sc_system* s, int reps, int* path, int* tile_cols, int* block_threads,
+                double* us_per_pass);
 
 /* set_particle for every particle (src/cpp/spherocyl_box.cpp:188-221): new
    positions / angles, wrapped, then cells rebuilt. */

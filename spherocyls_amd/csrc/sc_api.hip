@@ -1,8 +1,10 @@
-// Host side of the C ABI (include/spherocyls.h): system lifecycle, cell grids
-// and tiles, kernel orchestration on the system's own HIP stream, timing.
+// Host side of the C ABI (include/spherocyls.h): system lifecycle, cell grids,
+// tiles and workgroup maps, kernel orchestration on the system's own HIP
+// stream, the neighbour-list rebuild-flag sequence, timing and the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,14 +15,16 @@
 #include "sc_internal.h"
 
 namespace sc {
-// kernels (sc_device.hip)
+// kernels (sc_device.hip, sc_list.hip)
 template <int MODE>
 __global__ void k_tile(TileArgs a);
+template <int MODE>
+__global__ void k_list(ListArgs a);
+__global__ void k_reduce(ReduceArgs a);
+__global__ void k_segvec(Bank b, int ntot);
 __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
                           int* cnt, int* slot, int* err, int ntot, int wrap_angles);
-__global__ void k_update(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
-                         int* cnt, int* slot, int* cnt_clear, int ncells_total, int* err, int ntot,
-                         int mode, double max_move);
+__global__ void k_update(UpdateArgs u);
 __global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot);
 __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart);
 __global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
@@ -45,9 +49,9 @@ struct HipError {
   const char* what;
 };
 
-#define HIPCK(call)                                 \
-  do {                                              \
-    hipError_t e_ = (call);                         \
+#define HIPCK(call)                                  \
+  do {                                               \
+    hipError_t e_ = (call);                          \
     if (e_ != hipSuccess) throw HipError{e_, #call}; \
   } while (0)
 
@@ -70,6 +74,24 @@ struct EventPair {
   int kernel;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on XCD
+// b % 8).  Reorder a packing's spatially ordered work items so that each XCD
+// receives a contiguous run: neighbouring items share particles, and the
+// shared data then stays in one XCD's L2.
+template <typename T>
+std::vector<T> xcd_order(const std::vector<T>& spatial, int first_launch_index) {
+  const int nx = 8, T_ = (int)spatial.size();
+  std::vector<int> count(nx, 0), start(nx, 0), used(nx, 0);
+  for (int i = 0; i < T_; ++i) count[(first_launch_index + i) % nx]++;
+  for (int x = 1; x < nx; ++x) start[x] = start[x - 1] + count[x - 1];
+  std::vector<T> out(T_);
+  for (int i = 0; i < T_; ++i) {
+    int x = (first_launch_index + i) % nx;
+    out[i] = spatial[start[x] + used[x]++];
+  }
+  return out;
+}
+
 }  // namespace
 
 struct sc_system {
@@ -78,18 +100,23 @@ struct sc_system {
   int npack = 0, ntot = 0;
   std::vector<int> offsets;
   std::vector<PackInfo> packs;
-  std::vector<PackScal> scal;  // host mirror of d_scal[cur_scal] (valid after sync_scal)
-  std::vector<double> Rmax, Amax, packing, filled;
-  int tile_cols = 8, block = 128;
-  int lds_max = 64 * 1024;  // LDS bytes a workgroup may use on this device
+  std::vector<PackScal> scal;  // host copy of d_scal[cur_scal] (valid after download_scal)
+  std::vector<double> Rmax, Amax, filled;
+  int path = SC_PATH_LIST;
+  int tile_cols = 12, block = 128, list_block = 256;
+  int lds_max = 64 * 1024;
   int prec = SC_PREC_FP64;
-  int ncells = 0, ntiles = 0;
+  int ncells = 0, ntiles = 0, npblocks = 0;
   std::vector<int4> tiles;
+  std::vector<int2> pblocks, tile_ranges, list_ranges;
 
   PackInfo* d_packs = nullptr;
   PackScal* d_scal[2] = {nullptr, nullptr};
   int cur_scal = 0;
   int4* d_tiles = nullptr;
+  int2* d_pblocks = nullptr;
+  int2* d_tile_ranges = nullptr;
+  int2* d_list_ranges = nullptr;
   int* d_cnt[3] = {nullptr, nullptr, nullptr};
   int* d_slot[3] = {nullptr, nullptr, nullptr};
   int cur_buf = 0;
@@ -97,12 +124,12 @@ struct sc_system {
   int* d_perm = nullptr;
   Bank bank[2]{};
   int cur_bank = 0;
-  double *d_px = nullptr, *d_py = nullptr, *d_pphi = nullptr;
+  double *d_px = nullptr, *d_py = nullptr, *d_pphi = nullptr, *d_pnx = nullptr, *d_pny = nullptr;
   double *d_v0x = nullptr, *d_v0y = nullptr, *d_v0t = nullptr;
   int* d_pack_of = nullptr;
   double* d_tile_part = nullptr;
+  double* d_list_part = nullptr;
   double* d_pack_sum = nullptr;
-  unsigned* d_pack_done = nullptr;
   double* d_se_ring = nullptr;
   size_t se_ring_cap = 0;
   int* d_err = nullptr;
@@ -112,6 +139,14 @@ struct sc_system {
   int export_cap = 0;
   double* d_scale = nullptr;
   bool velocity_live = false;
+  // neighbour lists
+  int* d_nbr = nullptr;
+  int* d_nbr_n = nullptr;
+  double *d_xb = nullptr, *d_yb = nullptr;
+  double* d_list_center = nullptr;
+  int* d_flag[3] = {nullptr, nullptr, nullptr};
+  long long kseq = 0;  // position in the force/update kernel sequence
+  double gamma_window = 0.05;
 
   bool timing = false;
   std::vector<EventPair> ev_pending;
@@ -136,6 +171,9 @@ struct sc_system {
     (void)hipFree(p);
   }
   Bank& B() { return bank[cur_bank]; }
+  int* flag_in() { return d_flag[kseq % 3]; }
+  int* flag_out() { return d_flag[(kseq + 1) % 3]; }
+  int* flag_clear() { return d_flag[(kseq + 2) % 3]; }
 };
 
 namespace {
@@ -194,6 +232,7 @@ struct KernelTimer {
 
 void build_tiles(sc_system* s) {
   s->tiles.clear();
+  s->tile_ranges.assign(s->npack, make_int2(0, 0));
   int coff = 0;
   for (int k = 0; k < s->npack; ++k) {
     PackInfo& P = s->packs[k];
@@ -204,29 +243,40 @@ void build_tiles(sc_system* s) {
     P.tile_cols = tw;
     std::vector<int4> spatial;
     for (int r = 0; r < P.rows; ++r)
-      for (int c0 = 0; c0 < P.cols; c0 += tw) spatial.push_back(make_int4(k, r, c0, std::min(tw, P.cols - c0)));
+      for (int c0 = 0; c0 < P.cols; c0 += tw)
+        spatial.push_back(make_int4(k, r, c0, std::min(tw, P.cols - c0)));
     P.toff = (int)s->tiles.size();
     P.ntiles = (int)spatial.size();
-    // XCD-aware order: workgroup b runs on XCD b % 8, so give each XCD a
-    // spatially contiguous run of tiles (neighbouring tiles share staged rows).
-    const int nx = 8, T = P.ntiles;
-    std::vector<int> count(nx, 0);
-    for (int i = 0; i < T; ++i) count[(P.toff + i) % nx]++;
-    std::vector<int> start(nx, 0);
-    for (int x = 1; x < nx; ++x) start[x] = start[x - 1] + count[x - 1];
-    std::vector<int> used(nx, 0);
-    std::vector<int4> launch(T);
-    for (int i = 0; i < T; ++i) {
-      int x = (P.toff + i) % nx;
-      launch[i] = spatial[start[x] + used[x]++];
-    }
+    s->tile_ranges[k] = make_int2(P.toff, P.ntiles);
+    auto launch = xcd_order(spatial, P.toff);
     s->tiles.insert(s->tiles.end(), launch.begin(), launch.end());
   }
   s->ncells = coff;
   s->ntiles = (int)s->tiles.size();
 }
 
-// (Re)allocate everything whose size depends on the cell grid / tiles.
+void build_pblocks(sc_system* s) {
+  s->pblocks.clear();
+  s->list_ranges.assign(s->npack, make_int2(0, 0));
+  for (int k = 0; k < s->npack; ++k) {
+    const PackInfo& P = s->packs[k];
+    std::vector<int2> spatial;
+    for (int st = P.poff; st < P.poff + P.n; st += s->list_block) spatial.push_back(make_int2(k, st));
+    s->list_ranges[k] = make_int2((int)s->pblocks.size(), (int)spatial.size());
+    auto launch = xcd_order(spatial, (int)s->pblocks.size());
+    s->pblocks.insert(s->pblocks.end(), launch.begin(), launch.end());
+  }
+  s->npblocks = (int)s->pblocks.size();
+}
+
+template <typename T>
+void upload_vec(sc_system* s, T*& d, const std::vector<T>& h) {
+  s->dfree(d);
+  d = s->dalloc<T>(h.size());
+  HIPCK(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s->stream));
+}
+
+// (Re)allocate everything whose size depends on the cell grid / work maps.
 void alloc_grid(sc_system* s) {
   for (int i = 0; i < 3; ++i) {
     s->dfree(s->d_cnt[i]);
@@ -237,32 +287,28 @@ void alloc_grid(sc_system* s) {
   }
   s->dfree(s->d_cstart);
   s->d_cstart = s->dalloc<int>(s->ncells);
-  s->dfree(s->d_tiles);
-  s->d_tiles = s->dalloc<int4>(s->ntiles);
-  HIPCK(hipMemcpyAsync(s->d_tiles, s->tiles.data(), sizeof(int4) * s->ntiles, hipMemcpyHostToDevice,
-                       s->stream));
+  upload_vec(s, s->d_tiles, s->tiles);
+  upload_vec(s, s->d_tile_ranges, s->tile_ranges);
   s->dfree(s->d_tile_part);
   s->d_tile_part = s->dalloc<double>((size_t)s->ntiles * kNSum);
   HIPCK(hipMemcpyAsync(s->d_packs, s->packs.data(), sizeof(PackInfo) * s->npack,
                        hipMemcpyHostToDevice, s->stream));
 }
 
+void alloc_pblocks(sc_system* s) {
+  build_pblocks(s);
+  upload_vec(s, s->d_pblocks, s->pblocks);
+  upload_vec(s, s->d_list_ranges, s->list_ranges);
+  s->dfree(s->d_list_part);
+  s->d_list_part = s->dalloc<double>((size_t)s->npblocks * kNSum);
+}
+
 Bank alloc_bank(sc_system* s) {
   Bank b{};
   size_t n = s->ntot;
-  b.x = s->dalloc<double>(n);
-  b.y = s->dalloc<double>(n);
-  b.phi = s->dalloc<double>(n);
-  b.R = s->dalloc<double>(n);
-  b.A = s->dalloc<double>(n);
-  b.I = s->dalloc<double>(n);
-  b.RC = s->dalloc<double>(n);
-  b.fx = s->dalloc<double>(n);
-  b.fy = s->dalloc<double>(n);
-  b.tau = s->dalloc<double>(n);
-  b.vx = s->dalloc<double>(n);
-  b.vy = s->dalloc<double>(n);
-  b.vpsi = s->dalloc<double>(n);
+  for (double** p : {&b.x, &b.y, &b.phi, &b.R, &b.A, &b.I, &b.RC, &b.fx, &b.fy, &b.tau, &b.vx,
+                     &b.vy, &b.vpsi, &b.nx, &b.ny})
+    *p = s->dalloc<double>(n);
   b.id = s->dalloc<int>(n);
   for (double* p : {b.fx, b.fy, b.tau, b.vx, b.vy, b.vpsi})
     HIPCK(hipMemsetAsync(p, 0, sizeof(double) * n, s->stream));
@@ -297,6 +343,16 @@ int check_errors(sc_system* s) {
 
 int grid_for(int n, int block) { return (n + block - 1) / block; }
 
+// Every neighbour-list row is rebuilt by the next force pass.
+void force_rebuild(sc_system* s) {
+  HIPCK(hipMemsetAsync(s->flag_in(), 1, sizeof(int) * s->npack, s->stream));
+}
+
+void segvec(sc_system* s) {
+  hipLaunchKernelGGL(k_segvec, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, s->B(),
+                     s->ntot);
+}
+
 // Bin every particle of the current bank into bucket buffer `buf` (cleared).
 void bin_all(sc_system* s, int buf, int wrap_angles) {
   HIPCK(hipMemsetAsync(s->d_cnt[buf], 0, sizeof(int) * s->ncells, s->stream));
@@ -305,7 +361,8 @@ void bin_all(sc_system* s, int buf, int wrap_angles) {
                      s->d_slot[buf], s->d_err, s->ntot, wrap_angles);
 }
 
-// Counting sort of the current bank by the cells of bucket buffer cur_buf.
+// Counting sort of the current bank by the cells of bucket buffer cur_buf:
+// per-packing scan of the counts, per-cell ordering by id, coalesced gather.
 void resort(sc_system* s) {
   KernelTimer t(s, SC_KERNEL_SORT);
   int buf = s->cur_buf;
@@ -318,48 +375,7 @@ void resort(sc_system* s) {
   hipLaunchKernelGGL(k_gather, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, src, dst,
                      s->d_perm, s->ntot, s->velocity_live ? 1 : 0);
   s->cur_bank = 1 - s->cur_bank;
-}
-
-TileArgs base_args(sc_system* s) {
-  TileArgs a{};
-  a.packs = s->d_packs;
-  a.scal = s->d_scal[s->cur_scal];
-  a.scal_next = s->d_scal[1 - s->cur_scal];
-  a.tiles = s->d_tiles;
-  a.cnt = s->d_cnt[s->cur_buf];
-  a.slot = s->d_slot[s->cur_buf];
-  a.ncells_total = s->ncells;
-  a.b = s->B();
-  a.sx = a.b.x;
-  a.sy = a.b.y;
-  a.sphi = a.b.phi;
-  a.px = s->d_px;
-  a.py = s->d_py;
-  a.pphi = s->d_pphi;
-  a.v0x = s->d_v0x;
-  a.v0y = s->d_v0y;
-  a.v0t = s->d_v0t;
-  a.tile_part = s->d_tile_part;
-  a.pack_sum = s->d_pack_sum;
-  a.pack_done = s->d_pack_done;
-  a.se_ring = s->d_se_ring;
-  a.err = s->d_err;
-  a.npack = s->npack;
-  a.list_groups = s->block / kLanes;
-  a.stage_cap = 3 * (s->tile_cols + 2) * kCap;
-  return a;
-}
-
-template <int MODE>
-void launch_tile(sc_system* s, const TileArgs& a, int kernel_id) {
-  size_t lds = tile_lds_bytes(a.stage_cap, a.list_groups, s->block);
-  if (lds > (size_t)s->lds_max) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
-  if (lds > 64 * 1024) {  // wide tiles need more than the default dynamic LDS
-    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
-  KernelTimer t(s, kernel_id);
-  hipLaunchKernelGGL(k_tile<MODE>, dim3(s->ntiles), dim3(s->block), lds, s->stream, a);
+  force_rebuild(s);  // list rows hold storage indices
 }
 
 void ensure_se_ring(sc_system* s, size_t nrec) {
@@ -370,14 +386,152 @@ void ensure_se_ring(sc_system* s, size_t nrec) {
   s->se_ring_cap = need;
 }
 
-// Plain force pass at the current state (calc_forces / calc_se).
+// ---- one force pass (either path) -----------------------------------------
+
+// What a force pass does besides forces.  Positions and cells it evaluates:
+// the current bank (pred, forces) or the predicted state (corr).
+struct PassSpec {
+  int mode = kModeForces;
+  int se_every = 0, se_always = 0, do_fire = 0, do_gd = 0, do_resid = 0;
+  int resize = 0, shrink = 0, corrected_image = 0;
+  double resize_scale = 1.0;
+  int read_buf = 0, fill_buf = -1, clear_buf = -1;
+  int kernel_id = SC_KERNEL_FORCE;
+};
+
+void launch_force(sc_system* s, const PassSpec& ps) {
+  const bool predicted = ps.mode == kModeCorr;
+  if (s->path == SC_PATH_TILE) {
+    TileArgs a{};
+    a.packs = s->d_packs;
+    a.scal = s->d_scal[s->cur_scal];
+    a.scal_next = s->d_scal[1 - s->cur_scal];
+    a.tiles = s->d_tiles;
+    a.cnt = s->d_cnt[ps.read_buf];
+    a.slot = s->d_slot[ps.read_buf];
+    a.cnt_fill = ps.fill_buf >= 0 ? s->d_cnt[ps.fill_buf] : nullptr;
+    a.slot_fill = ps.fill_buf >= 0 ? s->d_slot[ps.fill_buf] : nullptr;
+    a.cnt_clear = ps.clear_buf >= 0 ? s->d_cnt[ps.clear_buf] : nullptr;
+    a.ncells_total = s->ncells;
+    a.b = s->B();
+    a.sx = predicted ? s->d_px : a.b.x;
+    a.sy = predicted ? s->d_py : a.b.y;
+    a.sphi = predicted ? s->d_pphi : a.b.phi;
+    a.px = s->d_px;
+    a.py = s->d_py;
+    a.pphi = s->d_pphi;
+    a.v0x = s->d_v0x;
+    a.v0y = s->d_v0y;
+    a.v0t = s->d_v0t;
+    a.tile_part = s->d_tile_part;
+    a.err = s->d_err;
+    a.resize_scale = ps.resize_scale;
+    a.se_every = ps.se_every;
+    a.se_always = ps.se_always;
+    a.do_fire = ps.do_fire;
+    a.do_gd = ps.do_gd;
+    a.do_resid = ps.do_resid;
+    a.write_forces = 1;
+    a.resize = ps.resize;
+    a.shrink = ps.shrink;
+    a.corrected_image = ps.corrected_image;
+    a.npack = s->npack;
+    a.list_groups = s->block / kLanes;
+    a.stage_cap = 3 * (s->tile_cols + 2) * kCap;
+    size_t lds = tile_lds_bytes(a.stage_cap, a.list_groups, s->block);
+    if (lds > (size_t)s->lds_max) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
+    auto fn = ps.mode == kModeForces ? &k_tile<kModeForces>
+                                     : (ps.mode == kModePred ? &k_tile<kModePred> : &k_tile<kModeCorr>);
+    if (lds > 64 * 1024)
+      HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    KernelTimer t(s, ps.kernel_id);
+    hipLaunchKernelGGL(fn, dim3(s->ntiles), dim3(s->block), lds, s->stream, a);
+  } else {
+    ListArgs a{};
+    a.packs = s->d_packs;
+    a.scal = s->d_scal[s->cur_scal];
+    a.scal_next = s->d_scal[1 - s->cur_scal];
+    a.pblocks = s->d_pblocks;
+    a.b = s->B();
+    a.ex = predicted ? s->d_px : a.b.x;
+    a.ey = predicted ? s->d_py : a.b.y;
+    a.enx = predicted ? s->d_pnx : a.b.nx;
+    a.eny = predicted ? s->d_pny : a.b.ny;
+    a.px = s->d_px;
+    a.py = s->d_py;
+    a.pphi = s->d_pphi;
+    a.pnx = s->d_pnx;
+    a.pny = s->d_pny;
+    a.v0x = s->d_v0x;
+    a.v0y = s->d_v0y;
+    a.v0t = s->d_v0t;
+    a.cnt = s->d_cnt[ps.read_buf];
+    a.slot = s->d_slot[ps.read_buf];
+    a.cnt_fill = ps.fill_buf >= 0 ? s->d_cnt[ps.fill_buf] : nullptr;
+    a.slot_fill = ps.fill_buf >= 0 ? s->d_slot[ps.fill_buf] : nullptr;
+    a.cnt_clear = ps.clear_buf >= 0 ? s->d_cnt[ps.clear_buf] : nullptr;
+    a.ncells_total = s->ncells;
+    a.nbr = s->d_nbr;
+    a.nbr_n = s->d_nbr_n;
+    a.xb = s->d_xb;
+    a.yb = s->d_yb;
+    a.list_center = s->d_list_center;
+    a.flag_in = s->flag_in();
+    a.flag_out = s->flag_out();
+    a.flag_clear = s->flag_clear();
+    a.gamma_window = s->gamma_window;
+    a.part = s->d_list_part;
+    a.err = s->d_err;
+    a.resize_scale = ps.resize_scale;
+    a.ntot = s->ntot;
+    a.npack = s->npack;
+    a.se_every = ps.se_every;
+    a.se_always = ps.se_always;
+    a.do_fire = ps.do_fire;
+    a.do_gd = ps.do_gd;
+    a.do_resid = ps.do_resid;
+    a.resize = ps.resize;
+    a.shrink = ps.shrink;
+    a.corrected_image = ps.corrected_image;
+    auto fn = ps.mode == kModeForces ? &k_list<kModeForces>
+                                     : (ps.mode == kModePred ? &k_list<kModePred> : &k_list<kModeCorr>);
+    KernelTimer t(s, ps.kernel_id);
+    hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block), 0, s->stream, a);
+    s->kseq++;
+  }
+}
+
+// Per-packing reduction of the last force pass's partial sums.  It reads
+// scal[cur] and, for FIRE / gradient descent, writes the decisions to scal[next].
+void launch_reduce(sc_system* s, const PassSpec& ps, const FireParams* fp) {
+  ReduceArgs r{};
+  r.packs = s->d_packs;
+  r.scal = s->d_scal[s->cur_scal];
+  r.scal_next = s->d_scal[1 - s->cur_scal];
+  r.part = s->path == SC_PATH_TILE ? s->d_tile_part : s->d_list_part;
+  r.ranges = s->path == SC_PATH_TILE ? s->d_tile_ranges : s->d_list_ranges;
+  r.pack_sum = s->d_pack_sum;
+  r.se_ring = s->d_se_ring;
+  if (fp) r.fire = *fp;
+  r.se_every = ps.se_every;
+  r.se_always = ps.se_always;
+  r.do_fire = ps.do_fire;
+  r.do_gd = ps.do_gd;
+  r.npack = s->npack;
+  hipLaunchKernelGGL(k_reduce, dim3(s->npack), dim3(256), 0, s->stream, r);
+}
+
+// Plain force pass at the current state (calc_forces / calc_se), reduced.
 void force_pass(sc_system* s, bool se, int se_every) {
-  TileArgs a = base_args(s);
-  a.se_always = se ? 1 : 0;
-  a.se_every = se_every;
-  a.do_resid = 1;
-  a.write_forces = 1;
-  launch_tile<kModeForces>(s, a, SC_KERNEL_FORCE);
+  PassSpec ps;
+  ps.mode = kModeForces;
+  ps.se_always = se ? 1 : 0;
+  ps.se_every = se_every;
+  ps.do_resid = 1;
+  ps.read_buf = s->cur_buf;
+  launch_force(s, ps);
+  launch_reduce(s, ps, nullptr);
 }
 
 // Zero the two bucket buffers that do not hold the current cells, so the
@@ -391,6 +545,70 @@ void set_prec_all(sc_system* s, int prec) {
   for (auto& S : s->scal) S.prec = prec;
 }
 
+// One Heun sub-step (shear or resize) on every packing.  Bucket buffers
+// rotate: c0 holds the cells of x, the predictor bins x~ into c1 and zeroes
+// c2, the corrector reads c1, bins x' into c2 and zeroes c0.
+void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every, int st,
+                  double scale) {
+  const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
+  PassSpec p;
+  p.mode = kModePred;
+  p.resize = resize;
+  p.shrink = shrink;
+  p.corrected_image = corrected;
+  p.resize_scale = scale;
+  p.se_every = se_every;
+  p.read_buf = c0;
+  p.fill_buf = c1;
+  p.clear_buf = c2;
+  p.kernel_id = SC_KERNEL_PREDICT;
+  launch_force(s, p);
+  if (se_every > 0 && st > 0 && st % se_every == 0) launch_reduce(s, p, nullptr);
+  PassSpec c = p;
+  c.mode = kModeCorr;
+  c.se_every = 0;
+  c.read_buf = c1;
+  c.fill_buf = c2;
+  c.clear_buf = c0;
+  c.kernel_id = SC_KERNEL_CORRECT;
+  launch_force(s, c);
+  s->cur_buf = c2;
+  s->cur_scal = 1 - s->cur_scal;
+}
+
+// Gradient-descent (mode 0) / FIRE (mode 1) update after a reduced force pass.
+void launch_update(sc_system* s, int mode, double max_move) {
+  const int b1 = (s->cur_buf + 1) % 3, b2 = (s->cur_buf + 2) % 3;
+  UpdateArgs u{};
+  u.b = s->B();
+  u.pack_of = s->d_pack_of;
+  u.packs = s->d_packs;
+  u.scal = s->d_scal[1 - s->cur_scal];
+  u.cnt = s->d_cnt[b1];
+  u.slot = s->d_slot[b1];
+  u.cnt_clear = s->d_cnt[b2];
+  u.ncells_total = s->ncells;
+  u.err = s->d_err;
+  const bool list = s->path == SC_PATH_LIST;
+  u.xb = s->d_xb;
+  u.yb = s->d_yb;
+  u.flag_out = list ? s->flag_out() : nullptr;
+  u.flag_clear = list ? s->flag_clear() : nullptr;
+  u.skin = 0;  // per packing: the cell padding (read in the kernel)
+  u.max_move = max_move;
+  u.ntot = s->ntot;
+  u.npack = s->npack;
+  u.mode = mode;
+  {
+    KernelTimer t(s, SC_KERNEL_FIRE_UPDATE);
+    hipLaunchKernelGGL(k_update, dim3(grid_for(std::max(s->ntot, s->npack), 256)), dim3(256), 0,
+                       s->stream, u);
+  }
+  if (list) s->kseq++;
+  s->cur_buf = b1;
+  s->cur_scal = 1 - s->cur_scal;
+}
+
 int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint);
 
 }  // namespace
@@ -400,7 +618,7 @@ int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint);
 // ===========================================================================
 extern "C" {
 
-int sc_version(void) { return 10000; }
+int sc_version(void) { return 10100; }
 
 const char* sc_status_string(int status) {
   switch (status) {
@@ -463,7 +681,6 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
     s->scal.assign(npack, PackScal{});
     s->Rmax.assign(npack, 0.0);
     s->Amax.assign(npack, 0.0);
-    s->packing.assign(npack, 0.0);
     s->filled.assign(npack, 0.0);
     std::vector<double> hx(x, x + ntot), hy(y, y + ntot), hp(phi, phi + ntot), hR(R, R + ntot),
         hA(A, A + ntot), hI(ntot), hRC(ntot);
@@ -496,7 +713,6 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
         hid[i] = i - P.poff;
         hpack[i] = k;
       }
-      s->packing[k] = filled / (L[k] * L[k]);
       s->filled[k] = filled;
       P.pad = pad[k];
       cell_grid(L[k], s->Rmax[k], s->Amax[k], P.pad, P.rows, P.cols);
@@ -510,25 +726,29 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
     s->d_scal[0] = s->dalloc<PackScal>(npack);
     s->d_scal[1] = s->dalloc<PackScal>(npack);
     s->d_pack_sum = s->dalloc<double>((size_t)npack * kNSum);
-    s->d_pack_done = s->dalloc<unsigned>(npack);
-    HIPCK(hipMemsetAsync(s->d_pack_done, 0, sizeof(unsigned) * npack, s->stream));
     HIPCK(hipMemsetAsync(s->d_pack_sum, 0, sizeof(double) * npack * kNSum, s->stream));
     s->d_err = s->dalloc<int>(1);
     HIPCK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
     s->bank[0] = alloc_bank(s);
     s->bank[1] = alloc_bank(s);
-    s->d_px = s->dalloc<double>(ntot);
-    s->d_py = s->dalloc<double>(ntot);
-    s->d_pphi = s->dalloc<double>(ntot);
-    s->d_v0x = s->dalloc<double>(ntot);
-    s->d_v0y = s->dalloc<double>(ntot);
-    s->d_v0t = s->dalloc<double>(ntot);
+    for (double** p : {&s->d_px, &s->d_py, &s->d_pphi, &s->d_pnx, &s->d_pny, &s->d_v0x, &s->d_v0y,
+                       &s->d_v0t, &s->d_xb, &s->d_yb})
+      *p = s->dalloc<double>(ntot);
     s->d_pack_of = s->dalloc<int>(ntot);
     s->d_perm = s->dalloc<int>(ntot);
     HIPCK(hipMemsetAsync(s->d_perm, 0, sizeof(int) * ntot, s->stream));
     s->d_io = s->dalloc<double>((size_t)3 * ntot);
     s->d_scale = s->dalloc<double>(npack);
+    s->d_nbr = s->dalloc<int>((size_t)kNbrCap * ntot);
+    s->d_nbr_n = s->dalloc<int>(ntot);
+    HIPCK(hipMemsetAsync(s->d_nbr_n, 0, sizeof(int) * ntot, s->stream));
+    s->d_list_center = s->dalloc<double>(npack);
+    for (int i = 0; i < 3; ++i) {
+      s->d_flag[i] = s->dalloc<int>(npack);
+      HIPCK(hipMemsetAsync(s->d_flag[i], 0, sizeof(int) * npack, s->stream));
+    }
     alloc_grid(s);
+    alloc_pblocks(s);
     upload_scal(s);
 
     Bank& b = s->bank[0];
@@ -547,12 +767,12 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
                          s->stream));
     s->cur_bank = 0;
     s->cur_buf = 0;
+    segvec(s);
     bin_all(s, 0, 0);
     resort(s);
     HIPCK(hipStreamSynchronize(s->stream));
     int st = check_errors(s);
     if (st != SC_OK) {
-      std::string m = s->err;
       sc_system_destroy(s);
       return st;
     }
@@ -579,21 +799,23 @@ int sc_system_destroy(sc_system* s) {
   return SC_OK;
 }
 
-#define API_BEGIN                      \
-  if (!s) return SC_ERR_ARG;           \
-  try {                                \
+#define API_BEGIN            \
+  if (!s) return SC_ERR_ARG; \
+  try {                      \
     HIPCK(hipSetDevice(s->device));
-#define API_END                                                                   \
-  }                                                                               \
-  catch (const HipError& e) {                                                     \
+#define API_END                                                                       \
+  }                                                                                   \
+  catch (const HipError& e) {                                                         \
     return fail(s, SC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e)); \
-  }                                                                               \
-  catch (const StatusError& e) {                                                  \
-    return fail(s, e.status, e.msg);                                              \
+  }                                                                                   \
+  catch (const StatusError& e) {                                                      \
+    return fail(s, e.status, e.msg);                                                  \
   }
 
 int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate, double step) {
-  if (!s || k < -1 || k >= s->npack || !(step > 0)) return SC_ERR_ARG;
+  if (!s || k < -1 || k >= s->npack || !(step > 0) || !std::isfinite(strain_rate) ||
+      !(resize_rate >= 0 && resize_rate < 1))
+    return SC_ERR_ARG;
   API_BEGIN
   for (int i = 0; i < s->npack; ++i) {
     if (k >= 0 && i != k) continue;
@@ -609,7 +831,7 @@ int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate,
 }
 
 int sc_set_gamma(sc_system* s, int k, double gamma, double gamma_total) {
-  if (!s || k < -1 || k >= s->npack) return SC_ERR_ARG;
+  if (!s || k < -1 || k >= s->npack || !std::isfinite(gamma)) return SC_ERR_ARG;
   API_BEGIN
   download_scal(s);
   for (int i = 0; i < s->npack; ++i) {
@@ -618,6 +840,7 @@ int sc_set_gamma(sc_system* s, int k, double gamma, double gamma_total) {
     s->scal[i].gamma_total = gamma_total;
   }
   upload_scal(s);
+  force_rebuild(s);  // the lists' strain window moved
   HIPCK(hipStreamSynchronize(s->stream));
   return SC_OK;
   API_END
@@ -635,6 +858,25 @@ int sc_set_precision(sc_system* s, int mode) {
   API_END
 }
 
+int sc_set_path(sc_system* s, int path) {
+  if (!s || (path != SC_PATH_TILE && path != SC_PATH_LIST)) return SC_ERR_ARG;
+  API_BEGIN
+  if (path != s->path) {
+    s->path = path;
+    segvec(s);  // the tile path does not keep (nx, ny) current
+    force_rebuild(s);
+  }
+  HIPCK(hipStreamSynchronize(s->stream));
+  return SC_OK;
+  API_END
+}
+
+int sc_get_path(const sc_system* s, int* path) {
+  if (!s || !path) return SC_ERR_ARG;
+  *path = s->path;
+  return SC_OK;
+}
+
 int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
   if (!s || tile_cols < 1 || tile_cols > kMaxTileCols) return SC_ERR_ARG;
   if (block_threads != 64 && block_threads != 128 && block_threads != 256) return SC_ERR_ARG;
@@ -646,10 +888,8 @@ int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
   s->tile_cols = tile_cols;
   s->block = block_threads;
   build_tiles(s);
-  s->dfree(s->d_tiles);
-  s->d_tiles = s->dalloc<int4>(s->ntiles);
-  HIPCK(hipMemcpyAsync(s->d_tiles, s->tiles.data(), sizeof(int4) * s->ntiles, hipMemcpyHostToDevice,
-                       s->stream));
+  upload_vec(s, s->d_tiles, s->tiles);
+  upload_vec(s, s->d_tile_ranges, s->tile_ranges);
   s->dfree(s->d_tile_part);
   s->d_tile_part = s->dalloc<double>((size_t)s->ntiles * kNSum);
   HIPCK(hipMemcpyAsync(s->d_packs, s->packs.data(), sizeof(PackInfo) * s->npack,
@@ -662,8 +902,20 @@ int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
 int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads) {
   if (!s) return SC_ERR_ARG;
   if (tile_cols) *tile_cols = s->tile_cols;
-  if (block_threads) *block_threads = s->block;
+  if (block_threads) *block_threads = s->path == SC_PATH_TILE ? s->block : s->list_block;
   return SC_OK;
+}
+
+int sc_set_list_block(sc_system* s, int block_threads) {
+  if (!s || (block_threads != 64 && block_threads != 128 && block_threads != 256))
+    return SC_ERR_ARG;
+  API_BEGIN
+  HIPCK(hipStreamSynchronize(s->stream));
+  s->list_block = block_threads;
+  alloc_pblocks(s);
+  HIPCK(hipStreamSynchronize(s->stream));
+  return SC_OK;
+  API_END
 }
 
 int sc_set_positions(sc_system* s, const double* x, const double* y, const double* phi) {
@@ -680,6 +932,7 @@ int sc_set_positions(sc_system* s, const double* x, const double* y, const doubl
                      s->ntot);
   s->cur_buf = 0;
   bin_all(s, 0, 1);
+  segvec(s);
   resort(s);
   return check_errors(s);
   API_END
@@ -754,41 +1007,6 @@ int sc_calc_se(sc_system* s) {
   API_END
 }
 
-// One Heun sub-step (shear or resize) on every packing.  Bucket buffers
-// rotate: c holds the cells of x, the predictor bins x~ into c+1 and zeroes
-// c+2, the corrector reads c+1, bins x' into c+2 and zeroes c (ready to take
-// the next step's predicted cells).
-static void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every,
-                         double scale) {
-  const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
-  TileArgs a = base_args(s);
-  a.resize = resize;
-  a.shrink = shrink;
-  a.corrected_image = corrected;
-  a.resize_scale = scale;
-  a.se_every = se_every;
-  a.cnt_fill = s->d_cnt[c1];
-  a.slot_fill = s->d_slot[c1];
-  a.cnt_clear = s->d_cnt[c2];
-  launch_tile<kModePred>(s, a, SC_KERNEL_PREDICT);
-  TileArgs c = base_args(s);
-  c.resize = resize;
-  c.shrink = shrink;
-  c.corrected_image = corrected;
-  c.resize_scale = scale;
-  c.cnt = s->d_cnt[c1];
-  c.slot = s->d_slot[c1];
-  c.sx = s->d_px;
-  c.sy = s->d_py;
-  c.sphi = s->d_pphi;
-  c.cnt_fill = s->d_cnt[c2];
-  c.slot_fill = s->d_slot[c2];
-  c.cnt_clear = s->d_cnt[c0];
-  launch_tile<kModeCorr>(s, c, SC_KERNEL_CORRECT);
-  s->cur_buf = c2;
-  s->cur_scal = 1 - s->cur_scal;
-}
-
 static void reset_step_counters(sc_system* s) {
   download_scal(s);
   for (auto& S : s->scal) {
@@ -811,9 +1029,9 @@ int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out) {
   if (nrec > 0) ensure_se_ring(s, nrec);
   reset_step_counters(s);
   clear_other_buffers(s);
-  const int resort_every = 1000;
+  const int resort_every = 1000;  // locality only: cells are exact every pass
   for (int st = 0; st < nsteps; ++st) {
-    heun_substep(s, false, false, false, se_every, 1.0);
+    heun_substep(s, false, false, false, se_every, st, 1.0);
     if ((st + 1) % resort_every == 0 && st + 1 < nsteps) resort(s);
   }
   force_pass(s, false, se_every);
@@ -839,7 +1057,7 @@ int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, i
   for (int st = 0; st < nsteps; ++st) {
     // all packings share one resize rate and step in this call
     double scale = std::pow(1.0 - s->packs[0].resize_rate, s->packs[0].dt);
-    heun_substep(s, true, shrink != 0, corrected_image != 0, se_every, scale);
+    heun_substep(s, true, shrink != 0, corrected_image != 0, se_every, st, scale);
     for (auto& v : Lm) v = shrink ? v * scale : v / scale;
     int st2 = reconfigure_if_needed(s, &Lm);
     if (st2 != SC_OK) return st2;
@@ -861,24 +1079,17 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
   fp.emin = emin;
   fp.max_iter = nmax;
   const int chunk = 64;
-  int done = 0;
   while (true) {
     for (int m = 0; m < chunk; ++m) {
-      TileArgs a = base_args(s);
-      a.do_gd = 1;
-      a.se_always = 1;
-      a.write_forces = 1;
-      a.fire = fp;
-      launch_tile<kModeForces>(s, a, SC_KERNEL_FIRE_FORCE);
-      int b1 = (s->cur_buf + 1) % 3, b2 = (s->cur_buf + 2) % 3;
-      {
-        KernelTimer t(s, SC_KERNEL_FIRE_UPDATE);
-        hipLaunchKernelGGL(k_update, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, s->B(),
-                           s->d_pack_of, s->d_packs, s->d_scal[1 - s->cur_scal], s->d_cnt[b1],
-                           s->d_slot[b1], s->d_cnt[b2], s->ncells, s->d_err, s->ntot, 0, 0.0);
-      }
-      s->cur_buf = b1;
-      s->cur_scal = 1 - s->cur_scal;
+      PassSpec ps;
+      ps.mode = kModeForces;
+      ps.do_gd = 1;
+      ps.se_always = 1;
+      ps.read_buf = s->cur_buf;
+      ps.kernel_id = SC_KERNEL_FIRE_FORCE;
+      launch_force(s, ps);
+      launch_reduce(s, ps, &fp);
+      launch_update(s, 0, 0.0);
     }
     int e = check_errors(s);
     if (e != SC_OK) return e;
@@ -887,6 +1098,7 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
     for (auto& S : s->scal) all &= S.converged != 0;
     if (all) break;
   }
+  int done = 0;
   for (auto& S : s->scal) done = std::max(done, S.iters);
   if (steps_done) *steps_done = done;
   // positions are final; leave forces, energy and stress of the final state
@@ -898,7 +1110,7 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
 int sc_affine_resize(sc_system* s, const double* scale) {
   if (!s || !scale) return SC_ERR_ARG;
   for (int k = 0; k < s->npack; ++k)
-    if (!(scale[k] > 0)) return SC_ERR_ARG;
+    if (!(scale[k] > 0) || !std::isfinite(scale[k])) return SC_ERR_ARG;
   API_BEGIN
   download_scal(s);
   for (int k = 0; k < s->npack; ++k) {
@@ -935,6 +1147,8 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
   fp.ftol = params[8];
   fp.max_iter = params[9];
   fp.switch_frms = switch_frms;
+  if (!(fp.dt0 > 0) || !(fp.dt_max >= fp.dt0) || !(fp.max_iter >= 0) || !(fp.max_move > 0))
+    return fail(s, SC_ERR_ARG, "invalid FIRE parameters");
   for (auto& P : s->packs)
     if (!(fp.max_move < 0.5 * P.pad)) return fail(s, SC_ERR_ARG, "FIRE max_move must be < pad/2");
   download_scal(s);
@@ -958,21 +1172,14 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
   int it = 0;
   while (true) {
     for (int m = 0; m < chunk; ++m, ++it) {
-      TileArgs a = base_args(s);
-      a.do_fire = 1;
-      a.write_forces = 1;
-      a.fire = fp;
-      launch_tile<kModeForces>(s, a, SC_KERNEL_FIRE_FORCE);
-      int b1 = (s->cur_buf + 1) % 3, b2 = (s->cur_buf + 2) % 3;
-      {
-        KernelTimer t(s, SC_KERNEL_FIRE_UPDATE);
-        hipLaunchKernelGGL(k_update, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, s->B(),
-                           s->d_pack_of, s->d_packs, s->d_scal[1 - s->cur_scal], s->d_cnt[b1],
-                           s->d_slot[b1], s->d_cnt[b2], s->ncells, s->d_err, s->ntot, 1,
-                           fp.max_move);
-      }
-      s->cur_buf = b1;
-      s->cur_scal = 1 - s->cur_scal;
+      PassSpec ps;
+      ps.mode = kModeForces;
+      ps.do_fire = 1;
+      ps.read_buf = s->cur_buf;
+      ps.kernel_id = SC_KERNEL_FIRE_FORCE;
+      launch_force(s, ps);
+      launch_reduce(s, ps, &fp);
+      launch_update(s, 1, fp.max_move);
       if ((it + 1) % resort_every == 0) resort(s);
     }
     int e = check_errors(s);
@@ -993,7 +1200,6 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     if (s->scal[k].converged != 1) status = SC_ERR_NOT_CONVERGED;
   }
   // forces / energy / stress of the final state in FP64
-  download_scal(s);
   set_prec_all(s, kFP64);
   upload_scal(s);
   force_pass(s, true, 0);
@@ -1097,12 +1303,76 @@ int sc_synchronize(sc_system* s) {
   API_END
 }
 
+// Project autotuner: times the force pass of the system as it is (no state
+// change beyond the forces) for each candidate path and launch shape, keeps the
+// fastest.  reps passes per candidate after one warm-up pass.
+int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* block_out,
+                double* us_out) {
+  if (!s || reps < 1) return SC_ERR_ARG;
+  API_BEGIN
+  struct Cand {
+    int path, tile, block;
+  };
+  std::vector<Cand> cands = {{SC_PATH_LIST, 0, 64},   {SC_PATH_LIST, 0, 128},
+                             {SC_PATH_LIST, 0, 256},  {SC_PATH_TILE, 8, 128},
+                             {SC_PATH_TILE, 12, 128}, {SC_PATH_TILE, 16, 256}};
+  hipEvent_t e0, e1;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  double best = 1e30;
+  Cand pick = cands[0];
+  const bool was_timing = s->timing;
+  s->timing = false;
+  for (const Cand& c : cands) {
+    if (c.path == SC_PATH_TILE) {
+      if (tile_lds_bytes(3 * (c.tile + 2) * kCap, c.block / kLanes, c.block) > (size_t)s->lds_max)
+        continue;
+      if (sc_set_tiling(s, c.tile, c.block) != SC_OK) continue;
+    } else if (sc_set_list_block(s, c.block) != SC_OK) {
+      continue;
+    }
+    sc_set_path(s, c.path);
+    PassSpec ps;
+    ps.mode = kModeForces;
+    ps.read_buf = s->cur_buf;
+    launch_force(s, ps);  // warm-up (rebuilds lists)
+    HIPCK(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r) {
+      ps.read_buf = s->cur_buf;
+      launch_force(s, ps);
+    }
+    HIPCK(hipEventRecord(e1, s->stream));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    double us = 1e3 * ms / reps;
+    if (us < best) {
+      best = us;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->timing = was_timing;
+  if (pick.path == SC_PATH_TILE) sc_set_tiling(s, pick.tile, pick.block);
+  else sc_set_list_block(s, pick.block);
+  sc_set_path(s, pick.path);
+  int e = check_errors(s);
+  if (e != SC_OK) return e;
+  if (path_out) *path_out = pick.path;
+  if (tile_cols_out) *tile_cols_out = pick.tile;
+  if (block_out) *block_out = pick.block;
+  if (us_out) *us_out = best;
+  return SC_OK;
+  API_END
+}
+
 }  // extern "C"
 
 namespace {
 // reconfigure_cells (src/cpp/resize_dynamics.cpp:106-121): when the box length
 // changes the number of cell rows or columns, rebuild the grid, tiles and
-// buckets and re-sort.  Uses the host mirror of L in s->scal.
+// buckets and re-sort.
 int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint) {
   if (Lhint) {
     bool any = false;

@@ -24,163 +24,9 @@
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
+#include "sc_physics.h"
 
 namespace sc {
-
-// ---------------------------------------------------------------------------
-// pair math
-// ---------------------------------------------------------------------------
-template <typename T>
-struct PairOut {
-  T fx, fy, tp, tq, s, t, dx, dy, dist, dV;
-};
-
-// Segment-segment closest approach and harmonic contact force, following
-// calc_force_pair (src/cpp/strain_dynamics.cpp:38-64) term by term.  Returns
-// false when |D| >= sigma (or D is NaN).  FP64: no contraction, so the rounding
-// matches the reference's x86-64 build.
-__device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double nxA, double nyA,
-                                                 double nxB, double nyB, double Ap, double Aq,
-                                                 double sig, PairOut<double>& o) {
-#pragma clang fp contract(off)
-  double a = Ap * Ap;
-  double b = -(nxA * nxB + nyA * nyB);
-  double c = Aq * Aq;
-  double d = nxA * ddx + nyA * ddy;
-  double e = -nxB * ddx - nyB * ddy;
-  double det = a * c - b * b;
-  double t = fmin(fmax((b * d - a * e) / det, -1.0), 1.0);
-  double s = -(b * t + d) / a;
-  double sabs = fabs(s);
-  s = fmin(fmax(s, -1.0), 1.0);
-  if (sabs > 1.0) t = fmin(fmax(-(b * s + e) / c, -1.0), 1.0);
-  double Dx = ddx + s * nxA - t * nxB;
-  double Dy = ddy + s * nyA - t * nyB;
-  double d2 = Dx * Dx + Dy * Dy;
-  if (!(d2 < sig * sig)) return false;
-  double dist = sqrt(d2);
-  double dV = (1.0 - dist / sig) / sig;
-  double fx = Dx * dV / dist;
-  double fy = Dy * dV / dist;
-  o.fx = fx;
-  o.fy = fy;
-  o.tp = s * nxA * fy - s * nyA * fx;
-  o.tq = t * nyB * fx - t * nxB * fy;
-  o.s = s;
-  o.t = t;
-  o.dx = Dx;
-  o.dy = Dy;
-  o.dist = dist;
-  o.dV = dV;
-  return true;
-}
-
-// The same closest-approach force in FP32 (mixed mode), contraction allowed.
-__device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA, float nyA,
-                                                 float nxB, float nyB, float Ap, float Aq,
-                                                 float sig, PairOut<float>& o) {
-#pragma clang fp contract(fast)
-  float a = Ap * Ap;
-  float b = -(nxA * nxB + nyA * nyB);
-  float c = Aq * Aq;
-  float d = nxA * ddx + nyA * ddy;
-  float e = -nxB * ddx - nyB * ddy;
-  float det = a * c - b * b;
-  float t = fminf(fmaxf((b * d - a * e) / det, -1.0f), 1.0f);
-  float s = -(b * t + d) / a;
-  float sabs = fabsf(s);
-  s = fminf(fmaxf(s, -1.0f), 1.0f);
-  if (sabs > 1.0f) t = fminf(fmaxf(-(b * s + e) / c, -1.0f), 1.0f);
-  float Dx = ddx + s * nxA - t * nxB;
-  float Dy = ddy + s * nyA - t * nyB;
-  float d2 = Dx * Dx + Dy * Dy;
-  if (!(d2 < sig * sig)) return false;
-  float dist = sqrtf(d2);
-  float dV = (1.0f - dist / sig) / sig;
-  float fx = Dx * dV / dist;
-  float fy = Dy * dV / dist;
-  o.fx = fx;
-  o.fy = fy;
-  o.tp = s * nxA * fy - s * nyA * fx;
-  o.tq = t * nyB * fx - t * nxB * fy;
-  o.s = s;
-  o.t = t;
-  o.dx = Dx;
-  o.dy = Dy;
-  o.dist = dist;
-  o.dV = dV;
-  return true;
-}
-
-// Minimum image in sheared coordinates (src/cpp/strain_dynamics.cpp:28-29).
-__device__ __forceinline__ double min_image(double d, double L) {
-#pragma clang fp contract(off)
-  return d + L * (double)((d < -0.5 * L) - (d > 0.5 * L));
-}
-
-// Wrap rule of find_cells (src/cpp/spherocyl_box.cpp:279-286); bounded so a
-// runaway coordinate cannot spin.  Returns false for a non-finite coordinate.
-__device__ __forceinline__ bool wrap_coord(double& v, double L) {
-  if (!isfinite(v)) return false;
-  int guard = 0;
-  while (v > L && guard < 8) { v -= L; ++guard; }
-  while (v < 0.0 && guard < 16) { v += L; ++guard; }
-  if (guard >= 8) v = v - L * floor(v / L);
-  return true;
-}
-
-__device__ __forceinline__ void wrap_angle(double& p) {
-  if (!isfinite(p)) return;
-  int guard = 0;
-  while (p > kRefPi && guard < 8) { p -= 2 * kRefPi; ++guard; }
-  while (p < -kRefPi && guard < 16) { p += 2 * kRefPi; ++guard; }
-}
-
-// Bin one particle into bucket buffer (cnt, slot); the cell rule is find_cells'
-// (src/cpp/spherocyl_box.cpp:293-295), clamped so x == L stays in range.
-__device__ __forceinline__ void bin_into(int g, double xw, double yw, double Lx, double Ly,
-                                         const PackInfo& P, int* cnt, int* slot, int* err) {
-  double w = Lx / P.cols, h = Ly / P.rows;
-  int col = (int)(xw / w);
-  int row = (int)(yw / h);
-  col = min(max(col, 0), P.cols - 1);
-  row = min(max(row, 0), P.rows - 1);
-  int c = P.coff + row * P.cols + col;
-  int k = atomicAdd(&cnt[c], 1);
-  if (k < kCap) slot[c * kCap + k] = g;
-  else atomicOr(err, kErrCellOverflow);
-}
-
-// isolated_rotation, src/cpp/strain_dynamics.cpp:174-177
-__device__ __forceinline__ double iso_rot(double RC, double phi) {
-#pragma clang fp contract(off)
-  return 0.5 * (1 - RC * cos(2 * phi));
-}
-
-// ---------------------------------------------------------------------------
-// block reduction helpers (fixed order -> deterministic)
-// ---------------------------------------------------------------------------
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red, int nthr) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double x = v[i];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    v[i] = x;
-  }
-  const int w = tid >> 6, nw = nthr >> 6;
-  if ((tid & 63) == 0)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) s_red[w * NV + i] = v[i];
-  __syncthreads();
-  if (tid < NV) {
-    double x = 0;
-    for (int k = 0; k < nw; ++k) x += s_red[k * NV + tid];
-    s_red[tid] = x;  // each slot tid is read (k = 0) before this write by the same thread
-  }
-  __syncthreads();
-}
 
 // LDS carve-up of the tile kernel (sizes in entries).
 struct TileLds {
@@ -232,8 +78,9 @@ size_t tile_lds_bytes(int cap, int groups, int nthr) {
   return off;
 }
 
-// Last-tile finalisation of the reduced sums of packing pk (thread 0 only).
-__device__ void finalize_pack(const TileArgs& a, int pk, const PackInfo& P, const PackScal& S,
+// Finalisation of the reduced sums of packing pk (thread 0 of k_reduce):
+// stress/energy records, FIRE decisions, gradient-descent stop test.
+__device__ void finalize_pack(const ReduceArgs& a, int pk, const PackInfo& P, const PackScal& S,
                               const double* tot, bool do_se) {
 #pragma clang fp contract(off)
   double* ps = a.pack_sum + (size_t)pk * kNSum;
@@ -302,11 +149,37 @@ __device__ void finalize_pack(const TileArgs& a, int pk, const PackInfo& P, cons
       S2.step = S.step + 1;
     }
     a.scal_next[pk] = S2;
-  } else {
-    // plain force pass: keep the residual for sc_get_box
-    a.pack_sum[(size_t)pk * kNSum + kSumR2] = tot[kSumR2];
   }
-  a.pack_done[pk] = 0u;
+}
+
+// One workgroup per packing: sum the packing's block partials in block order
+// (deterministic), then finalise.  Runs as its own launch after the force pass,
+// so the partials are visible without device-scope fences in the force kernel.
+__global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
+  __shared__ double s_red[4 * kNSum];
+  const int pk = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+  const PackInfo P = a.packs[pk];
+  const PackScal S = a.scal[pk];
+  if ((a.do_fire || a.do_gd) && S.converged) {
+    if (tid == 0) a.scal_next[pk] = S;
+    return;
+  }
+  const bool do_se = a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0);
+  const int2 rg = a.ranges[pk];
+  double tot[kNSum];
+#pragma unroll
+  for (int i = 0; i < kNSum; ++i) tot[i] = 0.0;
+  for (int t = rg.x + tid; t < rg.x + rg.y; t += nthr) {
+    const double* tp = a.part + (size_t)t * kNSum;
+#pragma unroll
+    for (int i = 0; i < kNSum; ++i) tot[i] += tp[i];
+  }
+  block_sum<kNSum>(tot, s_red, nthr);
+  if (tid == 0) {
+    double fin[kNSum];
+    for (int i = 0; i < kNSum; ++i) fin[i] = s_red[i];
+    finalize_pack(a, pk, P, S, fin, do_se);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -316,7 +189,6 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
 #pragma clang fp contract(off)
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ int s_last;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
 
@@ -329,10 +201,7 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
   const PackInfo P = a.packs[pk];
   const PackScal S = a.scal[pk];
 
-  if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) {
-    if (blockIdx.x == P.toff && tid == 0) a.scal_next[pk] = S;
-    return;
-  }
+  if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) return;
 
   const bool do_se =
       (MODE != kModeCorr) &&
@@ -431,7 +300,6 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
 
   const int own_lo = L.cs[ci * nc + (full ? 0 : 1)];
   const int own_hi = L.cs[ci * nc + (full ? nc : 1 + ntw)];
-  const int n_own = own_hi - own_lo;
 
   // box scalars for this pass
   const double dt = P.dt, gd = P.strain_rate;
@@ -704,30 +572,6 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
   if (!do_sum) return;
   block_sum<kNSum>(acc, L.red, nthr);
   if (tid < kNSum) a.tile_part[(size_t)blockIdx.x * kNSum + tid] = L.red[tid];
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    unsigned t = atomicAdd(&a.pack_done[pk], 1u);
-    s_last = (t == (unsigned)P.ntiles - 1u);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double tot[kNSum];
-#pragma unroll
-  for (int i = 0; i < kNSum; ++i) tot[i] = 0.0;
-  for (int t = P.toff + tid; t < P.toff + P.ntiles; t += nthr) {
-    const double* tp = a.tile_part + (size_t)t * kNSum;
-#pragma unroll
-    for (int i = 0; i < kNSum; ++i) tot[i] += __builtin_nontemporal_load(tp + i);
-  }
-  __syncthreads();
-  block_sum<kNSum>(tot, L.red, nthr);
-  if (tid == 0) {
-    double fin[kNSum];
-    for (int i = 0; i < kNSum; ++i) fin[i] = L.red[i];
-    finalize_pack(a, pk, P, S, fin, do_se);
-  }
 }
 
 template __global__ void k_tile<kModeForces>(TileArgs);
@@ -764,20 +608,23 @@ __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, con
 // Gradient-descent / FIRE position update (one thread per particle) + binning.
 // mode 0: gradient_relax_step (src/cpp/relaxation_dynamics.cpp:15-19)
 // mode 1: FIRE update (specification in oracle/sc_oracle.cpp)
-__global__ void k_update(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
-                         int* cnt, int* slot, int* cnt_clear, int ncells_total, int* err, int ntot,
-                         int mode, double max_move) {
+// With neighbour lists (flag_out != null) the update raises the packing's
+// rebuild flag when a particle leaves its skin/2 box around the build position.
+__global__ void k_update(UpdateArgs u) {
 #pragma clang fp contract(off)
+  const Bank& b = u.b;
   int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cnt_clear)
-    for (int i = g; i < ncells_total; i += gridDim.x * blockDim.x) cnt_clear[i] = 0;
-  if (g >= ntot) return;
-  int pk = pack_of[g];
-  const PackInfo P = packs[pk];
-  const PackScal S = scal[pk];
+  if (u.cnt_clear)
+    for (int i = g; i < u.ncells_total; i += gridDim.x * blockDim.x) u.cnt_clear[i] = 0;
+  if (u.flag_clear && g < u.npack) u.flag_clear[g] = 0;
+  if (g >= u.ntot) return;
+  int pk = u.pack_of[g];
+  const PackInfo P = u.packs[pk];
+  const PackScal S = u.scal[pk];
   double x = b.x[g], y = b.y[g], p = b.phi[g];
-  if (!S.converged) {
-    if (mode == 0) {
+  const bool moved = !S.converged;
+  if (moved) {
+    if (u.mode == 0) {
       double dt = P.dt;
       x += dt * b.fx[g];
       y += dt * b.fy[g];
@@ -806,8 +653,8 @@ __global__ void k_update(Bank b, const int* pack_of, const PackInfo* packs, cons
       b.vpsi[g] = up;
       double mx = dtn * ux, my = dtn * uy;
       double m2 = mx * mx + my * my;
-      if (m2 > max_move * max_move) {
-        double sc = max_move / sqrt(m2);
+      if (m2 > u.max_move * u.max_move) {
+        double sc = u.max_move / sqrt(m2);
         mx *= sc;
         my *= sc;
       }
@@ -817,12 +664,22 @@ __global__ void k_update(Bank b, const int* pack_of, const PackInfo* packs, cons
     }
   }
   bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
-  bool ok = okx && oky;
-  if (!ok || !isfinite(p)) atomicOr(err, kErrNonFinite);
+  if (!okx || !oky || !isfinite(p)) atomicOr(u.err, kErrNonFinite);
   b.x[g] = x;
   b.y[g] = y;
-  b.phi[g] = p;
-  bin_into(g, x, y, S.Lx, S.Ly, P, cnt, slot, err);
+  if (moved) {
+    b.phi[g] = p;
+    double sn, cs;
+    sincos(p, &sn, &cs);
+    b.nx[g] = b.A[g] * cs;
+    b.ny[g] = b.A[g] * sn;
+  }
+  bin_into(g, x, y, S.Lx, S.Ly, P, u.cnt, u.slot, u.err);
+  if (u.flag_out && moved) {
+    const double hs = 0.5 * P.pad;  // list skin = the packing's cell padding
+    if (fabs(min_image(x - u.xb[g], S.Lx)) >= hs || fabs(min_image(y - u.yb[g], S.Ly)) >= hs)
+      u.flag_out[pk] = 1;
+  }
 }
 
 // Affine rescale (x, y) *= scale[pack] (quasi-static compression step).
@@ -915,6 +772,8 @@ __global__ void k_gather(Bank src, Bank dst, const int* perm, int ntot, int with
   dst.fy[d] = src.fy[s];
   dst.tau[d] = src.tau[s];
   dst.id[d] = src.id[s];
+  dst.nx[d] = src.nx[s];
+  dst.ny[d] = src.ny[s];
   if (with_velocity) {
     dst.vx[d] = src.vx[s];
     dst.vy[d] = src.vy[s];

@@ -66,8 +66,10 @@ struct FireParams {
 enum ErrBit : int { kErrCellOverflow = 1, kErrList = 4, kErrNonFinite = 8 };
 
 // Per-particle arrays of one storage bank (sorted by cell within a packing).
+// (nx, ny) = A (cos phi, sin phi) is kept in step with phi by every kernel
+// that moves particles, so force passes never recompute it.
 struct Bank {
-  double *x, *y, *phi, *R, *A, *I, *RC, *fx, *fy, *tau, *vx, *vy, *vpsi;
+  double *x, *y, *phi, *R, *A, *I, *RC, *fx, *fy, *tau, *vx, *vy, *vpsi, *nx, *ny;
   int* id;
 };
 
@@ -91,10 +93,7 @@ struct TileArgs {
   const double* sphi;
   double *px, *py, *pphi; // predicted positions (pred writes, corr reads)
   double *v0x, *v0y, *v0t;
-  double* tile_part;      // [tiles][kNSum]
-  double* pack_sum;       // [npack][kNSum] latest reduced sums
-  unsigned* pack_done;    // [npack] last-block tickets
-  double* se_ring;        // recorded stress/energy [rec][npack][kNSe]
+  double* tile_part;      // [tiles][kNSum] per-tile partial sums
   int* err;
   FireParams fire;
   double resize_scale;    // (1 - rate)^dt of the resize step
@@ -110,6 +109,74 @@ struct TileArgs {
   int npack;
   int stage_cap;          // LDS staging capacity (entries)
   int list_groups;        // LDS contact-list slots
+};
+
+// Neighbour-list force path.  Each particle keeps a list of candidate partners
+// chosen with a skin; a force kernel rebuilds the lists of a packing (each
+// thread its own row) when the previous kernel raised the packing's flag.
+constexpr int kNbrCap = 64;
+
+struct ListArgs {
+  const PackInfo* packs;
+  const PackScal* scal;
+  PackScal* scal_next;
+  const int2* pblocks;     // per workgroup: packing, first particle
+  Bank b;                  // current bank
+  const double *ex, *ey, *enx, *eny;  // evaluated state (current or predicted)
+  double *px, *py, *pphi, *pnx, *pny; // predicted state (pred writes, corr reads)
+  double *v0x, *v0y, *v0t;
+  const int* cnt;          // cells of the evaluated state
+  const int* slot;
+  int* cnt_fill;           // cells of the state this pass produces
+  int* slot_fill;
+  int* cnt_clear;
+  int ncells_total;
+  int* nbr;                // [kNbrCap][ntot] partner indices
+  int* nbr_n;              // [ntot]
+  double *xb, *yb;         // positions at the last list build
+  double* list_center;     // [npack] strain at the last list build
+  const int* flag_in;      // [npack] rebuild before evaluating
+  int* flag_out;           // [npack] raised for the next force pass
+  int* flag_clear;         // [npack] zeroed here (written by the pass after next)
+  double gamma_window;     // lists cover strains center +- gamma_window
+  double* part;            // [workgroups][kNSum] partial sums
+  int* err;
+  double resize_scale;
+  int ntot, npack;
+  int se_every, se_always, do_fire, do_gd, do_resid;
+  int resize, shrink, corrected_image;
+};
+
+// Arguments of the gradient-descent / FIRE update kernel.
+struct UpdateArgs {
+  Bank b;
+  const int* pack_of;
+  const PackInfo* packs;
+  const PackScal* scal;   // decisions of the preceding reduction
+  int* cnt;               // buckets filled with the new positions
+  int* slot;
+  int* cnt_clear;
+  int ncells_total;
+  int* err;
+  const double *xb, *yb;  // list build positions (list path)
+  int* flag_out;          // list path: raise the rebuild flag
+  int* flag_clear;
+  double skin, max_move;   // skin unused: the packing's pad is the skin
+  int ntot, npack, mode;
+};
+
+// Arguments of the per-packing reduction / finalisation kernel.
+struct ReduceArgs {
+  const PackInfo* packs;
+  const PackScal* scal;   // scalars of the state the force pass evaluated
+  PackScal* scal_next;    // FIRE / GD decisions
+  const double* part;     // block partial sums [blocks][kNSum]
+  const int2* ranges;     // per packing: first block, number of blocks
+  double* pack_sum;       // [npack][kNSum] latest reduced (scaled) sums
+  double* se_ring;        // recorded stress/energy [rec][npack][kNSe]
+  FireParams fire;
+  int se_every, se_always, do_fire, do_gd;
+  int npack;
 };
 
 }  // namespace sc

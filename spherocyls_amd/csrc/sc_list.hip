@@ -1,0 +1,390 @@
+// Neighbour-list force kernels (one thread per particle).
+//
+// Latency, not bandwidth or FP64 rate, bounds the tile kernel: every tile walks
+// three dependent global-load rounds and three barriers before it computes.  Here
+// a particle's partners are kept in a column-major list ([k][ntot], coalesced
+// per k), chosen with a skin, and gathered straight from the cell-sorted
+// particle arrays, so a wavefront never waits at a barrier and thousands of
+// independent particles hide the gather latency.
+//
+// Lists are rebuilt lazily inside the force pass: the kernel that produces a
+// state (predictor, corrector, FIRE/GD update) raises its packing's flag when a
+// particle has moved more than skin/2 (per sheared coordinate) from its position
+// at the last build, when a Lees-Edwards remap happened, or when the strain
+// leaves the window the lists were built for.  The next force pass then rebuilds
+// every row of that packing from the exact per-pass cell buckets, each thread its
+// own row, before using it; no host decision and no extra launch are involved.
+//
+// Selection rule (skin s <= cell pad; S = R_p + A_p + R_q + A_q; dx_s, dy minimum
+// images in sheared coordinates; strain window [g_lo, g_hi]):
+//     |dy| < S + s  and  ( |dx_s + g_lo dy| < S + 1.5 s  or  |dx_s + g_hi dy| < S + 1.5 s )
+// A pair that touches at any strain in the window with every particle within
+// s/2 of its build position satisfies it: its lab x-separation at that strain
+// changed by less than (1 + |g|) s <= 1.5 s, and a linear function of the strain
+// is smallest in magnitude at an end of the window unless it crosses zero, when
+// one end is within |dy| (g_hi - g_lo) / 2.  Such a pair also lies in adjacent
+// cells at build time (|dx_s| < 1.118 S + s <= 2.24 (Rmax + Amax) + pad).
+#include <hip/hip_runtime.h>
+
+#include "sc_internal.h"
+#include "sc_physics.h"
+
+namespace sc {
+
+// Build particle g's row of the list from the cells of the evaluated state.
+// Cells are visited in a fixed order and members in increasing id, so the row
+// (and every sum over it) is independent of the atomic order of the buckets.
+__device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double Ly, int g,
+                         double xi, double yi, double reach_i, double glo, double ghi) {
+#pragma clang fp contract(off)
+  double xw = xi, yw = yi;
+  wrap_coord(xw, Lx);
+  wrap_coord(yw, Ly);
+  const double w = Lx / P.cols, h = Ly / P.rows;
+  const int col = min(max((int)(xw / w), 0), P.cols - 1);
+  const int row = min(max((int)(yw / h), 0), P.rows - 1);
+  int rs[3], cs[3], nr = 0, nc = 0;
+  for (int d = -1; d <= 1; ++d) {
+    int r = (row + d + P.rows) % P.rows;
+    bool dup = false;
+    for (int k = 0; k < nr; ++k) dup |= rs[k] == r;
+    if (!dup) rs[nr++] = r;
+    int c = (col + d + P.cols) % P.cols;
+    dup = false;
+    for (int k = 0; k < nc; ++k) dup |= cs[k] == c;
+    if (!dup) cs[nc++] = c;
+  }
+  const double s = P.pad, sx = 1.5 * P.pad;  // skin = the packing's cell padding
+  int n = 0;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) {
+      const int cell = P.coff + rs[i] * P.cols + cs[j];
+      const int m = min(a.cnt[cell], kCap);
+      const int* sl = a.slot + (size_t)cell * kCap;
+      int last = -1;
+      for (int it = 0; it < m; ++it) {
+        int best = -1, bid = 0x7fffffff;
+        for (int k = 0; k < m; ++k) {
+          int q = sl[k];
+          int idq = a.b.id[q];
+          if (idq > last && idq < bid) {
+            bid = idq;
+            best = q;
+          }
+        }
+        last = bid;
+        if (best == g || best < 0) continue;
+        const int q = best;
+        const double reach = reach_i + a.b.R[q] + a.b.A[q];
+        const double dy = min_image(yi - a.ey[q], Ly);
+        if (!(fabs(dy) < reach + s)) continue;
+        const double dx = min_image(xi - a.ex[q], Lx);
+        const double flo = dx + glo * dy, fhi = dx + ghi * dy;
+        if (fabs(flo) < reach + sx || fabs(fhi) < reach + sx) {
+          if (n < kNbrCap) a.nbr[(size_t)n * a.ntot + g] = q;
+          ++n;
+        }
+      }
+    }
+  if (n > kNbrCap) {
+    atomicOr(a.err, kErrList);
+    n = kNbrCap;
+  }
+  a.nbr_n[g] = n;
+  return n;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_list(ListArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double s_red[4 * kNSum];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (a.cnt_clear)
+    for (int i = blockIdx.x * nthr + tid; i < a.ncells_total; i += gridDim.x * nthr)
+      a.cnt_clear[i] = 0;
+  if (a.flag_clear && blockIdx.x == 0)
+    for (int i = tid; i < a.npack; i += nthr) a.flag_clear[i] = 0;
+
+  const int2 pb = a.pblocks[blockIdx.x];
+  const int pk = pb.x;
+  const PackInfo P = a.packs[pk];
+  const PackScal S = a.scal[pk];
+  if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) return;
+  const int g = pb.y + tid;
+  const bool active = g < P.poff + P.n;
+
+  const bool do_se =
+      (MODE != kModeCorr) &&
+      (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
+  const bool do_sum = do_se || (MODE == kModeForces && (a.do_fire || a.do_gd || a.do_resid));
+
+  const double dt = P.dt, gd = P.strain_rate;
+  double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
+  double L_new_x = Lx, L_new_y = Ly;
+  if (MODE != kModeForces && a.resize) {
+    L_new_x = a.shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
+    L_new_y = a.shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
+  }
+  if (MODE == kModeCorr) {
+    gam = S.gamma + dt * gd;
+    if (a.resize && a.corrected_image) {
+      Lx = L_new_x;
+      Ly = L_new_y;
+    }
+  }
+  const bool fp64 = S.prec == kFP64;
+
+  double acc[kNSum];
+#pragma unroll
+  for (int i = 0; i < kNSum; ++i) acc[i] = 0.0;
+
+  if (active) {
+    const double xi = a.ex[g], yi = a.ey[g];
+    const double nxi = a.enx[g], nyi = a.eny[g];
+    const double Ri = a.b.R[g], Ai = a.b.A[g];
+    const int idi = a.b.id[g];
+    const double reach_i = Ri + Ai;
+
+    // ---- list (rebuilt here when the previous pass asked for it)
+    const bool rebuild = a.flag_in[pk] != 0;
+    double center, xbi, ybi;
+    int n;
+    if (rebuild) {
+      center = gam;
+      n = build_row(a, P, Lx, Ly, g, xi, yi, reach_i, gam - a.gamma_window,
+                    gam + a.gamma_window);
+      xbi = xi;
+      ybi = yi;
+      a.xb[g] = xi;
+      a.yb[g] = yi;
+      if (g == P.poff) a.list_center[pk] = gam;
+    } else {
+      center = a.list_center[pk];
+      n = a.nbr_n[g];
+      xbi = a.xb[g];
+      ybi = a.yb[g];
+    }
+
+    // ---- forces over the row
+    double Fx = 0.0, Fy = 0.0, Tq = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const int q = a.nbr[(size_t)k * a.ntot + g];
+      const double xq = a.ex[q], yq = a.ey[q];
+      const double Rq = a.b.R[q], Aq = a.b.A[q];
+      double dx = min_image(xi - xq, Lx);
+      double dy = min_image(yi - yq, Ly);
+      dx += gam * dy;
+      const double reach = reach_i + Rq + Aq;
+      if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
+      const int idq = a.b.id[q];
+      const double nxq = a.enx[q], nyq = a.eny[q];
+      // the reference's orientation: lower particle id first
+      const bool iP = idi < idq;
+      double ddx = min_image(iP ? xi - xq : xq - xi, Lx);
+      double ddy = min_image(iP ? yi - yq : yq - yi, Ly);
+      ddx += gam * ddy;
+      const double sig = Ri + Rq;
+      const double nxA = iP ? nxi : nxq, nyA = iP ? nyi : nyq;
+      const double nxB = iP ? nxq : nxi, nyB = iP ? nyq : nyi;
+      const double Ap = iP ? Ai : Aq, Aqq = iP ? Aq : Ai;
+      PairOut<double> r;
+      bool hit;
+      if (fp64) {
+        hit = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, r);
+      } else {
+        PairOut<float> rf;
+        hit = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
+                               (float)nyB, (float)Ap, (float)Aqq, (float)sig, rf);
+        r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
+        r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
+      }
+      if (!hit) continue;
+      if (iP) {
+        Fx += r.fx;
+        Fy += r.fy;
+        Tq += r.tp;
+        if (do_se) {
+          double cxa = r.s * nxA - 0.5 * r.dx;
+          double cya = r.s * nyA - 0.5 * r.dy;
+          double cxb = r.t * nxB + 0.5 * r.dx;
+          double cyb_ref = r.t * nxB + 0.5 * r.dy;
+          double cyb = r.t * nyB + 0.5 * r.dy;
+          acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
+          acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
+          acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
+          acc[kSePxy] += r.fy * cxa;
+          acc[kSeCPxx] += r.fx * (cxa - cxb);
+          acc[kSeCPyy] += r.fy * (cya - cyb);
+          acc[kSeCPxy] += r.fy * (cxa - cxb);
+          acc[kSeCPyx] += r.fx * (cya - cyb);
+          acc[kSeNc] += 1.0;
+        }
+      } else {
+        Fx -= r.fx;
+        Fy -= r.fy;
+        Tq += r.tq;
+      }
+    }
+
+    // ---- per-particle epilogue
+    const double half_skin = 0.5 * P.pad;
+    bool raise = false;
+    if (MODE == kModeForces) {
+      a.b.fx[g] = Fx;
+      a.b.fy[g] = Fy;
+      a.b.tau[g] = Tq;
+      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
+      if (a.do_fire) {
+        double sq = sqrt(a.b.I[g]);
+        double gp = Tq / sq;
+        double vx = a.b.vx[g], vy = a.b.vy[g], vp = a.b.vpsi[g];
+        acc[kSumP] += Fx * vx + Fy * vy + gp * vp;
+        acc[kSumG] += Fx * Fx + Fy * Fy + gp * gp;
+        acc[kSumV] += vx * vx + vy * vy + vp * vp;
+      }
+    } else if (MODE == kModePred) {
+      const double p0 = a.b.phi[g];
+      double v0x, v0y, v0t, qx, qy, qp;
+      if (!a.resize) {
+        // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
+        v0x = Fx - S.gamma * Fy;
+        v0y = Fy;
+        v0t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], p0);
+        qx = xi + dt * v0x;
+        qy = yi + dt * v0y;
+        qp = p0 + dt * v0t;
+      } else {
+        // resize_step predictor, src/cpp/resize_dynamics.cpp:174-190
+        v0x = Fx;
+        v0y = Fy;
+        v0t = Tq;
+        qx = xi + dt * v0x;
+        qy = yi + dt * v0y;
+        qp = p0 + dt * v0t / a.b.I[g];
+        if (a.shrink) {
+          qx *= a.resize_scale;
+          qy *= a.resize_scale;
+        } else {
+          qx /= a.resize_scale;
+          qy /= a.resize_scale;
+        }
+      }
+      a.px[g] = qx;
+      a.py[g] = qy;
+      a.pphi[g] = qp;
+      double sn, cs;
+      sincos(qp, &sn, &cs);
+      a.pnx[g] = Ai * cs;
+      a.pny[g] = Ai * sn;
+      a.v0x[g] = v0x;
+      a.v0y[g] = v0y;
+      a.v0t[g] = v0t;
+      const double Lbx = a.resize ? L_new_x : Lx, Lby = a.resize ? L_new_y : Ly;
+      double wx = qx, wy = qy;
+      bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
+      if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
+      bin_into(g, wx, wy, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
+      // the corrector evaluates x~ at gamma + dt gdot, with the rescaled box
+      // length in resize runs
+      const double gnext = S.gamma + dt * gd;
+      raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
+              fabs(min_image(qy - ybi, Lby)) >= half_skin ||
+              fabs(gnext - center) >= a.gamma_window || a.resize;
+      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
+    } else {  // kModeCorr
+      double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
+      double Lbx = Lx, Lby = Ly;
+      bool remap = false;
+      if (!a.resize) {
+        // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
+        double v1x = Fx - gam * Fy;
+        double v1y = Fy;
+        double v1t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], a.pphi[g]);
+        x1 += 0.5 * dt * (a.v0x[g] + v1x);
+        y1 += 0.5 * dt * (a.v0y[g] + v1y);
+        p1 += 0.5 * dt * (a.v0t[g] + v1t);
+        double gnew = S.gamma + gd * dt;
+        if (gnew >= 0.5) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
+          remap = true;
+          if (y1 < 0) y1 += S.Ly;
+          else if (y1 > S.Ly) y1 -= S.Ly;
+          x1 += y1;
+          if (x1 < 0) x1 += S.Lx;
+          else {
+            int guard = 0;
+            while (x1 > S.Lx && guard++ < 8) x1 -= S.Lx;
+          }
+        }
+        Lbx = S.Lx;
+        Lby = S.Ly;
+      } else {
+        // resize_step corrector, src/cpp/resize_dynamics.cpp:196-215
+        x1 += 0.5 * dt * (a.v0x[g] + Fx);
+        y1 += 0.5 * dt * (a.v0y[g] + Fy);
+        p1 += 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
+        if (a.shrink) {
+          x1 *= a.resize_scale;
+          y1 *= a.resize_scale;
+        } else {
+          x1 /= a.resize_scale;
+          y1 /= a.resize_scale;
+        }
+        Lbx = L_new_x;
+        Lby = L_new_y;
+      }
+      bool okx = wrap_coord(x1, Lbx), oky = wrap_coord(y1, Lby);
+      if (!okx || !oky || !isfinite(p1)) atomicOr(a.err, kErrNonFinite);
+      a.b.x[g] = x1;
+      a.b.y[g] = y1;
+      a.b.phi[g] = p1;
+      double sn, cs;
+      sincos(p1, &sn, &cs);
+      a.b.nx[g] = Ai * cs;
+      a.b.ny[g] = Ai * sn;
+      bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
+      double gnext = S.gamma + gd * dt;
+      if (remap) gnext -= 1.0;
+      raise = remap || a.resize || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
+              fabs(min_image(y1 - ybi, Lby)) >= half_skin || fabs(gnext - center) >= a.gamma_window;
+    }
+    if (raise && a.flag_out) a.flag_out[pk] = 1;
+  }
+
+  // ---- scalar advance (corrector: one thread per packing)
+  if (MODE == kModeCorr && g == P.poff) {
+    PackScal S2 = S;
+    S2.step = S.step + 1;
+    if (!a.resize) {
+      S2.gamma = S.gamma + gd * dt;
+      S2.gamma_total = S.gamma_total + gd * dt;
+      if (S2.gamma >= 0.5) {
+        S2.gamma -= 1.0;
+        S2.gamma_total = (double)(int)(S2.gamma_total + 1) + S2.gamma;
+      }
+    } else {
+      S2.Lx = L_new_x;
+      S2.Ly = L_new_y;
+    }
+    a.scal_next[pk] = S2;
+  }
+
+  if (!do_sum) return;
+  block_sum<kNSum>(acc, s_red, nthr);
+  if (tid < kNSum) a.part[(size_t)blockIdx.x * kNSum + tid] = s_red[tid];
+}
+
+template __global__ void k_list<kModeForces>(ListArgs);
+template __global__ void k_list<kModePred>(ListArgs);
+template __global__ void k_list<kModeCorr>(ListArgs);
+
+// (nx, ny) = A (cos phi, sin phi) for every particle of the bank.
+__global__ void k_segvec(Bank b, int ntot) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  double sn, cs;
+  sincos(b.phi[g], &sn, &cs);
+  b.nx[g] = b.A[g] * cs;
+  b.ny[g] = b.A[g] * sn;
+}
+
+}  // namespace sc

@@ -61,6 +61,10 @@ def load():
         "sc_set_gamma": ([sysp, C.c_int, C.c_double, C.c_double], C.c_int),
         "sc_set_precision": ([sysp, C.c_int], C.c_int),
         "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
+        "sc_set_path": ([sysp, C.c_int], C.c_int),
+        "sc_get_path": ([sysp, _ip], C.c_int),
+        "sc_set_list_block": ([sysp, C.c_int], C.c_int),
+        "sc_autotune": ([sysp, C.c_int, _ip, _ip, _ip, _dp], C.c_int),
         "sc_get_tiling": ([sysp, _ip, _ip], C.c_int),
         "sc_set_positions": ([sysp, _dp, _dp, _dp], C.c_int),
         "sc_get_positions": ([sysp, _dp, _dp, _dp], C.c_int),
@@ -177,6 +181,24 @@ class System:
 
     def set_tiling(self, tile_cols, block):
         self._ck(self.lib.sc_set_tiling(self.h, tile_cols, block))
+
+    def set_path(self, path):
+        self._ck(self.lib.sc_set_path(self.h, {"tile": 0, "list": 1}.get(path, path)))
+
+    def path(self):
+        v = C.c_int()
+        self._ck(self.lib.sc_get_path(self.h, C.byref(v)))
+        return ("tile", "list")[v.value]
+
+    def set_list_block(self, block):
+        self._ck(self.lib.sc_set_list_block(self.h, block))
+
+    def autotune(self, reps=5):
+        p, t, b, us = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._ck(self.lib.sc_autotune(self.h, reps, C.byref(p), C.byref(t), C.byref(b),
+                                      C.byref(us)))
+        return dict(path=("tile", "list")[p.value], tile_cols=t.value, block=b.value,
+                    us_per_pass=us.value)
 
     def tiling(self):
         a, b = C.c_int(), C.c_int()

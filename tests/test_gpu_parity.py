@@ -19,9 +19,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def system_from_golden(g):
+PATHS = ["list", "tile"]
+
+
+def system_from_golden(g, path="list"):
     pk = configs.Packing(float(g["L"][0]), g["x"], g["y"], g["phi"], g["A"], g["R"], 0.0)
     s = native.System(pk, pads=float(g["pad"][0]))
+    s.set_path(path)
     s.set_gamma(float(g["gamma"][0]), 0.0)
     return s, pk
 
@@ -46,10 +50,11 @@ def pairset(p, q):
     return set(zip(np.minimum(p, q).tolist(), np.maximum(p, q).tolist()))
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("name", golden_names("static_"))
-def test_static_forces_stress_sets(name):
+def test_static_forces_stress_sets(name, path):
     g = golden(name)
-    s, pk = system_from_golden(g)
+    s, pk = system_from_golden(g, path)
     try:
         s.calc_se()
         fx, fy, tau = s.forces()
@@ -72,10 +77,11 @@ def test_static_forces_stress_sets(name):
         s.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("name", golden_names("heun_"))
-def test_heun_trajectory(name):
+def test_heun_trajectory(name, path):
     g = golden(name)
-    s, pk = system_from_golden(g)
+    s, pk = system_from_golden(g, path)
     L = pk.L
     try:
         s.set_dynamics(strain_rate=float(g["strain_rate"][0]), step=float(g["dt"][0]))
@@ -105,11 +111,16 @@ def test_heun_trajectory(name):
         s.close()
 
 
-def test_resize_steps_corrected_image():
+@pytest.mark.parametrize("path", PATHS)
+def test_resize_steps_corrected_image(path):
     """Resize Heun steps with the rescaled box length in the predictor pass,
-    against the oracle restatement step for step."""
+    against the oracle restatement step for step.  The oracle runs with a
+    padding large enough that its Verlet list stays complete: at pad 0.1 the
+    predictor moves particles by up to dt|F| = 0.089 > pad/2 here, and the
+    reference's list (src/cpp/resize_dynamics.cpp:97-103) then misses contacts
+    of the predicted state, which the per-pass cell binning on the GPU finds."""
     g = golden("resize128")
-    s, pk = system_from_golden(g)
+    s, pk = system_from_golden(g, path)
     nsteps, rate, dt = int(g["nsteps"][0]), float(g["rate"][0]), float(g["dt"][0])
     try:
         s.set_dynamics(resize_rate=rate, step=dt)
@@ -119,7 +130,7 @@ def test_resize_steps_corrected_image():
         fx, fy, tau = s.forces()
     finally:
         s.close()
-    ob = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, float(g["pad"][0]))
+    ob = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 1.5)
     ob.set_params(0.0, 0.0, 0.0, rate, dt)
     ob.calc_forces()
     for _ in range(nsteps):
@@ -157,9 +168,10 @@ def test_resize_steps_reference_image_quirk():
     assert np.max(np.abs(wrap_diff(y, g["yN"], L))) < 5e-3
 
 
-def test_gradient_descent():
+@pytest.mark.parametrize("path", PATHS)
+def test_gradient_descent(path):
     g = golden("gd128")
-    s, pk = system_from_golden(g)
+    s, pk = system_from_golden(g, path)
     try:
         s.set_dynamics(step=float(g["dt"][0]))
         n = int(g["nsteps"][0])
@@ -178,12 +190,14 @@ FIRE = dict(dt0=0.05, dt_max=0.5, n_min=5, f_inc=1.1, f_dec=0.5, alpha0=0.1, f_a
             max_move=0.02, ftol=1e-10)
 
 
-def test_fire_first_iterations_match_oracle():
+@pytest.mark.parametrize("path", PATHS)
+def test_fire_first_iterations_match_oracle(path):
     R, A = configs.bidisperse(400)
     pk = configs.make_packing(R, A, 0.80, 51)
     k = 40
     params = native.fire_params(**FIRE, max_iter=k)
     s = native.System(pk, pads=0.1)
+    s.set_path(path)
     try:
         it, fr, st = s.fire(params, check=False)
         assert st == native.SC_ERR_NOT_CONVERGED and it[0] == k
@@ -271,6 +285,7 @@ def test_tiling_shapes_give_identical_forces():
     R, A = configs.bidisperse(3000)
     pk = configs.make_packing(R, A, 0.85, 57, gamma=0.2)
     s = native.System(pk, pads=0.1)
+    s.set_path("tile")
     try:
         s.set_gamma(0.2)
         ref = None
@@ -287,9 +302,11 @@ def test_tiling_shapes_give_identical_forces():
         s.close()
 
 
-def test_large_system_force_parity():
+@pytest.mark.parametrize("path", PATHS)
+def test_large_system_force_parity(path):
     pk = configs.shear_large(1)
     s = native.System(pk, pads=0.1)
+    s.set_path(path)
     try:
         s.calc_se()
         f = s.forces()
@@ -332,3 +349,57 @@ def test_error_paths():
         assert np.all(np.isfinite(s.forces()[0]))
     finally:
         s.close()
+
+
+def test_list_and_tile_paths_agree_and_autotune():
+    """Both force paths give the same forces to rounding; list-path results do
+    not depend on the workgroup size; the autotuner returns a working shape."""
+    R, A = configs.bidisperse(5000, aspect=3.0)
+    pk = configs.make_packing(R, A, 0.84, 61, gamma=-0.3)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_gamma(-0.3)
+        s.set_path("tile")
+        s.calc_forces()
+        ft = s.forces()
+        s.set_path("list")
+        ref = None
+        for blk in (64, 128, 256):
+            s.set_list_block(blk)
+            s.calc_forces()
+            f = s.forces()
+            if ref is None:
+                ref = f
+            for a, b in zip(f, ref):
+                np.testing.assert_array_equal(a, b)
+        ob = oracle_at(pk, -0.3, 0.1)
+        sc = ob.force_scale()
+        assert_forces(ref, ft, sc, rel=1e-14)
+        choice = s.autotune(reps=3)
+        assert choice["path"] in ("list", "tile") and choice["us_per_pass"] > 0
+        s.calc_forces()
+        assert_forces(s.forces(), ref, sc, rel=1e-14)
+    finally:
+        s.close()
+
+
+def test_list_rebuilds_follow_motion():
+    """Many Heun steps at a high strain rate (Lees-Edwards remaps and strain-window
+    rebuilds) and a FIRE run (displacement rebuilds) on the list path equal the
+    tile path, which has no lists at all."""
+    R, A = configs.bidisperse(800)
+    pk = configs.make_packing(R, A, 0.82, 63, gamma=0.3)
+    out = {}
+    for path in PATHS:
+        s = native.System(pk, pads=0.1)
+        try:
+            s.set_path(path)
+            s.set_gamma(0.3)
+            s.set_dynamics(strain_rate=0.5, step=0.01)
+            s.strain_steps(300)
+            out[path] = s.positions() + (s.box()["gamma"],)
+        finally:
+            s.close()
+    assert out["list"][3] == out["tile"][3] and out["list"][3] < 0
+    assert np.max(np.abs(wrap_diff(out["list"][0], out["tile"][0], pk.L))) < 1e-9
+    assert np.max(np.abs(wrap_diff(out["list"][2], out["tile"][2], 2 * np.pi))) < 1e-9
