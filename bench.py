@@ -84,7 +84,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="shear_large", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--path", default="auto", choices=["auto", "list", "tile"])
+    ap.add_argument("--tile", type=int, default=12, help="tile path: columns per tile")
+    ap.add_argument("--lanes", type=int, default=1, help="list path: lanes per particle")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -99,9 +101,17 @@ def main():
     s.set_dynamics(strain_rate=strain_rate, step=dt)
     if args.precision == "mixed":
         s.set_precision("mixed")
-    if args.tile and args.block:
-        s.set_tiling(args.tile, args.block)
-    tile, block = s.tiling()
+    if args.path == "auto":
+        tuned = s.autotune(reps=5)  # the project's autotuner: path and launch shape
+    else:
+        s.set_path(args.path)
+        if args.path == "tile":
+            s.set_tiling(args.tile, args.block or 128)
+        else:
+            s.set_list_block(args.block or 256, args.lanes)
+        tuned = None
+    path = s.path()
+    shape, block = s.tiling()
 
     s.strain_steps(args.warmup)
     s.synchronize()
@@ -137,10 +147,12 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-sum",
         "data": "synthetic (seeded non-crossing random packing)",
         "config": {"workload": pk.name, "n_particles": pk.n, "strain_rate": strain_rate, "dt": dt,
-                   "cell_pad": 0.1, "tile_cols": tile, "block_threads": block},
+                   "cell_pad": 0.1, "force_path": path,
+                   ("tile_cols" if path == "tile" else "lanes_per_particle"): shape,
+                   "block_threads": block, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile<corrector>", "avg_us": corr_avg * 1e6,
+                     "kernel": f"k_{path}<corrector>", "avg_us": corr_avg * 1e6,
                      "predictor_avg_us": pred_avg * 1e6,
                      "bytes_per_particle": BYTES_CORR},
     }

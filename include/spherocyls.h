@@ -109,13 +109,16 @@ int sc_set_path(sc_system* s, int path);
 int sc_get_path(const sc_system* s, int* path);
 /* Tile path shape: columns per tile and threads per workgroup. */
 int sc_set_tiling(sc_system* s, int tile_cols, int block_threads);
-/* Returns the tile width and the workgroup size of the active path. */
+/* Active path's shape: tile width (tile path) or lanes per particle (list path),
+   and its workgroup size. */
 int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads);
-/* List path workgroup size (64, 128 or 256). */
-int sc_set_list_block(sc_system* s, int block_threads);
+/* List path shape: workgroup size (64, 128 or 256) and lanes per particle
+ * (1, 2, 4 or 8; more lanes give small systems more parallelism). */
+int sc_set_list_block(sc_system* s, int block_threads, int lanes);
 /* Autotuner: times reps force passes of this system for each candidate path and
- * launch shape and keeps the fastest (outputs may be NULL). */
-int sc_autotune(sc_system* s, int reps, int* path, int* tile_cols, int* block_threads,
+ * launch shape and keeps the fastest.  *shape gets the tile width (tile path)
+ * or the lanes per particle (list path).  Outputs may be NULL. */
+int sc_autotune(sc_system* s, int reps, int* path, int* shape, int* block_threads,
                 double* us_per_pass);
 
 /* set_particle for every particle (src/cpp/spherocyl_box.cpp:188-221): new

@@ -18,7 +18,7 @@ namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
 template <int MODE>
 __global__ void k_tile(TileArgs a);
-template <int MODE>
+template <int MODE, int LANES>
 __global__ void k_list(ListArgs a);
 __global__ void k_reduce(ReduceArgs a);
 __global__ void k_segvec(Bank b, int ntot);
@@ -103,7 +103,7 @@ struct sc_system {
   std::vector<PackScal> scal;  // host copy of d_scal[cur_scal] (valid after download_scal)
   std::vector<double> Rmax, Amax, filled;
   int path = SC_PATH_LIST;
-  int tile_cols = 12, block = 128, list_block = 256;
+  int tile_cols = 12, block = 128, list_block = 256, list_lanes = 1;
   int lds_max = 64 * 1024;
   int prec = SC_PREC_FP64;
   int ncells = 0, ntiles = 0, npblocks = 0;
@@ -261,7 +261,8 @@ void build_pblocks(sc_system* s) {
   for (int k = 0; k < s->npack; ++k) {
     const PackInfo& P = s->packs[k];
     std::vector<int2> spatial;
-    for (int st = P.poff; st < P.poff + P.n; st += s->list_block) spatial.push_back(make_int2(k, st));
+    const int per_block = s->list_block / s->list_lanes;
+    for (int st = P.poff; st < P.poff + P.n; st += per_block) spatial.push_back(make_int2(k, st));
     s->list_ranges[k] = make_int2((int)s->pblocks.size(), (int)spatial.size());
     auto launch = xcd_order(spatial, (int)s->pblocks.size());
     s->pblocks.insert(s->pblocks.end(), launch.begin(), launch.end());
@@ -494,8 +495,16 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.resize = ps.resize;
     a.shrink = ps.shrink;
     a.corrected_image = ps.corrected_image;
-    auto fn = ps.mode == kModeForces ? &k_list<kModeForces>
-                                     : (ps.mode == kModePred ? &k_list<kModePred> : &k_list<kModeCorr>);
+    using Fn = void (*)(ListArgs);
+    static const Fn table[3][4] = {
+        {&k_list<kModeForces, 1>, &k_list<kModeForces, 2>, &k_list<kModeForces, 4>,
+         &k_list<kModeForces, 8>},
+        {&k_list<kModePred, 1>, &k_list<kModePred, 2>, &k_list<kModePred, 4>,
+         &k_list<kModePred, 8>},
+        {&k_list<kModeCorr, 1>, &k_list<kModeCorr, 2>, &k_list<kModeCorr, 4>,
+         &k_list<kModeCorr, 8>}};
+    const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
+    Fn fn = table[ps.mode][li];
     KernelTimer t(s, ps.kernel_id);
     hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block), 0, s->stream, a);
     s->kseq++;
@@ -551,22 +560,29 @@ void set_prec_all(sc_system* s, int prec) {
 void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every, int st,
                   double scale) {
   const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
+  if (se_every > 0 && st > 0 && st % se_every == 0) {
+    // stress / energy record of the state the step starts from (the reference
+    // evaluates it with calc_se after the previous step, strain_dynamics.cpp:289-291)
+    PassSpec r;
+    r.mode = kModeForces;
+    r.se_every = se_every;
+    r.read_buf = c0;
+    launch_force(s, r);
+    launch_reduce(s, r, nullptr);
+  }
   PassSpec p;
   p.mode = kModePred;
   p.resize = resize;
   p.shrink = shrink;
   p.corrected_image = corrected;
   p.resize_scale = scale;
-  p.se_every = se_every;
   p.read_buf = c0;
   p.fill_buf = c1;
   p.clear_buf = c2;
   p.kernel_id = SC_KERNEL_PREDICT;
   launch_force(s, p);
-  if (se_every > 0 && st > 0 && st % se_every == 0) launch_reduce(s, p, nullptr);
   PassSpec c = p;
   c.mode = kModeCorr;
-  c.se_every = 0;
   c.read_buf = c1;
   c.fill_buf = c2;
   c.clear_buf = c0;
@@ -901,17 +917,19 @@ int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
 
 int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads) {
   if (!s) return SC_ERR_ARG;
-  if (tile_cols) *tile_cols = s->tile_cols;
+  if (tile_cols) *tile_cols = s->path == SC_PATH_TILE ? s->tile_cols : s->list_lanes;
   if (block_threads) *block_threads = s->path == SC_PATH_TILE ? s->block : s->list_block;
   return SC_OK;
 }
 
-int sc_set_list_block(sc_system* s, int block_threads) {
+int sc_set_list_block(sc_system* s, int block_threads, int lanes) {
   if (!s || (block_threads != 64 && block_threads != 128 && block_threads != 256))
     return SC_ERR_ARG;
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return SC_ERR_ARG;
   API_BEGIN
   HIPCK(hipStreamSynchronize(s->stream));
   s->list_block = block_threads;
+  s->list_lanes = lanes;
   alloc_pblocks(s);
   HIPCK(hipStreamSynchronize(s->stream));
   return SC_OK;
@@ -1311,11 +1329,14 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
   if (!s || reps < 1) return SC_ERR_ARG;
   API_BEGIN
   struct Cand {
-    int path, tile, block;
+    int path, tile, block, lanes;
   };
-  std::vector<Cand> cands = {{SC_PATH_LIST, 0, 64},   {SC_PATH_LIST, 0, 128},
-                             {SC_PATH_LIST, 0, 256},  {SC_PATH_TILE, 8, 128},
-                             {SC_PATH_TILE, 12, 128}, {SC_PATH_TILE, 16, 256}};
+  std::vector<Cand> cands;
+  for (int lanes : {1, 2, 4, 8})
+    for (int blk : {64, 128, 256}) cands.push_back({SC_PATH_LIST, 0, blk, lanes});
+  for (Cand c : {Cand{SC_PATH_TILE, 8, 128, 0}, Cand{SC_PATH_TILE, 12, 128, 0},
+                 Cand{SC_PATH_TILE, 16, 256, 0}})
+    cands.push_back(c);
   hipEvent_t e0, e1;
   HIPCK(hipEventCreate(&e0));
   HIPCK(hipEventCreate(&e1));
@@ -1328,19 +1349,30 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
       if (tile_lds_bytes(3 * (c.tile + 2) * kCap, c.block / kLanes, c.block) > (size_t)s->lds_max)
         continue;
       if (sc_set_tiling(s, c.tile, c.block) != SC_OK) continue;
-    } else if (sc_set_list_block(s, c.block) != SC_OK) {
+    } else if (sc_set_list_block(s, c.block, c.lanes) != SC_OK) {
       continue;
     }
     sc_set_path(s, c.path);
+    // The Heun predictor is the representative pass: same gathers and pair math
+    // as the corrector, and it leaves the state (positions) untouched.  Its
+    // output buckets are zeroed before every repetition (same cost for every
+    // candidate, so the ranking is unaffected).
     PassSpec ps;
-    ps.mode = kModeForces;
+    ps.mode = kModePred;
     ps.read_buf = s->cur_buf;
-    launch_force(s, ps);  // warm-up (rebuilds lists)
-    HIPCK(hipEventRecord(e0, s->stream));
-    for (int r = 0; r < reps; ++r) {
-      ps.read_buf = s->cur_buf;
+    ps.fill_buf = (s->cur_buf + 1) % 3;
+    ps.clear_buf = (s->cur_buf + 2) % 3;
+    auto one = [&]() {
+      HIPCK(hipMemsetAsync(s->d_cnt[ps.fill_buf], 0, sizeof(int) * s->ncells, s->stream));
       launch_force(s, ps);
-    }
+    };
+    PassSpec fwd;
+    fwd.mode = kModeForces;
+    fwd.read_buf = s->cur_buf;
+    launch_force(s, fwd);  // warm-up: rebuilds the lists at the current state
+    one();
+    HIPCK(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r) one();
     HIPCK(hipEventRecord(e1, s->stream));
     HIPCK(hipEventSynchronize(e1));
     float ms = 0;
@@ -1355,12 +1387,12 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
   (void)hipEventDestroy(e1);
   s->timing = was_timing;
   if (pick.path == SC_PATH_TILE) sc_set_tiling(s, pick.tile, pick.block);
-  else sc_set_list_block(s, pick.block);
+  else sc_set_list_block(s, pick.block, pick.lanes);
   sc_set_path(s, pick.path);
   int e = check_errors(s);
   if (e != SC_OK) return e;
   if (path_out) *path_out = pick.path;
-  if (tile_cols_out) *tile_cols_out = pick.tile;
+  if (tile_cols_out) *tile_cols_out = pick.path == SC_PATH_TILE ? pick.tile : pick.lanes;
   if (block_out) *block_out = pick.block;
   if (us_out) *us_out = best;
   return SC_OK;

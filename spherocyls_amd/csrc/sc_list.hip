@@ -94,7 +94,72 @@ __device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double
   return n;
 }
 
-template <int MODE>
+// Force of partner q on particle i (i's data in registers), accumulated into
+// (Fx, Fy, Tq) and, for the lower-id side of the pair, into the stress/energy
+// sums.  The pair is evaluated in the reference's orientation (lower particle
+// id first, src/cpp/spherocyl_box.cpp:514-520) so both sides agree bitwise.
+template <bool SUMS>
+__device__ __forceinline__ void pair_into(double xi, double yi, double nxi, double nyi, double Ri,
+                                          double Ai, int idi, double xq, double yq, double nxq,
+                                          double nyq, double Rq, double Aq, int idq, double Lx,
+                                          double Ly, double gam, bool fp64, bool do_se,
+                                          double& Fx, double& Fy, double& Tq, double* acc) {
+#pragma clang fp contract(off)
+  const bool iP = idi < idq;
+  double ddx = min_image(iP ? xi - xq : xq - xi, Lx);
+  double ddy = min_image(iP ? yi - yq : yq - yi, Ly);
+  ddx += gam * ddy;
+  const double sig = Ri + Rq;
+  const double nxA = iP ? nxi : nxq, nyA = iP ? nyi : nyq;
+  const double nxB = iP ? nxq : nxi, nyB = iP ? nyq : nyi;
+  const double Ap = iP ? Ai : Aq, Aqq = iP ? Aq : Ai;
+  PairOut<double> r;
+  bool hit;
+  if (fp64) {
+    hit = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, r);
+  } else {
+    PairOut<float> rf;
+    hit = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB, (float)nyB,
+                           (float)Ap, (float)Aqq, (float)sig, rf);
+    r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
+    r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
+  }
+  if (!hit) return;
+  if (iP) {
+    Fx += r.fx;
+    Fy += r.fy;
+    Tq += r.tp;
+    if (SUMS && do_se) {
+      // contact arms, src/cpp/spherocyl_box.cpp:483-486
+      double cxa = r.s * nxA - 0.5 * r.dx;
+      double cya = r.s * nyA - 0.5 * r.dy;
+      double cxb = r.t * nxB + 0.5 * r.dx;
+      double cyb_ref = r.t * nxB + 0.5 * r.dy;
+      double cyb = r.t * nyB + 0.5 * r.dy;
+      acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
+      acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
+      acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
+      acc[kSePxy] += r.fy * cxa;
+      acc[kSeCPxx] += r.fx * (cxa - cxb);
+      acc[kSeCPyy] += r.fy * (cya - cyb);
+      acc[kSeCPxy] += r.fy * (cxa - cxb);
+      acc[kSeCPyx] += r.fx * (cya - cyb);
+      acc[kSeNc] += 1.0;
+    }
+  } else {
+    Fx -= r.fx;
+    Fy -= r.fy;
+    Tq += r.tq;
+  }
+}
+
+// LANES threads share a particle: lane l takes list entries l, l + LANES, ...
+// in batches of kGather whose loads are all issued before any is used, so a
+// lane has several independent gathers in flight.  The LANES partial sums are
+// combined with xor shuffles (fixed order for a given LANES).
+constexpr int kGather = 4;
+
+template <int MODE, int LANES>
 __global__ void __launch_bounds__(256) k_list(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double s_red[4 * kNSum];
@@ -110,13 +175,17 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   const PackInfo P = a.packs[pk];
   const PackScal S = a.scal[pk];
   if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) return;
-  const int g = pb.y + tid;
+  const int lane = tid % LANES;
+  const int g = pb.y + tid / LANES;
   const bool active = g < P.poff + P.n;
 
+  // Only the plain force pass reduces (stress/energy records, FIRE/GD sums,
+  // residual); the Heun passes carry no accumulators, which keeps their
+  // register footprint and occupancy where the hot loop needs it.
+  constexpr bool kSums = MODE == kModeForces;
   const bool do_se =
-      (MODE != kModeCorr) &&
-      (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
-  const bool do_sum = do_se || (MODE == kModeForces && (a.do_fire || a.do_gd || a.do_resid));
+      kSums && (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
+  const bool do_sum = kSums && (do_se || a.do_fire || a.do_gd || a.do_resid);
 
   const double dt = P.dt, gd = P.strain_rate;
   double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
@@ -134,9 +203,9 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   }
   const bool fp64 = S.prec == kFP64;
 
-  double acc[kNSum];
+  double acc[kSums ? kNSum : 1];
 #pragma unroll
-  for (int i = 0; i < kNSum; ++i) acc[i] = 0.0;
+  for (int i = 0; i < (kSums ? kNSum : 1); ++i) acc[i] = 0.0;
 
   if (active) {
     const double xi = a.ex[g], yi = a.ey[g];
@@ -146,101 +215,82 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
     const double reach_i = Ri + Ai;
 
     // ---- list (rebuilt here when the previous pass asked for it)
+    // Every lane of the particle builds the same row (identical stores), so a
+    // lane only ever reads back what it wrote itself; rebuilds are rare.
     const bool rebuild = a.flag_in[pk] != 0;
     double center, xbi, ybi;
     int n;
     if (rebuild) {
       center = gam;
-      n = build_row(a, P, Lx, Ly, g, xi, yi, reach_i, gam - a.gamma_window,
-                    gam + a.gamma_window);
       xbi = xi;
       ybi = yi;
-      a.xb[g] = xi;
-      a.yb[g] = yi;
-      if (g == P.poff) a.list_center[pk] = gam;
+      n = build_row(a, P, Lx, Ly, g, xi, yi, reach_i, gam - a.gamma_window, gam + a.gamma_window);
+      if (lane == 0) {
+        a.xb[g] = xi;
+        a.yb[g] = yi;
+        if (g == P.poff) a.list_center[pk] = gam;
+      }
     } else {
-      center = a.list_center[pk];
       n = a.nbr_n[g];
+      center = a.list_center[pk];
       xbi = a.xb[g];
       ybi = a.yb[g];
     }
 
     // ---- forces over the row
     double Fx = 0.0, Fy = 0.0, Tq = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int q = a.nbr[(size_t)k * a.ntot + g];
-      const double xq = a.ex[q], yq = a.ey[q];
-      const double Rq = a.b.R[q], Aq = a.b.A[q];
-      double dx = min_image(xi - xq, Lx);
-      double dy = min_image(yi - yq, Ly);
-      dx += gam * dy;
-      const double reach = reach_i + Rq + Aq;
-      if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
-      const int idq = a.b.id[q];
-      const double nxq = a.enx[q], nyq = a.eny[q];
-      // the reference's orientation: lower particle id first
-      const bool iP = idi < idq;
-      double ddx = min_image(iP ? xi - xq : xq - xi, Lx);
-      double ddy = min_image(iP ? yi - yq : yq - yi, Ly);
-      ddx += gam * ddy;
-      const double sig = Ri + Rq;
-      const double nxA = iP ? nxi : nxq, nyA = iP ? nyi : nyq;
-      const double nxB = iP ? nxq : nxi, nyB = iP ? nyq : nyi;
-      const double Ap = iP ? Ai : Aq, Aqq = iP ? Aq : Ai;
-      PairOut<double> r;
-      bool hit;
-      if (fp64) {
-        hit = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, r);
-      } else {
-        PairOut<float> rf;
-        hit = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
-                               (float)nyB, (float)Ap, (float)Aqq, (float)sig, rf);
-        r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
-        r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
+    for (int k0 = lane; k0 < n; k0 += LANES * kGather) {
+      int qs[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) {
+        const int k = k0 + j * LANES;
+        qs[j] = k < n ? a.nbr[(size_t)k * a.ntot + g] : g;
       }
-      if (!hit) continue;
-      if (iP) {
-        Fx += r.fx;
-        Fy += r.fy;
-        Tq += r.tp;
-        if (do_se) {
-          double cxa = r.s * nxA - 0.5 * r.dx;
-          double cya = r.s * nyA - 0.5 * r.dy;
-          double cxb = r.t * nxB + 0.5 * r.dx;
-          double cyb_ref = r.t * nxB + 0.5 * r.dy;
-          double cyb = r.t * nyB + 0.5 * r.dy;
-          acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
-          acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
-          acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
-          acc[kSePxy] += r.fy * cxa;
-          acc[kSeCPxx] += r.fx * (cxa - cxb);
-          acc[kSeCPyy] += r.fy * (cya - cyb);
-          acc[kSeCPxy] += r.fy * (cxa - cxb);
-          acc[kSeCPyx] += r.fx * (cya - cyb);
-          acc[kSeNc] += 1.0;
-        }
-      } else {
-        Fx -= r.fx;
-        Fy -= r.fy;
-        Tq += r.tq;
+      double xq[kGather], yq[kGather], Rq[kGather], Aq[kGather];
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) {
+        xq[j] = a.ex[qs[j]];
+        yq[j] = a.ey[qs[j]];
+        Rq[j] = a.b.R[qs[j]];
+        Aq[j] = a.b.A[qs[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < kGather; ++j) {
+        if (k0 + j * LANES >= n) break;
+        double dx = min_image(xi - xq[j], Lx);
+        double dy = min_image(yi - yq[j], Ly);
+        dx += gam * dy;
+        const double reach = reach_i + Rq[j] + Aq[j];
+        if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
+        const int q = qs[j];
+        pair_into<kSums>(xi, yi, nxi, nyi, Ri, Ai, idi, xq[j], yq[j], a.enx[q], a.eny[q], Rq[j],
+                         Aq[j], a.b.id[q], Lx, Ly, gam, fp64, do_se, Fx, Fy, Tq, acc);
       }
     }
-
-    // ---- per-particle epilogue
+    if (LANES > 1) {
+#pragma unroll
+      for (int off = LANES >> 1; off > 0; off >>= 1) {
+        Fx += __shfl_xor(Fx, off);
+        Fy += __shfl_xor(Fy, off);
+        Tq += __shfl_xor(Tq, off);
+      }
+    }
+    // ---- per-particle epilogue (lane 0)
     const double half_skin = 0.5 * P.pad;
     bool raise = false;
-    if (MODE == kModeForces) {
+    if (lane != 0) {
+    } else if (MODE == kModeForces) {
       a.b.fx[g] = Fx;
       a.b.fy[g] = Fy;
       a.b.tau[g] = Tq;
-      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
-      if (a.do_fire) {
+      if (kSums && do_sum) acc[kSumR2 % (kSums ? kNSum : 1)] += Fx * Fx + Fy * Fy + Tq * Tq;
+      if (kSums && a.do_fire) {
         double sq = sqrt(a.b.I[g]);
         double gp = Tq / sq;
         double vx = a.b.vx[g], vy = a.b.vy[g], vp = a.b.vpsi[g];
-        acc[kSumP] += Fx * vx + Fy * vy + gp * vp;
-        acc[kSumG] += Fx * Fx + Fy * Fy + gp * gp;
-        acc[kSumV] += vx * vx + vy * vy + vp * vp;
+        acc[kSumP % (kSums ? kNSum : 1)] += Fx * vx + Fy * vy + gp * vp;
+        acc[kSumG % (kSums ? kNSum : 1)] += Fx * Fx + Fy * Fy + gp * gp;
+        acc[kSumV % (kSums ? kNSum : 1)] += vx * vx + vy * vy + vp * vp;
       }
     } else if (MODE == kModePred) {
       const double p0 = a.b.phi[g];
@@ -290,7 +340,6 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
               fabs(min_image(qy - ybi, Lby)) >= half_skin ||
               fabs(gnext - center) >= a.gamma_window || a.resize;
-      if (do_sum) acc[kSumR2] += Fx * Fx + Fy * Fy + Tq * Tq;
     } else {  // kModeCorr
       double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
       double Lbx = Lx, Lby = Ly;
@@ -351,7 +400,7 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   }
 
   // ---- scalar advance (corrector: one thread per packing)
-  if (MODE == kModeCorr && g == P.poff) {
+  if (MODE == kModeCorr && g == P.poff && lane == 0) {
     PackScal S2 = S;
     S2.step = S.step + 1;
     if (!a.resize) {
@@ -368,14 +417,22 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
     a.scal_next[pk] = S2;
   }
 
-  if (!do_sum) return;
-  block_sum<kNSum>(acc, s_red, nthr);
-  if (tid < kNSum) a.part[(size_t)blockIdx.x * kNSum + tid] = s_red[tid];
+  if constexpr (kSums) {
+    if (!do_sum) return;
+    block_sum<kNSum>(acc, s_red, nthr);
+    if (tid < kNSum) a.part[(size_t)blockIdx.x * kNSum + tid] = s_red[tid];
+  }
 }
 
-template __global__ void k_list<kModeForces>(ListArgs);
-template __global__ void k_list<kModePred>(ListArgs);
-template __global__ void k_list<kModeCorr>(ListArgs);
+#define SC_LIST_INST(L)                                       \
+  template __global__ void k_list<kModeForces, L>(ListArgs); \
+  template __global__ void k_list<kModePred, L>(ListArgs);   \
+  template __global__ void k_list<kModeCorr, L>(ListArgs);
+SC_LIST_INST(1)
+SC_LIST_INST(2)
+SC_LIST_INST(4)
+SC_LIST_INST(8)
+#undef SC_LIST_INST
 
 // (nx, ny) = A (cos phi, sin phi) for every particle of the bank.
 __global__ void k_segvec(Bank b, int ntot) {

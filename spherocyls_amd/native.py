@@ -63,7 +63,7 @@ def load():
         "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_set_path": ([sysp, C.c_int], C.c_int),
         "sc_get_path": ([sysp, _ip], C.c_int),
-        "sc_set_list_block": ([sysp, C.c_int], C.c_int),
+        "sc_set_list_block": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_autotune": ([sysp, C.c_int, _ip, _ip, _ip, _dp], C.c_int),
         "sc_get_tiling": ([sysp, _ip, _ip], C.c_int),
         "sc_set_positions": ([sysp, _dp, _dp, _dp], C.c_int),
@@ -190,15 +190,15 @@ class System:
         self._ck(self.lib.sc_get_path(self.h, C.byref(v)))
         return ("tile", "list")[v.value]
 
-    def set_list_block(self, block):
-        self._ck(self.lib.sc_set_list_block(self.h, block))
+    def set_list_block(self, block, lanes=1):
+        self._ck(self.lib.sc_set_list_block(self.h, block, lanes))
 
     def autotune(self, reps=5):
         p, t, b, us = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         self._ck(self.lib.sc_autotune(self.h, reps, C.byref(p), C.byref(t), C.byref(b),
                                       C.byref(us)))
-        return dict(path=("tile", "list")[p.value], tile_cols=t.value, block=b.value,
-                    us_per_pass=us.value)
+        path = ("tile", "list")[p.value]
+        return dict(path=path, shape=t.value, block=b.value, us_per_pass=us.value)
 
     def tiling(self):
         a, b = C.c_int(), C.c_int()

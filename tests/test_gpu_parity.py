@@ -365,7 +365,7 @@ def test_list_and_tile_paths_agree_and_autotune():
         s.set_path("list")
         ref = None
         for blk in (64, 128, 256):
-            s.set_list_block(blk)
+            s.set_list_block(blk, 1)
             s.calc_forces()
             f = s.forces()
             if ref is None:
@@ -374,6 +374,10 @@ def test_list_and_tile_paths_agree_and_autotune():
                 np.testing.assert_array_equal(a, b)
         ob = oracle_at(pk, -0.3, 0.1)
         sc = ob.force_scale()
+        for lanes in (2, 4, 8):  # other summation trees: equal to rounding
+            s.set_list_block(128, lanes)
+            s.calc_forces()
+            assert_forces(s.forces(), ref, sc, rel=1e-14)
         assert_forces(ref, ft, sc, rel=1e-14)
         choice = s.autotune(reps=3)
         assert choice["path"] in ("list", "tile") and choice["us_per_pass"] > 0
