@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0
 #   corrector: read x~ (24) + R,A (16) + v0 (24) + x (24), write x (24) + moved (32) = 144
 BYTES_PRED = 88
 BYTES_CORR = 144
+# HBM bytes per corrector launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+# scaled by a plain copy's readings (profiles/k_list_heun_shear_N262144.md):
+# (workload, path, precision) -> bytes
+MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (25833.5 * 2.0 + 28460.3) * 1024}
 
 WORKLOADS = {
     "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
@@ -77,6 +81,21 @@ def cpu_baseline(pk, strain_rate, dt, budget_s=15.0):
                       f"({cpu_model()}, nproc={os.cpu_count()})"}
 
 
+def copy_calibration(mib):
+    """Plain device copy of a known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings give the factor that turns those counters into HBM bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +108,12 @@ def main():
     ap.add_argument("--lanes", type=int, default=1, help="list path: lanes per particle")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(packings and slabs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -102,7 +126,7 @@ def main():
     if args.precision == "mixed":
         s.set_precision("mixed")
     if args.path == "auto":
-        tuned = s.autotune(reps=5)  # the project's autotuner: path and launch shape
+        tuned = s.autotune(reps=10)  # the project's autotuner: path and launch shape
     else:
         s.set_path(args.path)
         if args.path == "tile":
@@ -151,7 +175,8 @@ def main():
                    ("tile_cols" if path == "tile" else "lanes_per_particle"): shape,
                    "block_threads": block, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, path, args.precision)),
                      "kernel": f"k_{path}<corrector>", "avg_us": corr_avg * 1e6,
                      "predictor_avg_us": pred_avg * 1e6,
                      "bytes_per_particle": BYTES_CORR},

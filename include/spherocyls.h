@@ -5,7 +5,7 @@
  * The reference (tmarschall/spherocyls) has no FFI: its boundary is the C++ class
  * SpherocylBox (src/cpp/spherocyl_box.h:85-120), whose methods the entry points
  * below replace one by one (each cites the method it stands for).  The C++ host
- * class in spherocyls_amd/host/spherocyl_box.h keeps that class's names and
+ * class in include/spherocyl_box.h keeps that class's names and
  * semantics on top of this ABI.
  *
  * Conventions
@@ -154,8 +154,21 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done);
 int sc_affine_resize(sc_system* s, const double* scale);
 
 /* FIRE minimisation of every packing to F_rms < ftol (no reference counterpart;
- * specification in oracle/sc_oracle.cpp).  params[10]:
- *   dt0 dt_max n_min f_inc f_dec alpha0 f_alpha max_move ftol max_iter
+ * the reference's own minimisers are gradient descent, relaxation_dynamics.cpp,
+ * and the GPU-side conjugate gradient of src/cuda/energy_minimize.cu:1196-1314,
+ * whose F_rms convention :1277 is used here).  FIRE of Bitzek et al., PRL 97,
+ * 170201 (2006), with the half-step retreat of FIRE 2.0 (Guenole et al., Comput.
+ * Mater. Sci. 175, 109584 (2020)), in generalised coordinates (x, y,
+ * psi = phi sqrt(I)) with generalised force g = (Fx, Fy, tau/sqrt(I)).  One
+ * iteration per packing, all sums in FP64:
+ *   1. forces at x; stop if sqrt(sum(Fx^2+Fy^2+tau^2)/(3N)) < ftol
+ *   2. P = g.v, G = g.g, V = v.v
+ *   3. P >= 0: v <- (1-alpha) v + alpha sqrt(V/G) g; npos += 1;
+ *              if npos > n_min: dt <- min(f_inc dt, dt_max), alpha <- f_alpha alpha
+ *      else:   x <- x - dt v / 2; v <- 0; dt <- f_dec dt; alpha <- alpha0; npos <- 0
+ *   4. v <- v + dt g; displacement dt v with its (x, y) part shortened to
+ *      max_move if longer; x <- x + displacement
+ * params[10]: dt0 dt_max n_min f_inc f_dec alpha0 f_alpha max_move ftol max_iter.
  * In SC_PREC_MIXED mode the pair math runs in FP32 until F_rms < switch_frms,
  * then in FP64 to the end.  iters[npack] and frms[npack] may be NULL. */
 int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int* iters,
@@ -183,10 +196,25 @@ int sc_timing_reset(sc_system* s);
 int sc_synchronize(sc_system* s);
 
 /* Seeded random placement (counterpart of SpherocylBox::random_configuration,
- * src/cpp/spherocyl_box.cpp:694-790, RANDOM mode): uniform centres in [0,L)^2
- * (sheared coordinates at strain gamma) and angles in [0,pi), redrawing any
- * particle whose segment comes within sqrt(1e-5)(R_p+R_q) of an earlier one
- * under the Lees-Edwards pair separation.  Host only; needs no GPU. */
+ * src/cpp/spherocyl_box.cpp:694-790, and its initialConfig modes,
+ * src/cpp/spherocyl_box.h:8 plus the grid modes 4/5 of :718-737): centres
+ * uniform in [0,L)^2 (sheared coordinates at strain gamma; grid modes: one per
+ * site of a rows x cols grid in the particle aspect ratio) and angles uniform in
+ * [0,pi) (ALIGNED modes: 0).  RANDOM modes redraw a particle whose segment comes
+ * within sqrt(1e-5)(R_p+R_q) of an earlier one under the Lees-Edwards pair
+ * separation; ZERO_E and GRID modes redraw any contact.  Host only; needs no
+ * GPU.  SC_ERR_PLACEMENT after max_tries draws for one particle, or when the
+ * grid sites are too small for the particles. */
+#define SC_CONFIG_RANDOM 0
+#define SC_CONFIG_RANDOM_ALIGNED 1
+#define SC_CONFIG_ZERO_E 2
+#define SC_CONFIG_ZERO_E_ALIGNED 3
+#define SC_CONFIG_GRID 4
+#define SC_CONFIG_GRID_ALIGNED 5
+int sc_random_configuration(int n, double L, double gamma, const double* A, const double* R,
+                            int config, unsigned long long seed, int max_tries, double* x,
+                            double* y, double* phi);
+/* sc_random_configuration in RANDOM mode. */
 int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
                       unsigned long long seed, int max_tries, double* x, double* y, double* phi);
 

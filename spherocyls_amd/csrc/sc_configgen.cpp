@@ -1,14 +1,17 @@
-// Seeded random packing generator (host code, C ABI).
+// Seeded random configuration generator (host code, C ABI).
 //
 // Counterpart of SpherocylBox::random_configuration (src/cpp/spherocyl_box.cpp:694-790):
 // particles are placed one after another at uniform random centres and angles,
-// and a placement is redrawn while it "crosses" an already placed particle, i.e.
-// while the squared closest distance between the two segments is below
-// 1e-5 * (R_p + R_q)^2 or is NaN (the reference's RANDOM mode, check_particle_cross
-// at :597-645).  Overlaps are allowed, exact crossings are not: a crossing gives a
-// zero-length contact vector, whose force direction is undefined (0/0 in the
-// pair force).  Unlike the reference (srand(time(0)), :739) the stream is seeded.
+// and a placement is redrawn while it is rejected against an already placed
+// particle.  RANDOM / RANDOM_ALIGNED reject a "crossing", i.e. a squared closest
+// segment distance below 1e-5 (R_p + R_q)^2 or NaN (check_particle_cross,
+// :597-645): overlaps are allowed, exact crossings are not, since a crossing
+// gives a zero-length contact vector whose force direction is undefined (0/0 in
+// the pair force).  ZERO_E modes and the grid modes reject any contact
+// (check_particle_contact, :547-595).  ALIGNED modes set every angle to 0.
+// Unlike the reference (srand(time(0)), :739) the stream is seeded.
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -71,49 +74,78 @@ double seg_dist2(double xp, double yp, double pp, double Ap, double xq, double y
 
 }  // namespace
 
-extern "C" int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
-                                 unsigned long long seed, int max_tries, double* x, double* y,
-                                 double* phi) {
-  if (n <= 0 || !(L > 0) || !(std::fabs(gamma) <= 0.5) || !A || !R || !x || !y || !phi)
+extern "C" int sc_random_configuration(int n, double L, double gamma, const double* A,
+                                       const double* R, int config, unsigned long long seed,
+                                       int max_tries, double* x, double* y, double* phi) {
+  if (n <= 0 || !(L > 0) || !(std::fabs(gamma) <= 0.5) || !A || !R || !x || !y || !phi ||
+      config < SC_CONFIG_RANDOM || config > SC_CONFIG_GRID_ALIGNED || max_tries < 1)
     return SC_ERR_ARG;
-  double reach = 0;
+  double reach = 0, amax = 0, rmax = 0;
   for (int i = 0; i < n; ++i) {
     if (!(R[i] > 0) || !(A[i] >= 0)) return SC_ERR_ARG;
     reach = std::fmax(reach, 2 * (R[i] + A[i]));
+    amax = std::fmax(amax, A[i]);
+    rmax = std::fmax(rmax, R[i]);
   }
-  // Uniform grid in sheared coordinates; with |gamma| <= 0.5 a pair within one
-  // reach (lab frame) is at most 1.12 reach apart in sheared x.
+  const bool aligned = config % 2 == 1;
+  const bool no_contact = config >= SC_CONFIG_ZERO_E;  // zero-energy and grid modes
+  const bool on_grid = config >= SC_CONFIG_GRID;
+  // Grid modes (reference config > 3): one particle per rectangular site, rows
+  // and columns in the ratio of the particle's aspect (A+R)/R, each centre
+  // drawn inside its site away from the site edges by (A+R, R).
+  int grows = 1, gcols = 1;
+  double gw = 0, gh = 0, gxo = amax + rmax, gyo = rmax;
+  if (on_grid) {
+    double aspect = (amax + rmax) / rmax;
+    double dcols = std::sqrt(n / aspect);
+    grows = (int)(aspect * dcols);
+    gcols = (int)dcols;
+    while ((long long)grows * gcols < n) ++grows;  // every particle gets a site
+    gw = L / gcols;
+    gh = L / grows;
+    if (gw < 2 * (amax + rmax) || gh < 2 * rmax) return SC_ERR_PLACEMENT;
+  }
+  // Uniform search grid in sheared coordinates; with |gamma| <= 0.5 a pair within
+  // one reach (lab frame) is at most 1.12 reach apart in sheared x.
   int nc = std::max(1, (int)(L / (1.12 * reach)));
   double w = L / nc;
   std::vector<std::vector<int>> grid((size_t)nc * nc);
   Rng rng(seed);
   const double kPi = 3.14159265358979323846;
+  auto rejects = [&](int p, double px, double py, double pp, int q) {
+    double sig = R[p] + R[q];
+    double d2 = seg_dist2(px, py, pp, A[p], x[q], y[q], phi[q], A[q], L, gamma);
+    double lim = no_contact ? sig * sig : 1e-5 * sig * sig;
+    return d2 < lim || d2 != d2;
+  };
   for (int p = 0; p < n; ++p) {
     int tries = 0;
     for (;;) {
       if (++tries > max_tries) return SC_ERR_PLACEMENT;
-      double px = L * rng.uniform(), py = L * rng.uniform(), pp = kPi * rng.uniform();
+      double px, py;
+      if (on_grid) {
+        int gr = p / gcols, gc = p % gcols;
+        px = gc * gw + gxo + (gw - 2 * gxo) * rng.uniform();
+        py = gr * gh + gyo + (gh - 2 * gyo) * rng.uniform();
+      } else {
+        px = L * rng.uniform();
+        py = L * rng.uniform();
+      }
+      double pp = aligned ? 0.0 : kPi * rng.uniform();
       int cx = std::min(nc - 1, (int)(px / w)), cy = std::min(nc - 1, (int)(py / w));
       bool bad = false;
-      int span = nc < 3 ? 0 : 1;
-      for (int oy = -span; oy <= span && !bad; ++oy)
-        for (int ox = -span; ox <= span && !bad; ++ox) {
-          int gx = (cx + ox + nc) % nc, gy = (cy + oy + nc) % nc;
-          for (int q : grid[(size_t)gy * nc + gx]) {
-            double sig = R[p] + R[q];
-            double d2 = seg_dist2(px, py, pp, A[p], x[q], y[q], phi[q], A[q], L, gamma);
-            if (d2 < 1e-5 * sig * sig || d2 != d2) {
-              bad = true;
-              break;
-            }
+      if (nc < 3) {  // small box: compare with everybody
+        for (int q = 0; q < p && !bad; ++q) bad = rejects(p, px, py, pp, q);
+      } else {
+        for (int oy = -1; oy <= 1 && !bad; ++oy)
+          for (int ox = -1; ox <= 1 && !bad; ++ox) {
+            int gx = (cx + ox + nc) % nc, gy = (cy + oy + nc) % nc;
+            for (int q : grid[(size_t)gy * nc + gx])
+              if (rejects(p, px, py, pp, q)) {
+                bad = true;
+                break;
+              }
           }
-        }
-      if (nc < 3 && !bad) {  // small box: compare with everybody
-        for (int q = 0; q < p && !bad; ++q) {
-          double sig = R[p] + R[q];
-          double d2 = seg_dist2(px, py, pp, A[p], x[q], y[q], phi[q], A[q], L, gamma);
-          if (d2 < 1e-5 * sig * sig || d2 != d2) bad = true;
-        }
       }
       if (bad) continue;
       x[p] = px;
@@ -124,4 +156,10 @@ extern "C" int sc_random_packing(int n, double L, double gamma, const double* A,
     }
   }
   return SC_OK;
+}
+
+extern "C" int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
+                                 unsigned long long seed, int max_tries, double* x, double* y,
+                                 double* phi) {
+  return sc_random_configuration(n, L, gamma, A, R, SC_CONFIG_RANDOM, seed, max_tries, x, y, phi);
 }

@@ -1,0 +1,867 @@
+// SpherocylBox on the GPU: the reference's box protocol (file output, cadences,
+// restart) on the host, every force pass and step through the C ABI.
+//
+// Protocol loops follow the reference's, including its output quirks:
+//   - run_strain writes an SE line after every nSESaveT-th sub-step of an outer
+//     step and again at the end of every outer step t with t*nIntSteps a
+//     multiple of nSESaveT, repeating the last computed values
+//     (src/cpp/strain_dynamics.cpp:285-299); resize_box likewise
+//     (src/cpp/resize_dynamics.cpp:293-306, 378-389);
+//   - sub-step counters restart at 1 in every outer step, so an SE cadence that
+//     does not divide the sub-steps per unit time restarts each outer step;
+//   - the SE-save and position-save cadences are rounded to whole sub-steps /
+//     outer steps with int(x + 0.5).
+// Cadences that round to zero make the reference divide by zero; here they
+// throw.
+#include "spherocyl_box.h"
+#include "spherocyl_files.h"
+
+#include <cstring>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <sstream>
+
+namespace {
+
+// the reference's truncated pi (src/cpp/spherocyl_box.cpp:12)
+constexpr double kRefPi = 3.14159265358979;
+// SE records held per batched GPU call
+constexpr long kMaxRecordsPerCall = 65536;
+
+std::string time_name(const char* prefix, unsigned long t) {
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%s%010lu.dat", prefix, t);
+  return buf;
+}
+
+// Positions the stream's write pointer just after the line whose leading
+// integer equals t (restart of run_strain, src/cpp/strain_dynamics.cpp:250-279).
+// Returns false when no such line exists.
+bool seek_after_time(std::fstream& f, unsigned long t) {
+  std::string line;
+  while (std::getline(f, line)) {
+    std::size_t sp = line.find(' ');
+    if (std::strtoul(line.substr(0, sp).c_str(), nullptr, 10) == t && !line.empty()) {
+      std::streampos pos = f.tellg();
+      f.clear();
+      if (pos == std::streampos(-1)) {  // last line had no newline: append one
+        f.seekp(0, std::ios::end);
+        f << "\n";
+      } else {
+        f.seekp(pos);
+      }
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace
+
+// "t v0 v1 ...\n" with the stream's own precision
+static void se_line(std::ostream& out, unsigned long t, int nvals, const double* vals);
+
+// ---------------------------------------------------------------- construction
+
+SpherocylBox::SpherocylBox() {}
+
+SpherocylBox::SpherocylBox(double dL, int nParticles) { init(dL, nParticles, 0.0); }
+
+SpherocylBox::SpherocylBox(double dL, int nParticles, double dPadding) {
+  init(dL, nParticles, dPadding);
+}
+
+SpherocylBox::SpherocylBox(double dL, int nParticles, double* pdX, double* pdY, double* pdPhi,
+                           double dA, double dR)
+    : SpherocylBox(dL, nParticles, pdX, pdY, pdPhi, dA, dR, 0.0) {}
+
+SpherocylBox::SpherocylBox(double dL, int nParticles, double* pdX, double* pdY, double* pdPhi,
+                           double dA, double dR, double dPadding) {
+  init(dL, nParticles, dPadding);
+  for (int p = 0; p < nParticles; ++p) set_particle(p, pdX[p], pdY[p], pdPhi[p], dA, dR);
+  m_dPacking = packing_from_arrays();
+}
+
+SpherocylBox::SpherocylBox(double dL, int nParticles, double* pdX, double* pdY, double* pdPhi,
+                           double* pdA, double* pdR)
+    : SpherocylBox(dL, nParticles, pdX, pdY, pdPhi, pdA, pdR, 0.0) {}
+
+SpherocylBox::SpherocylBox(double dL, int nParticles, double* pdX, double* pdY, double* pdPhi,
+                           double* pdA, double* pdR, double dPadding) {
+  init(dL, nParticles, dPadding);
+  for (int p = 0; p < nParticles; ++p) set_particle(p, pdX[p], pdY[p], pdPhi[p], pdA[p], pdR[p]);
+  m_dPacking = packing_from_arrays();
+}
+
+SpherocylBox::~SpherocylBox() {
+  if (m_sys) sc_system_destroy(m_sys);
+}
+
+void SpherocylBox::init(double dL, int nParticles, double dPadding) {
+  if (!(dL > 0) || nParticles < 0 || !(dPadding >= 0))
+    throw SpherocylError(SC_ERR_ARG, "SpherocylBox: need L > 0, N >= 0, padding >= 0");
+  m_nParticles = nParticles;
+  m_dLx = m_dLy = dL;
+  m_dPadding = dPadding;
+  m_vX.assign(nParticles, 0.0);
+  m_vY.assign(nParticles, 0.0);
+  m_vPhi.assign(nParticles, 0.0);
+  m_vR.assign(nParticles, 0.5);
+  m_vA.assign(nParticles, 0.0);
+  m_bGeomStale = true;
+}
+
+void SpherocylBox::check(int status, const char* where) {
+  if (status == SC_OK) return;
+  std::string msg = std::string(where) + ": " + sc_status_string(status);
+  if (m_sys) {
+    const char* e = sc_last_error(m_sys);
+    if (e && *e) msg += std::string(" (") + e + ")";
+  }
+  throw SpherocylError(status, msg);
+}
+
+double SpherocylBox::packing_from_arrays() const {
+  // calculate_packing_fraction, src/cpp/spherocyl_box.cpp:357-366
+  double filled = 0.0;
+  for (int p = 0; p < m_nParticles; ++p) filled += (kRefPi * m_vR[p] + 4.0 * m_vA[p]) * m_vR[p];
+  return filled / (m_dLx * m_dLy);
+}
+
+void SpherocylBox::set_particle(int p, double dX, double dY, double dPhi, double dA, double dR) {
+  if (p < 0 || p >= m_nParticles || !(dA >= 0.0) || !(dR > 0.0))
+    throw SpherocylError(SC_ERR_ARG, "set_particle: index out of range or A < 0 or R <= 0");
+  sync_host();
+  // wrapping as src/cpp/spherocyl_box.cpp:204-215
+  while (dX > m_dLx) dX -= m_dLx;
+  while (dX < 0.0) dX += m_dLx;
+  while (dY > m_dLy) dY -= m_dLy;
+  while (dY < 0.0) dY += m_dLy;
+  while (dPhi > kRefPi) dPhi -= 2 * kRefPi;
+  while (dPhi < -kRefPi) dPhi += 2 * kRefPi;
+  m_vX[p] = dX;
+  m_vY[p] = dY;
+  m_vPhi[p] = dPhi;
+  if (m_vA[p] != dA || m_vR[p] != dR) m_bGeomStale = true;
+  m_vA[p] = dA;
+  m_vR[p] = dR;
+  m_bDevStale = true;
+}
+
+void SpherocylBox::set_cell_padding(double dPadding) {
+  if (!(dPadding >= 0)) throw SpherocylError(SC_ERR_ARG, "set_cell_padding: padding < 0");
+  sync_host();
+  m_dPadding = dPadding;
+  m_bGeomStale = true;
+}
+
+void SpherocylBox::set_gamma(double dGamma) {
+  sync_host();
+  m_dGamma = dGamma;
+  m_bDevStale = true;
+}
+
+void SpherocylBox::set_total_gamma(double dTotalGamma) {
+  sync_host();
+  m_dGammaTotal = dTotalGamma;
+  m_bDevStale = true;
+}
+
+void SpherocylBox::set_precision(bool bMixed) {
+  m_nPrecision = bMixed ? SC_PREC_MIXED : SC_PREC_FP64;
+  if (m_sys) check(sc_set_precision(m_sys, m_nPrecision), "set_precision");
+}
+
+// ------------------------------------------------------------ device state
+
+void SpherocylBox::ensure_device() {
+  if (m_nParticles <= 0) throw SpherocylError(SC_ERR_STATE, "box has no particles");
+  if (!m_sys || m_bGeomStale) {
+    sync_host();
+    if (m_sys) {
+      sc_system_destroy(m_sys);
+      m_sys = nullptr;
+    }
+    int offs[2] = {0, m_nParticles};
+    double L = m_dLx, pad = m_dPadding;
+    sc_system* s = nullptr;
+    int st = sc_system_create(&s, m_nDevice, 1, offs, &L, m_vX.data(), m_vY.data(), m_vPhi.data(),
+                              m_vA.data(), m_vR.data(), &pad);
+    if (st != SC_OK) {
+      std::string msg = std::string("SpherocylBox: ") + sc_status_string(st);
+      if (s) {
+        msg += std::string(" (") + sc_last_error(s) + ")";
+        sc_system_destroy(s);
+      }
+      throw SpherocylError(st, msg);
+    }
+    m_sys = s;
+    check(sc_set_precision(m_sys, m_nPrecision), "set_precision");
+    m_bGeomStale = false;
+    m_bDevStale = true;
+  }
+  if (m_bDevStale) {
+    check(sc_set_positions(m_sys, m_vX.data(), m_vY.data(), m_vPhi.data()), "set_positions");
+    check(sc_set_gamma(m_sys, 0, m_dGamma, m_dGammaTotal), "set_gamma");
+    m_bDevStale = false;
+  }
+  check(sc_set_dynamics(m_sys, 0, m_dStrainRate, m_dResizeRate, m_dStep), "set_dynamics");
+}
+
+void SpherocylBox::read_scalars() {
+  double b[SC_BOX_NFIELDS];
+  check(sc_get_box(m_sys, 0, b), "get_box");
+  m_dLx = b[SC_BOX_LX];
+  m_dLy = b[SC_BOX_LY];
+  m_dGamma = b[SC_BOX_GAMMA];
+  m_dGammaTotal = b[SC_BOX_GAMMA_TOTAL];
+  m_dFrms = b[SC_BOX_FRMS];
+  m_nContacts = (int)b[SC_BOX_NCONTACTS];
+}
+
+void SpherocylBox::sync_host() {
+  if (!m_sys || !m_bHostStale) return;
+  check(sc_get_positions(m_sys, m_vX.data(), m_vY.data(), m_vPhi.data()), "get_positions");
+  read_scalars();
+  m_bHostStale = false;
+}
+
+sc_system* SpherocylBox::handle() {
+  ensure_device();
+  m_bHostStale = true;  // the caller may advance the system directly
+  return m_sys;
+}
+
+void SpherocylBox::configure_cells() { ensure_device(); }
+void SpherocylBox::find_cells() { ensure_device(); }
+void SpherocylBox::find_neighbors() { ensure_device(); }
+
+// ------------------------------------------------------------ force passes
+
+void SpherocylBox::calc_forces() {
+  ensure_device();
+  check(sc_calc_forces(m_sys), "calc_forces");
+  read_scalars();
+}
+
+void SpherocylBox::calc_se() {
+  ensure_device();
+  check(sc_calc_se(m_sys), "calc_se");
+  double b[SC_BOX_NFIELDS];
+  check(sc_get_box(m_sys, 0, b), "get_box");
+  m_dEnergy = b[SC_BOX_E];
+  m_dPxx = b[SC_BOX_PXX];
+  m_dPyy = b[SC_BOX_PYY];
+  m_dPxy = b[SC_BOX_PXY];
+  for (int i = 0; i < 4; ++i) m_dCorr[i] = b[SC_BOX_CPXX + i];
+  read_scalars();
+}
+
+void SpherocylBox::stress(double* pdPxx, double* pdPyy, double* pdPxy) const {
+  if (pdPxx) *pdPxx = m_dPxx;
+  if (pdPyy) *pdPyy = m_dPyy;
+  if (pdPxy) *pdPxy = m_dPxy;
+}
+
+void SpherocylBox::corrected_stress(double* pdP) const {
+  for (int i = 0; i < 4; ++i) pdP[i] = m_dCorr[i];
+}
+
+double SpherocylBox::residual_force() {
+  if (m_sys) read_scalars();
+  return m_dFrms;
+}
+
+double SpherocylBox::gamma() {
+  sync_host();
+  return m_dGamma;
+}
+
+double SpherocylBox::total_gamma() {
+  sync_host();
+  return m_dGammaTotal;
+}
+
+void SpherocylBox::positions(std::vector<double>& x, std::vector<double>& y,
+                             std::vector<double>& phi) {
+  sync_host();
+  x = m_vX;
+  y = m_vY;
+  phi = m_vPhi;
+}
+
+void SpherocylBox::forces(std::vector<double>& fx, std::vector<double>& fy,
+                          std::vector<double>& tau) {
+  ensure_device();
+  fx.resize(m_nParticles);
+  fy.resize(m_nParticles);
+  tau.resize(m_nParticles);
+  check(sc_get_forces(m_sys, fx.data(), fy.data(), tau.data()), "get_forces");
+}
+
+static std::vector<std::pair<int, int>> pairs_of(
+    sc_system* s, int (*fn)(sc_system*, int, int*, int*, int, int*)) {
+  int n = 0;
+  int st = fn(s, 0, nullptr, nullptr, 0, &n);
+  if (st != SC_OK) throw SpherocylError(st, sc_last_error(s));
+  std::vector<int> p(std::max(n, 1)), q(std::max(n, 1));
+  st = fn(s, 0, p.data(), q.data(), n, &n);
+  if (st != SC_OK) throw SpherocylError(st, sc_last_error(s));
+  std::vector<std::pair<int, int>> out(n);
+  for (int i = 0; i < n; ++i) out[i] = {p[i], q[i]};
+  return out;
+}
+
+std::vector<std::pair<int, int>> SpherocylBox::contacts() {
+  ensure_device();
+  return pairs_of(m_sys, sc_get_contacts);
+}
+
+std::vector<std::pair<int, int>> SpherocylBox::neighbors() {
+  ensure_device();
+  return pairs_of(m_sys, sc_get_neighbors);
+}
+
+// ------------------------------------------------------------ output
+
+void SpherocylBox::save_positions(std::string strFile) {
+  sync_host();
+  std::string path = m_strOutputDir + "/" + strFile;
+  if (sc_write_positions(path.c_str(), m_nParticles, m_dLx, m_dPacking, m_dGamma, m_dGammaTotal,
+                         m_dStrainRate, m_dStep, m_vX.data(), m_vY.data(), m_vPhi.data(),
+                         m_vR.data(), m_vA.data()) != SC_OK)
+    throw SpherocylError(SC_ERR_ARG, "save_positions: cannot write " + path);
+}
+
+void SpherocylBox::write_se_line(std::ostream& out, unsigned long t, bool bPacking) {
+  double v[5];
+  int k = 0;
+  if (bPacking) v[k++] = m_dPacking;
+  v[k++] = m_dEnergy;
+  v[k++] = m_dPxx;
+  v[k++] = m_dPyy;
+  v[k++] = m_dPxy;
+  se_line(out, t, k, v);
+}
+
+void SpherocylBox::display(bool bParticles, bool bCells) {
+  sync_host();
+  std::cout << "Dimensions: " << m_dLx << " x " << m_dLy << std::endl;
+  std::cout << "Particles: " << m_nParticles << std::endl;
+  std::cout << "Packing fraction: " << m_dPacking << std::endl;
+  std::cout << "Gamma: " << m_dGamma << " (" << m_dGammaTotal << ")" << std::endl;
+  std::cout << "Strain rate: " << m_dStrainRate << std::endl;
+  std::cout << "Integration step size: " << m_dStep << std::endl;
+  std::cout << "\nEnergy: " << m_dEnergy << std::endl;
+  std::cout << "Pxx: " << m_dPxx << std::endl;
+  std::cout << "Pyy: " << m_dPyy << std::endl;
+  std::cout << "Pxy: " << m_dPxy << std::endl;
+  if (bParticles && m_nParticles > 0) {
+    std::vector<double> fx, fy, tau;
+    forces(fx, fy, tau);
+    std::cout << "\nParticles: (x, y, phi), (A, R) - (Fx, Fy, Tau)" << std::endl;
+    for (int p = 0; p < m_nParticles; ++p)
+      std::cout << p << ": (" << m_vX[p] << ", " << m_vY[p] << ", " << m_vPhi[p] << "), ("
+                << m_vA[p] << ", " << m_vR[p] << ") - (" << fx[p] << ", " << fy[p] << ", "
+                << tau[p] << ")\n";
+  }
+  if (bCells && m_sys) {
+    double b[SC_BOX_NFIELDS];
+    check(sc_get_box(m_sys, 0, b), "get_box");
+    std::cout << "\nCells: " << (int)b[SC_BOX_ROWS] << " rows x " << (int)b[SC_BOX_COLS]
+              << " columns, padding " << b[SC_BOX_PAD] << std::endl;
+  }
+}
+
+// ------------------------------------------------------------ configurations
+
+void SpherocylBox::random_configuration(double dA, double dR, initialConfig config) {
+  if (!(dA >= 0) || !(dR > 0)) throw SpherocylError(SC_ERR_ARG, "random_configuration: A < 0 or R <= 0");
+  if (m_nParticles <= 0) throw SpherocylError(SC_ERR_STATE, "random_configuration: no particles");
+  std::vector<double> A(m_nParticles, dA), R(m_nParticles, dR);
+  std::vector<double> x(m_nParticles), y(m_nParticles), phi(m_nParticles);
+  int st = sc_random_configuration(m_nParticles, m_dLx, m_dGamma, A.data(), R.data(), (int)config,
+                                   m_nSeed, 100000, x.data(), y.data(), phi.data());
+  if (st != SC_OK)
+    throw SpherocylError(st, std::string("random_configuration: ") + sc_status_string(st));
+  sync_host();
+  m_vX = x;
+  m_vY = y;
+  m_vPhi = phi;
+  m_vA = A;
+  m_vR = R;
+  m_dPacking = packing_from_arrays();
+  m_bGeomStale = true;
+  m_bDevStale = true;
+}
+
+// ------------------------------------------------------------ protocols
+
+long unsigned int SpherocylBox::run_strain(double dRunLength, long unsigned int nTime) {
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  std::fstream outfSE;
+  if (nTime == 0) {
+    outfSE.open(sePath.c_str(), std::ios::out);
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_strain: cannot open " + sePath);
+    calc_se();
+    outfSE << 0 << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " " << m_dPxy
+           << std::endl;
+    save_positions("sp0000000000.dat");
+  } else {
+    outfSE.open(sePath.c_str(), std::ios::in | std::ios::out);
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_strain: cannot open " + sePath);
+    if (!seek_after_time(outfSE, nTime))
+      throw SpherocylError(SC_ERR_ARG, "run_strain: start time not found in " + sePath);
+    calc_forces();
+  }
+  if (!(m_dStrainRate > 0) || !(m_dStep > 0))
+    throw SpherocylError(SC_ERR_ARG, "run_strain: strain rate and step must be positive");
+  const long nTSteps = long(dRunLength / m_dStrainRate + 0.5);
+  const long nIntSteps = long(1.0 / m_dStep + 0.5);
+  const long nPosSaveT = long(m_dPosSaveStep / m_dStrainRate + 0.5);
+  const long nSESaveT = long(m_dSESaveStep / (m_dStrainRate * m_dStep) + 0.5);
+  if (nIntSteps < 1 || nPosSaveT < 1 || nSESaveT < 1)
+    throw SpherocylError(SC_ERR_ARG,
+                         "run_strain: step, position or SE save cadence rounds to zero");
+  ensure_device();
+  // Outer steps are batched into one GPU call up to the next position save when
+  // the SE cadence divides the sub-steps per outer step (then the records fall
+  // on the same sub-steps as the reference's per-outer-step counter).
+  const bool batch = nIntSteps % nSESaveT == 0;
+  const int seEvery = nSESaveT <= nIntSteps ? (int)nSESaveT : 0;
+  std::vector<double> rec;
+  unsigned long t = nTime + 1;
+  const unsigned long tEnd = nTime + nTSteps;
+  while (t <= tEnd) {
+    unsigned long m = 1;
+    if (batch) {
+      unsigned long nextSave = ((t + nPosSaveT - 1) / nPosSaveT) * nPosSaveT;
+      m = std::min(nextSave, tEnd) - t + 1;
+      long perStep = nIntSteps / nSESaveT;
+      m = std::min<unsigned long>(m, std::max<long>(1, kMaxRecordsPerCall / std::max(perStep, 1L)));
+    }
+    const long nsub = (long)m * nIntSteps;
+    if (nsub > std::numeric_limits<int>::max())
+      throw SpherocylError(SC_ERR_ARG, "run_strain: too many sub-steps per outer step");
+    const long nrec = seEvery > 0 ? nsub / seEvery : 0;
+    rec.resize(std::max<long>(nrec, 1) * SC_SE_NFIELDS);
+    check(sc_strain_steps(m_sys, (int)nsub, seEvery, rec.data()), "strain_steps");
+    m_bHostStale = true;
+    long r = 0;
+    for (unsigned long k = 0; k < m; ++k, ++t) {
+      // records of this outer step: global sub-steps (r+1)*seEvery within it
+      while (r < nrec && (unsigned long)(((r + 1) * seEvery - 1) / nIntSteps) == k) {
+        const double* v = &rec[r * SC_SE_NFIELDS];
+        m_dEnergy = v[0];
+        m_dPxx = v[1];
+        m_dPyy = v[2];
+        m_dPxy = v[3];
+        for (int i = 0; i < 4; ++i) m_dCorr[i] = v[4 + i];
+        write_se_line(outfSE, t, false);
+        ++r;
+      }
+      if ((t * (unsigned long)nIntSteps) % (unsigned long)nSESaveT == 0)
+        write_se_line(outfSE, t, false);
+      if (t % (unsigned long)nPosSaveT == 0) {
+        outfSE.flush();
+        save_positions(time_name("sp", t));
+      }
+    }
+  }
+  read_scalars();
+  outfSE.flush();
+  outfSE.close();
+  nTime += nTSteps;
+  save_positions(time_name("sp", nTime));
+  return nTime;
+}
+
+long unsigned int SpherocylBox::run_strain(double dRunLength) {
+  // Continues from the last time stamp of the SE file (0 if there is none);
+  // src/cpp/strain_dynamics.cpp:321-393.  Lines carry Pyx as a sixth value
+  // (the corrected virial component: the reference's own never resets).
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  unsigned long nTime = 0;
+  {
+    std::ifstream inf(sePath.c_str());
+    std::string line, last;
+    while (std::getline(inf, line))
+      if (!line.empty()) last = line;
+    if (!last.empty()) nTime = std::strtoul(last.substr(0, last.find(' ')).c_str(), nullptr, 10);
+  }
+  std::ofstream outfSE;
+  outfSE.precision(14);
+  if (nTime == 0) {
+    outfSE.open(sePath.c_str());
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_strain: cannot open " + sePath);
+    calc_se();
+    outfSE << 0 << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " " << m_dPxy << " "
+           << m_dCorr[3] << std::endl;
+    save_positions("sp0000000000.dat");
+  } else {
+    outfSE.open(sePath.c_str(), std::ios::app);
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_strain: cannot open " + sePath);
+    calc_forces();
+  }
+  if (!(m_dStrainRate > 0) || !(m_dStep > 0))
+    throw SpherocylError(SC_ERR_ARG, "run_strain: strain rate and step must be positive");
+  const long nTSteps = long(dRunLength / m_dStrainRate + 0.5);
+  const long nIntSteps = long(1.0 / m_dStep + 0.5);
+  const long nPosSaveT = long(m_dPosSaveStep / m_dStrainRate + 0.5);
+  const long nSESaveT = long(m_dSESaveStep / (m_dStrainRate * m_dStep) + 0.5);
+  if (nIntSteps < 1 || nPosSaveT < 1 || nSESaveT < 1)
+    throw SpherocylError(SC_ERR_ARG,
+                         "run_strain: step, position or SE save cadence rounds to zero");
+  ensure_device();
+  const int seEvery = nSESaveT <= nIntSteps ? (int)nSESaveT : 0;
+  const long nrec = seEvery > 0 ? nIntSteps / seEvery : 0;
+  std::vector<double> rec(std::max<long>(nrec, 1) * SC_SE_NFIELDS);
+  auto line = [&](unsigned long t) {
+    outfSE << t << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " " << m_dPxy << " "
+           << m_dCorr[3] << "\n";
+  };
+  for (unsigned long t = nTime + 1; t <= nTime + nTSteps; ++t) {
+    check(sc_strain_steps(m_sys, (int)nIntSteps, seEvery, rec.data()), "strain_steps");
+    m_bHostStale = true;
+    for (long r = 0; r < nrec; ++r) {
+      const double* v = &rec[r * SC_SE_NFIELDS];
+      m_dEnergy = v[0];
+      m_dPxx = v[1];
+      m_dPyy = v[2];
+      m_dPxy = v[3];
+      for (int i = 0; i < 4; ++i) m_dCorr[i] = v[4 + i];
+      line(t);
+    }
+    if ((t * (unsigned long)nIntSteps) % (unsigned long)nSESaveT == 0) line(t);
+    if (t % (unsigned long)nPosSaveT == 0) {
+      outfSE.flush();
+      save_positions(time_name("sp", t));
+    }
+  }
+  read_scalars();
+  outfSE.flush();
+  outfSE.close();
+  nTime += nTSteps;
+  save_positions(time_name("sp", nTime));
+  return nTime;
+}
+
+long unsigned int SpherocylBox::resize_box(double dFinalPacking, long unsigned int nTime) {
+  if (!(m_dResizeRate != 0.0) || !(m_dStep > 0))
+    throw SpherocylError(SC_ERR_ARG, "resize_box: resize rate must be non-zero, step positive");
+  sync_host();
+  m_dPacking = packing_from_arrays();
+  const double dPArea = m_dPacking * (m_dLx * m_dLy);
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  std::fstream outfSE;
+  if (nTime == 0) {
+    outfSE.open(sePath.c_str(), std::ios::out);
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "resize_box: cannot open " + sePath);
+    calc_se();
+    outfSE << 0 << " " << m_dPacking << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " "
+           << m_dPxy << std::endl;
+    save_positions("sp0000000000.dat");
+  } else {
+    outfSE.open(sePath.c_str(), std::ios::in | std::ios::out);
+    if (!outfSE) throw SpherocylError(SC_ERR_ARG, "resize_box: cannot open " + sePath);
+    if (!seek_after_time(outfSE, nTime))
+      throw SpherocylError(SC_ERR_ARG, "resize_box: start time not found in " + sePath);
+    calc_forces();
+  }
+  const long nIntSteps = long(1.0 / m_dStep + 0.5);
+  const long nPosSaveT = long(m_dPosSaveStep / m_dResizeRate + 0.5);
+  const long nSESaveT = long(m_dSESaveStep / (m_dResizeRate * m_dStep) + 0.5);
+  if (nIntSteps < 1 || nPosSaveT < 1 || nSESaveT < 1)
+    throw SpherocylError(SC_ERR_ARG,
+                         "resize_box: step, position or SE save cadence rounds to zero");
+  ensure_device();
+  const bool bShrink = dFinalPacking > m_dPacking;
+  const int seEvery = nSESaveT <= nIntSteps ? (int)nSESaveT : 0;
+  const long nrec = seEvery > 0 ? nIntSteps / seEvery : 0;
+  std::vector<double> rec(std::max<long>(nrec, 1) * SC_SE_NFIELDS);
+  // the box length after each sub-step, with the device's arithmetic
+  const double scale = std::pow(1.0 - m_dResizeRate, m_dStep);
+  while (bShrink ? dFinalPacking > m_dPacking : dFinalPacking < m_dPacking) {
+    nTime += 1;
+    check(sc_resize_steps(m_sys, (int)nIntSteps, bShrink ? 1 : 0, m_bResizeCorrected ? 1 : 0,
+                          seEvery, rec.data()),
+          "resize_steps");
+    m_bHostStale = true;
+    double L = m_dLx;
+    long r = 0;
+    for (long s = 1; s <= nIntSteps; ++s) {
+      L = bShrink ? L * scale : L / scale;
+      if (seEvery > 0 && s % seEvery == 0 && r < nrec) {
+        const double* v = &rec[r * SC_SE_NFIELDS];
+        m_dEnergy = v[0];
+        m_dPxx = v[1];
+        m_dPyy = v[2];
+        m_dPxy = v[3];
+        for (int i = 0; i < 4; ++i) m_dCorr[i] = v[4 + i];
+        m_dPacking = dPArea / (L * L);
+        write_se_line(outfSE, nTime, true);
+        ++r;
+      }
+    }
+    read_scalars();
+    m_dPacking = dPArea / (m_dLx * m_dLy);
+    if ((nTime * (unsigned long)nIntSteps) % (unsigned long)nSESaveT == 0)
+      write_se_line(outfSE, nTime, true);
+    if (nTime % (unsigned long)nPosSaveT == 0) {
+      outfSE.flush();
+      save_positions(time_name("sp", nTime));
+    }
+  }
+  outfSE.flush();
+  outfSE.close();
+  save_positions(time_name("sp", nTime));
+  return nTime;
+}
+
+long unsigned int SpherocylBox::resize_box(double dFinalPacking) {
+  // Continues from the last time stamp of the SE file, appending with 14
+  // digits (src/cpp/resize_dynamics.cpp:331-416).
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  unsigned long nTime = 0;
+  {
+    std::ifstream inf(sePath.c_str());
+    std::string line, last;
+    while (std::getline(inf, line))
+      if (!line.empty()) last = line;
+    if (!last.empty()) nTime = std::strtoul(last.substr(0, last.find(' ')).c_str(), nullptr, 10);
+  }
+  if (nTime == 0) return resize_box(dFinalPacking, 0UL);
+  if (!(m_dResizeRate != 0.0) || !(m_dStep > 0))
+    throw SpherocylError(SC_ERR_ARG, "resize_box: resize rate must be non-zero, step positive");
+  sync_host();
+  m_dPacking = packing_from_arrays();
+  const double dPArea = m_dPacking * (m_dLx * m_dLy);
+  std::ofstream outfSE(sePath.c_str(), std::ios::app);
+  if (!outfSE) throw SpherocylError(SC_ERR_ARG, "resize_box: cannot open " + sePath);
+  outfSE.precision(14);
+  calc_forces();
+  const long nIntSteps = long(1.0 / m_dStep + 0.5);
+  const long nPosSaveT = long(m_dPosSaveStep / m_dResizeRate + 0.5);
+  const long nSESaveT = long(m_dSESaveStep / (m_dResizeRate * m_dStep) + 0.5);
+  if (nIntSteps < 1 || nPosSaveT < 1 || nSESaveT < 1)
+    throw SpherocylError(SC_ERR_ARG,
+                         "resize_box: step, position or SE save cadence rounds to zero");
+  ensure_device();
+  const bool bShrink = dFinalPacking > m_dPacking;
+  const int seEvery = nSESaveT <= nIntSteps ? (int)nSESaveT : 0;
+  const long nrec = seEvery > 0 ? nIntSteps / seEvery : 0;
+  std::vector<double> rec(std::max<long>(nrec, 1) * SC_SE_NFIELDS);
+  const double scale = std::pow(1.0 - m_dResizeRate, m_dStep);
+  while (bShrink ? dFinalPacking > m_dPacking : dFinalPacking < m_dPacking) {
+    nTime += 1;
+    check(sc_resize_steps(m_sys, (int)nIntSteps, bShrink ? 1 : 0, m_bResizeCorrected ? 1 : 0,
+                          seEvery, rec.data()),
+          "resize_steps");
+    m_bHostStale = true;
+    double L = m_dLx;
+    long r = 0;
+    for (long s = 1; s <= nIntSteps; ++s) {
+      L = bShrink ? L * scale : L / scale;
+      if (seEvery > 0 && s % seEvery == 0 && r < nrec) {
+        const double* v = &rec[r * SC_SE_NFIELDS];
+        m_dEnergy = v[0];
+        m_dPxx = v[1];
+        m_dPyy = v[2];
+        m_dPxy = v[3];
+        for (int i = 0; i < 4; ++i) m_dCorr[i] = v[4 + i];
+        m_dPacking = dPArea / (L * L);
+        write_se_line(outfSE, nTime, true);
+        ++r;
+      }
+    }
+    read_scalars();
+    m_dPacking = dPArea / (m_dLx * m_dLy);
+    if ((nTime * (unsigned long)nIntSteps) % (unsigned long)nSESaveT == 0)
+      write_se_line(outfSE, nTime, true);
+    if (nTime % (unsigned long)nPosSaveT == 0) {
+      outfSE.flush();
+      save_positions(time_name("sp", nTime));
+    }
+  }
+  outfSE.flush();
+  outfSE.close();
+  save_positions(time_name("sp", nTime));
+  return nTime;
+}
+
+void SpherocylBox::gradient_descent_minimize(int nMaxSteps, double dMinE) {
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  std::fstream outfSE(sePath.c_str(), std::ios::out);
+  if (!outfSE) throw SpherocylError(SC_ERR_ARG, "gradient_descent_minimize: cannot open " + sePath);
+  const long nPosSaveT = long(m_dPosSaveStep + 0.5);
+  if (nPosSaveT < 1)
+    throw SpherocylError(SC_ERR_ARG, "gradient_descent_minimize: position save cadence < 1");
+  ensure_device();
+  calc_se();
+  bool hitMin = false;
+  for (int s = 0; s < nMaxSteps; ++s) {
+    outfSE << s << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " " << m_dPxy
+           << std::endl;
+    if (m_dEnergy <= dMinE) {
+      outfSE.flush();
+      save_positions(time_name("sr", (unsigned long)s));
+      hitMin = true;
+      break;
+    } else if (s % nPosSaveT == 0) {
+      outfSE.flush();
+      save_positions(time_name("sr", (unsigned long)s));
+    }
+    // one step x += dt F, phi += dt tau / I, then forces, energy and stress of
+    // the new state (the next iteration's calc_se)
+    int done = 0;
+    check(sc_gd_steps(m_sys, 1, -std::numeric_limits<double>::infinity(), &done), "gd_steps");
+    m_bHostStale = true;
+    double b[SC_BOX_NFIELDS];
+    check(sc_get_box(m_sys, 0, b), "get_box");
+    m_dEnergy = b[SC_BOX_E];
+    m_dPxx = b[SC_BOX_PXX];
+    m_dPyy = b[SC_BOX_PYY];
+    m_dPxy = b[SC_BOX_PXY];
+    for (int i = 0; i < 4; ++i) m_dCorr[i] = b[SC_BOX_CPXX + i];
+  }
+  if (!hitMin) {
+    std::cout << "\nMaximum relaxation steps reached" << std::endl;
+    outfSE << nMaxSteps << " " << m_dEnergy << " " << m_dPxx << " " << m_dPyy << " " << m_dPxy
+           << std::endl;
+    save_positions(time_name("sr", (unsigned long)nMaxSteps));
+  } else {
+    std::cout << "\nEncountered zero energy" << std::endl;
+  }
+  read_scalars();
+  outfSE.close();
+}
+
+int SpherocylBox::fire_minimize(const double* pdParams, bool bThrowIfNotConverged) {
+  static const double kDefaults[10] = {0.05, 0.5, 5, 1.1, 0.5, 0.1, 0.99, 0.02, 1e-10, 200000};
+  ensure_device();
+  const double* p = pdParams ? pdParams : kDefaults;
+  int iters = 0;
+  double frms = 0;
+  // mixed precision: FP32 pair math down to a residual of 1e-6, FP64 after
+  const double sw = m_nPrecision == SC_PREC_MIXED ? 1e-6 : 0.0;
+  int st = sc_fire_minimize(m_sys, p, sw, &iters, &frms);
+  m_bHostStale = true;
+  if (st == SC_ERR_NOT_CONVERGED && !bThrowIfNotConverged) st = SC_OK;
+  check(st, "fire_minimize");
+  double b[SC_BOX_NFIELDS];
+  check(sc_get_box(m_sys, 0, b), "get_box");
+  m_dEnergy = b[SC_BOX_E];
+  m_dPxx = b[SC_BOX_PXX];
+  m_dPyy = b[SC_BOX_PYY];
+  m_dPxy = b[SC_BOX_PXY];
+  for (int i = 0; i < 4; ++i) m_dCorr[i] = b[SC_BOX_CPXX + i];
+  read_scalars();
+  return iters;
+}
+
+// ------------------------------------------------------------ position files
+
+PositionFile read_position_file(const std::string& path) {
+  PositionFile f;
+  int n = 0;
+  double h[6];
+  if (sc_read_positions(path.c_str(), 0, &n, h, nullptr, nullptr, nullptr, nullptr, nullptr) !=
+      SC_OK)
+    throw SpherocylError(SC_ERR_ARG, "cannot read positions from " + path);
+  f.x.resize(n);
+  f.y.resize(n);
+  f.phi.resize(n);
+  f.R.resize(n);
+  f.A.resize(n);
+  if (sc_read_positions(path.c_str(), n, &f.n, h, f.x.data(), f.y.data(), f.phi.data(),
+                        f.R.data(), f.A.data()) != SC_OK)
+    throw SpherocylError(SC_ERR_ARG, "cannot read positions from " + path);
+  f.L = h[0];
+  f.packing = h[1];
+  f.gamma = h[2];
+  f.gamma_total = h[3];
+  f.strain_rate = h[4];
+  f.step = h[5];
+  return f;
+}
+
+unsigned long position_file_time(const std::string& path) {
+  if (path.size() < 14) throw SpherocylError(SC_ERR_ARG, "file name too short for a time stamp");
+  return std::strtoul(path.substr(path.size() - 14, 10).c_str(), nullptr, 10);
+}
+
+// ------------------------------------------------------------ C entry points
+
+static void se_line(std::ostream& out, unsigned long t, int nvals, const double* vals) {
+  out << t;
+  for (int i = 0; i < nvals; ++i) out << " " << vals[i];
+  out << "\n";
+}
+
+extern "C" int sc_write_positions(const char* path, int n, double L, double packing, double gamma,
+                                  double gamma_total, double strain_rate, double step,
+                                  const double* x, const double* y, const double* phi,
+                                  const double* R, const double* A) {
+  if (!path || n < 0 || (n > 0 && (!x || !y || !phi || !R || !A))) return SC_ERR_ARG;
+  std::ofstream outf(path);
+  if (!outf) return SC_ERR_ARG;
+  outf.precision(14);
+  outf << n << " " << L << " " << packing << " " << gamma << " " << gamma_total << " "
+       << strain_rate << " " << step << std::endl;
+  for (int p = 0; p < n; ++p)
+    outf << x[p] << " " << y[p] << " " << phi[p] << " " << R[p] << " " << A[p] << std::endl;
+  outf.close();
+  return outf ? SC_OK : SC_ERR_ARG;
+}
+
+extern "C" int sc_read_positions(const char* path, int cap, int* n, double* header, double* x,
+                                 double* y, double* phi, double* R, double* A) {
+  if (!path || !n || !header || cap < 0 || (cap > 0 && (!x || !y || !phi || !R || !A)))
+    return SC_ERR_ARG;
+  std::ifstream in(path);
+  if (!in) return SC_ERR_ARG;
+  std::string head;
+  if (!std::getline(in, head)) return SC_ERR_ARG;
+  std::istringstream hs(head);
+  int nh = 0;
+  if (!(hs >> nh)) return SC_ERR_ARG;
+  for (int i = 0; i < 6; ++i)
+    if (!(hs >> header[i])) return SC_ERR_ARG;
+  int rows = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    std::istringstream ls(line);
+    double v[5];
+    for (double& d : v)
+      if (!(ls >> d)) return SC_ERR_ARG;
+    if (rows < cap) {
+      x[rows] = v[0];
+      y[rows] = v[1];
+      phi[rows] = v[2];
+      R[rows] = v[3];
+      A[rows] = v[4];
+    }
+    ++rows;
+  }
+  *n = rows;
+  return SC_OK;
+}
+
+extern "C" int sc_format_se_line(char* buf, int cap, unsigned long t, int nvals,
+                                 const double* vals, int precision) {
+  if (!buf || cap < 1 || nvals < 0 || (nvals > 0 && !vals) || precision < 1) return -1;
+  std::ostringstream os;
+  os.precision(precision);
+  se_line(os, t, nvals, vals);
+  const std::string s = os.str();
+  if ((int)s.size() + 1 > cap) return -1;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}

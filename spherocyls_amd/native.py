@@ -85,6 +85,13 @@ def load():
         "sc_synchronize": ([sysp], C.c_int),
         "sc_random_packing": ([C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_ulonglong, C.c_int,
                                _dp, _dp, _dp], C.c_int),
+        "sc_random_configuration": ([C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                     C.c_ulonglong, C.c_int, _dp, _dp, _dp], C.c_int),
+        "sc_write_positions": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp],
+                               C.c_int),
+        "sc_read_positions": ([C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "sc_format_se_line": ([C.c_char_p, C.c_int, C.c_ulong, C.c_int, _dp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -114,6 +121,59 @@ def random_packing(L, A, R, seed, gamma=0.0, max_tries=100000):
     if st != 0:
         raise SpherocylsError(st, "random placement")
     return x, y, phi
+
+
+def random_configuration(L, A, R, seed, config=0, gamma=0.0, max_tries=100000):
+    """Seeded placement in one of the reference's initialConfig modes
+    (SC_CONFIG_*); host code, no GPU needed."""
+    lib = load()
+    A, R = _f64(A), _f64(R)
+    n = A.shape[0]
+    x, y, phi = np.empty(n), np.empty(n), np.empty(n)
+    st = lib.sc_random_configuration(n, float(L), float(gamma), _p(A), _p(R), int(config),
+                                     C.c_ulonglong(seed), max_tries, _p(x), _p(y), _p(phi))
+    if st != 0:
+        raise SpherocylsError(st, "random placement")
+    return x, y, phi
+
+
+def write_positions(path, L, packing, gamma, gamma_total, strain_rate, step, x, y, phi, R, A):
+    """save_positions file (reference layout, 14 significant digits)."""
+    lib = load()
+    x, y, phi, R, A = (_f64(v) for v in (x, y, phi, R, A))
+    st = lib.sc_write_positions(os.fsencode(path), x.shape[0], L, packing, gamma, gamma_total,
+                                strain_rate, step, _p(x), _p(y), _p(phi), _p(R), _p(A))
+    if st != 0:
+        raise SpherocylsError(st, f"cannot write {path}")
+
+
+def read_positions(path):
+    """(header dict, dict of arrays x y phi R A) of a save_positions file."""
+    lib = load()
+    n = C.c_int()
+    h = np.zeros(6)
+    st = lib.sc_read_positions(os.fsencode(path), 0, C.byref(n), _p(h), None, None, None, None,
+                               None)
+    if st != 0:
+        raise SpherocylsError(st, f"cannot read {path}")
+    arrs = [np.empty(max(n.value, 1)) for _ in range(5)]
+    st = lib.sc_read_positions(os.fsencode(path), n.value, C.byref(n), _p(h),
+                               *[_p(a) for a in arrs])
+    if st != 0:
+        raise SpherocylsError(st, f"cannot read {path}")
+    head = dict(zip(("L", "packing", "gamma", "gamma_total", "strain_rate", "step"), h.tolist()))
+    return head, dict(zip(("x", "y", "phi", "R", "A"), (a[:n.value] for a in arrs)))
+
+
+def format_se_line(t, values, precision=6):
+    """One stress/energy line as the reference's writers format it."""
+    lib = load()
+    v = _f64(values)
+    buf = C.create_string_buffer(64 + 32 * v.shape[0])
+    k = lib.sc_format_se_line(buf, len(buf), t, v.shape[0], _p(v), precision)
+    if k < 0:
+        raise SpherocylsError(1, "SE line buffer")
+    return buf.value.decode()
 
 
 FIRE_DEFAULTS = dict(dt0=0.05, dt_max=0.5, n_min=5, f_inc=1.1, f_dec=0.5, alpha0=0.1,

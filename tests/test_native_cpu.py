@@ -14,10 +14,16 @@ from spherocyls_amd import configs, native
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+C_HEADERS = ("spherocyls.h", "spherocyl_files.h")
+
+
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "spherocyls.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sc_\w+)\s*\(", text)))
+    names = set()
+    for h in C_HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(sc_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -27,7 +33,17 @@ def test_library_exports_every_declared_symbol():
     exported = set(re.findall(r"\bT (sc_\w+)$", out, flags=re.M))
     missing = [f for f in declared_functions() if f not in exported]
     assert not missing, missing
-    assert len(declared_functions()) >= 30
+    assert len(declared_functions()) >= 35
+
+
+def test_cpp_box_class_is_exported():
+    """SpherocylBox (include/spherocyl_box.h) lives in the same library."""
+    out = subprocess.run(["nm", "-DC", "--defined-only", native.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for m in ("SpherocylBox::run_strain(double, unsigned long)", "SpherocylBox::resize_box(double)",
+              "SpherocylBox::gradient_descent_minimize(int, double)",
+              "SpherocylBox::save_positions(std::", "SpherocylBox::random_configuration("):
+        assert m in out, m
 
 
 def test_library_loads_and_describes_status():
@@ -94,3 +110,19 @@ def test_workload_generators():
     aspects = {round(float((p.A[0] + p.R[0]) / p.R[0]), 6) for p in ens}
     assert aspects <= {1.2, 2.0, 4.0}
     assert all(0.80 <= p.packing_fraction <= 0.86 for p in ens)
+
+
+def test_drivers_built():
+    for exe in ("spherocyl_strain", "spherocyl_resize", "spherocyl_minimize"):
+        assert os.access(os.path.join(ROOT, "spherocyls_amd", "bin", exe), os.X_OK)
+
+
+def test_driver_reports_missing_gpu_or_bad_input(tmp_path):
+    """Without a readable start file the driver exits with status 1 and a message
+    (the reference exits the same way), never a crash."""
+    import subprocess
+    r = subprocess.run([os.path.join(ROOT, "spherocyls_amd", "bin", "spherocyl_strain"),
+                        str(tmp_path / "missing.dat"), str(tmp_path), "64", "1e-4", "0.1", "1e-4",
+                        "0.01", "1e-5", "0.1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "cannot read" in r.stderr
