@@ -81,6 +81,78 @@ def cpu_baseline(pk, strain_rate, dt, budget_s=15.0):
                       f"({cpu_model()}, nproc={os.cpu_count()})"}
 
 
+# FIRE, per particle-iteration, non-fused force + update passes: 112 B for a
+# fused pass (read x, y, phi, R, A and v; write x, y, phi and v) plus 48 B for
+# writing the forces in the force pass and reading them back in the update pass
+BYTES_FIRE = 160
+
+
+def minimization(n_packings, reps, cpu_budget_s):
+    """packings/s minimised to F_rms < 1e-10 over a ragged ensemble: N log-uniform
+    in [64, 4096], phi U[0.80, 0.86], aspect {1.2, 2, 4}; each packing first
+    gets 2000 sheared Heun steps (strain rate 1e-4, dt 0.01), then FIRE with the
+    two-point contact model.  Three GPU paths from the same states: all packings
+    batched in mixed precision, batched in FP64, and one packing at a time in
+    FP64."""
+    from spherocyls_amd import ensemble
+
+    pks = configs.ensemble(n_packings)
+    t0 = time.perf_counter()
+    work = ensemble.prepare(pks, heun_steps=2000, strain_rate=1e-4, dt=0.01)
+    prep_s = time.perf_counter() - t0
+    params = native.fire_params(max_iter=100000)
+    out = {"metric": "packings/s minimized to F_rms<1e-10", "unit": "packings/s",
+           "packings": n_packings, "particles": work.particles, "contact_model": "two_point",
+           "prepare_s": prep_s}
+    runs = {}
+    for label, prec in (("batched_mixed", "mixed"), ("batched_fp64", "fp64")):
+        best = None
+        for _ in range(reps):
+            sec, it, fr, st = ensemble.minimize_batched(work, params, precision=prec)
+            if best is None or sec < best[0]:
+                best = (sec, it, fr, st)
+        sec, it, fr, st = best
+        conv = int(np.sum(fr < 1e-10))
+        pit = float(np.sum(np.array(work.sizes) * it))
+        runs[label] = {"value": n_packings / sec, "seconds": sec, "converged": conv,
+                       "max_iterations": int(it.max()), "mean_iterations": float(it.mean()),
+                       "particle_iterations_per_s": pit / sec,
+                       "hbm_frac_fire_bytes": pit * BYTES_FIRE / sec / (HBM_PEAK_GBS * 1e9)}
+    sec, it, fr = ensemble.minimize_one_by_one(work, params, precision="fp64")
+    runs["one_by_one_fp64"] = {"value": n_packings / sec, "seconds": sec,
+                               "converged": int(np.sum(fr < 1e-10)),
+                               "mean_iterations": float(it.mean())}
+    out.update(runs)
+    out["value"] = runs["batched_mixed"]["value"]
+    out["speedup_vs_one_by_one_fp64"] = runs["batched_mixed"]["value"] / runs["one_by_one_fp64"]["value"]
+    if cpu_budget_s > 0:
+        out["cpu_baseline"] = cpu_minimization(work, params, cpu_budget_s)
+    return out
+
+
+def cpu_minimization(work, params, budget_s):
+    """The oracle restatement's FIRE (two-point model, FP64, 1 thread) on the
+    same pre-minimisation states, packing after packing until the budget is
+    spent."""
+    import oracle
+
+    done, elapsed, iters = 0, 0.0, 0
+    for pk, g in zip(work.packings, work.gammas):
+        b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 0.1)
+        b.set_contact_model(1)
+        b.set_params(g, 0.0, 0.0, 0.0, 0.05)
+        t0 = time.perf_counter()
+        it, fr = b.fire(params)
+        elapsed += time.perf_counter() - t0
+        done += 1
+        iters += it
+        if elapsed > budget_s:
+            break
+    return {"value": done / elapsed, "unit": "packings/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} packings of the same ensemble, FIRE two-point FP64 on 1 core "
+                      f"({cpu_model()}), {iters} iterations"}
+
+
 def copy_calibration(mib):
     """Plain device copy of a known size: its FETCH_SIZE / WRITE_SIZE counter
     readings give the factor that turns those counters into HBM bytes."""
@@ -108,6 +180,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=1, help="list path: lanes per particle")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-minimization", action="store_true",
+                    help="skip the packings/s minimisation measurement")
+    ap.add_argument("--packings", type=int, default=256, help="minimisation ensemble size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -115,8 +190,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(packings and slabs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: ensembles "
+              "would split by packing and a single large box into y-slabs with halo "
+              "exchange, neither of which is built", file=sys.stderr)
         sys.exit(2)
 
     make, strain_rate, dt = WORKLOADS[args.workload]
@@ -183,6 +259,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pk, strain_rate, dt)
+    if not args.no_minimization:
+        out["minimization"] = minimization(args.packings, 1,
+                                           0.0 if args.no_cpu_baseline else 15.0)
     print(json.dumps(out), flush=True)
 
 

@@ -96,6 +96,17 @@ int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate,
 /* set_gamma / set_total_gamma  src/cpp/spherocyl_box.h:118-119 */
 int sc_set_gamma(sc_system* s, int k, double gamma, double gamma_total);
 int sc_set_precision(sc_system* s, int mode);
+/* Contact model.  SC_CONTACT_CLOSEST: one contact at the closest points of the
+ * two segments (the CPU reference, calc_force_pair src/cpp/strain_dynamics.cpp:
+ * 17-75; the default).  SC_CONTACT_TWO_POINT: a second contact at the other end
+ * of the overlap of nearly parallel rods, so aligned overlapping rods have no
+ * energy kink (the reference's GPU two-point model, src/cuda/2point_dynamics.cu:
+ * 391-546); needed for minimisation to F_rms < 1e-10 once rods pack side by
+ * side.  Energy, stress and forces sum over the contact points. */
+#define SC_CONTACT_CLOSEST 0
+#define SC_CONTACT_TWO_POINT 1
+int sc_set_contact_model(sc_system* s, int model);
+int sc_get_contact_model(const sc_system* s, int* model);
 /* Force-pass implementation (no reference counterpart; both give the same
  * forces to rounding):
  *   SC_PATH_LIST  one thread per particle over a per-particle partner list kept

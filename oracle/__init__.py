@@ -59,6 +59,8 @@ def _load_oracle():
             getattr(lib, name).argtypes = [C.c_void_p]
             getattr(lib, name).restype = None
         lib.orc_set_params.argtypes = [C.c_void_p] + [C.c_double] * 5
+        lib.orc_set_contact_model.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_set_contact_model.restype = None
         lib.orc_set_positions.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.orc_run_steps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         lib.orc_run_steps.restype = C.c_int
@@ -155,6 +157,10 @@ class OracleBox(_BoxBase):
 
     def set_params(self, gamma=0.0, gamma_total=0.0, strain_rate=0.0, resize_rate=0.0, step=0.05):
         self.lib.orc_set_params(self.h, gamma, gamma_total, strain_rate, resize_rate, step)
+
+    def set_contact_model(self, model):
+        """0: closest point (reference CPU); 1: two-point contact model."""
+        self.lib.orc_set_contact_model(self.h, int(model))
 
     def set_positions(self, x=None, y=None, phi=None):
         x, y, phi = (None if v is None else _f64(v) for v in (x, y, phi))

@@ -47,6 +47,7 @@ struct orc_box {
   double E = 0, Pxx = 0, Pyy = 0, Pxy = 0;           // reference formulas
   double cPxx = 0, cPyy = 0, cPxy = 0, cPyx = 0;     // corrected virial
   int ncontacts = 0;
+  int contact_model = 0;  // 0: closest point only (src/cpp), 1: two-point model
 };
 
 namespace {
@@ -181,7 +182,8 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
   double d = o.nxA * ddx + o.nyA * ddy;
   double e = -o.nxB * ddx - o.nyB * ddy;
   double det = a * c - bb * bb;
-  double t = std::fmin(std::fmax((bb * d - a * e) / det, -1.), 1.);
+  double tu = (bb * d - a * e) / det;
+  double t = std::fmin(std::fmax(tu, -1.), 1.);
   double s = -(bb * t + d) / a;
   double sabs = std::fabs(s);
   s = std::fmin(std::fmax(s, -1.), 1.);
@@ -192,6 +194,17 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
   o.dy = ddy + s * o.nyA - t * o.nyB;
   double d2 = o.dx * o.dx + o.dy * o.dy;
   o.contact = d2 < o.sigma * o.sigma;
+  // Crossed segments (the lines meet inside both segments): d = 0 under any
+  // small displacement, a plateau of the energy with zero gradient; the
+  // reference normalises the rounding noise in D there (a full-size force in a
+  // random direction) or divides 0 by 0.  Also taken for an exact D = 0.
+  const bool crossed = (det > 0 && std::fabs(tu) < 1 && sabs < 1) || d2 == 0.0;
+  if (o.contact && crossed) {
+    o.dist = 0.0;
+    o.dV = 1.0 / o.sigma;
+    o.fx = o.fy = o.tau_p = o.tau_q = 0.0;
+    return o;
+  }
   if (o.contact) {
     o.dist = std::sqrt(d2);
     o.dV = (1.0 - o.dist / o.sigma) / o.sigma;
@@ -203,6 +216,154 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
     o.dist = o.dV = o.fx = o.fy = o.tau_p = o.tau_q = 0;
   }
   return o;
+}
+
+// Fills o's force fields from its D vector (harmonic contact, as pair_eval).
+inline void contact_force(PairOut& o, double s, double t, bool crossed) {
+  o.s = s;
+  o.t = t;
+  double d2 = o.dx * o.dx + o.dy * o.dy;
+  o.contact = d2 < o.sigma * o.sigma;
+  if (o.contact && (crossed || d2 == 0.0)) {  // energy plateau, as in pair_eval
+    o.dist = 0.0;
+    o.dV = 1.0 / o.sigma;
+    o.fx = o.fy = o.tau_p = o.tau_q = 0.0;
+    return;
+  }
+  if (o.contact) {
+    o.dist = std::sqrt(d2);
+    o.dV = (1.0 - o.dist / o.sigma) / o.sigma;
+    o.fx = o.dx * o.dV / o.dist;
+    o.fy = o.dy * o.dV / o.dist;
+    o.tau_p = s * o.nxA * o.fy - s * o.nyA * o.fx;
+    o.tau_q = t * o.nyB * o.fx - t * o.nxB * o.fy;
+  } else {
+    o.dist = o.dV = o.fx = o.fy = o.tau_p = o.tau_q = 0;
+  }
+}
+
+inline double clamp1(double v) { return std::fmin(std::fmax(v, -1.0), 1.0); }
+
+// Two-point contact model of the reference's GPU code
+// (src/cuda/2point_dynamics.cu:391-546): besides the closest pair of points
+// (s1, t1), a second pair (s2, t2) -- for nearly parallel rods the other end of
+// their overlap -- carries its own harmonic contact when it overlaps too, so the
+// energy of two overlapping, nearly aligned rods has no kink at alignment.  The
+// second pair starts from the opposite end of whichever segment set the first
+// (s2 = -s1 or t2 = -t1), projected onto the other segment; a third candidate
+// (s3, t3) from the other segment's matching end replaces it when closer.  The
+// pair is evaluated once, lower id first, and both particles get +-F (the
+// reference evaluates it from each side).  Returns the number of contacts in
+// out[0..1] (out[0] is the closest-point contact of pair_eval).
+inline int pair_eval_2p(double xp, double yp, double php, double Rp, double Ap, double xq,
+                        double yq, double phq, double Rq, double Aq, double Lx, double Ly,
+                        double gam, PairOut* out) {
+  PairOut o;
+  double ddx = xp - xq;
+  double ddy = yp - yq;
+  o.sigma = Rp + Rq;
+  ddx += Lx * ((ddx < -0.5 * Lx) - (ddx > 0.5 * Lx));
+  ddy += Ly * ((ddy < -0.5 * Ly) - (ddy > 0.5 * Ly));
+  ddx += gam * ddy;
+  o.nxA = Ap * std::cos(php);
+  o.nyA = Ap * std::sin(php);
+  o.nxB = Aq * std::cos(phq);
+  o.nyB = Aq * std::sin(phq);
+  const double a = Ap * Ap;
+  const double b = -(o.nxA * o.nxB + o.nyA * o.nyB);
+  const double c = Aq * Aq;
+  const double d = o.nxA * ddx + o.nyA * ddy;
+  const double e = -o.nxB * ddx - o.nyB * ddy;
+  const double det = a * c - b * b;
+  double s1, t1, s2, t2, s3, t3;
+  bool crossed = false;
+  if (det <= 0) {  // parallel: the two ends of the overlap
+    s1 = clamp1(-(b + d) / a);
+    s2 = clamp1(-(d - b) / a);
+    if (std::fabs(s1) == 1) t1 = clamp1(-(b * s1 + e) / c);
+    else t1 = b < 0 ? clamp1(-(b + e) / c) : clamp1(-(e - b) / c);
+    if (std::fabs(s2) == 1) t2 = clamp1(-(b * s2 + e) / c);
+    else t2 = b < 0 ? clamp1(-(e - b) / c) : clamp1(-(b + e) / c);
+    s3 = s2;
+    t3 = t2;
+  } else {
+    const double tu = (b * d - a * e) / det;
+    t1 = clamp1(tu);
+    s1 = -(b * t1 + d) / a;
+    double sarg = std::fabs(s1);
+    s1 = clamp1(s1);
+    crossed = std::fabs(tu) < 1 && sarg < 1;
+    if (sarg >= 1) {
+      t1 = clamp1(-(b * s1 + e) / c);
+      s2 = -s1;
+      t2 = -(b * s2 + e) / c;
+      double targ = std::fabs(t2);
+      t2 = clamp1(t2);
+      if (targ > 1) {
+        s2 = clamp1(-(b * t2 + d) / a);
+        s3 = s2;
+        t3 = t2;
+      } else {
+        t3 = b < 0 ? -s1 : s1;
+        s3 = -(b * t3 + d) / a;
+        sarg = std::fabs(s3);
+        s3 = clamp1(s3);
+        if (sarg >= 1) t3 = clamp1(-(b * s3 + e) / c);
+      }
+    } else {
+      t2 = -t1;
+      s2 = -(b * t2 + d) / a;
+      sarg = std::fabs(s2);
+      s2 = clamp1(s2);
+      if (sarg > 1) {
+        t2 = clamp1(-(b * s2 + e) / c);
+        s3 = s2;
+        t3 = t2;
+      } else {
+        s3 = b < 0 ? -t1 : t1;
+        t3 = -(b * s3 + e) / c;
+        double targ = std::fabs(t3);
+        t3 = clamp1(t3);
+        if (targ >= 1) s3 = clamp1(-(b * t3 + d) / a);
+      }
+    }
+  }
+  int k = 0;
+  PairOut o1 = o;
+  o1.dx = ddx + s1 * o.nxA - t1 * o.nxB;
+  o1.dy = ddy + s1 * o.nyA - t1 * o.nyB;
+  contact_force(o1, s1, t1, crossed);
+  if (o1.contact) out[k++] = o1;
+  if (s1 != s2 && t1 != t2) {
+    PairOut o2 = o;
+    o2.dx = ddx + s2 * o.nxA - t2 * o.nxB;
+    o2.dy = ddy + s2 * o.nyA - t2 * o.nyB;
+    double d22 = o2.dx * o2.dx + o2.dy * o2.dy;
+    if (s3 != s1 && s3 != s2) {
+      double dx3 = ddx + s3 * o.nxA - t3 * o.nxB;
+      double dy3 = ddy + s3 * o.nyA - t3 * o.nyB;
+      if (dx3 * dx3 + dy3 * dy3 < d22) {
+        s2 = s3;
+        t2 = t3;
+        o2.dx = dx3;
+        o2.dy = dy3;
+      }
+    }
+    contact_force(o2, s2, t2, false);
+    if (o2.contact) out[k++] = o2;
+  }
+  return k;
+}
+
+// Contacts of pair (p, q) under the box's contact model: out[0..k-1].
+inline int pair_contacts(const orc_box* b, const double* x, const double* y, const double* phi,
+                         int p, int q, double gam, double Lx, double Ly, PairOut* out) {
+  if (b->contact_model == 1)
+    return pair_eval_2p(x[p], y[p], phi[p], b->R[p], b->A[p], x[q], y[q], phi[q], b->R[q],
+                        b->A[q], Lx, Ly, gam, out);
+  out[0] = pair_eval(x[p], y[p], phi[p], b->R[p], b->A[p], x[q], y[q], phi[q], b->R[q], b->A[q],
+                     Lx, Ly, gam);
+  return out[0].contact ? 1 : 0;
 }
 
 inline PairOut pair_main(const orc_box* b, int p, int q, double gam, double Lx, double Ly) {
@@ -220,14 +381,18 @@ void forces_main(orc_box* b) {
   for (int p = 0; p < b->n; ++p)
     for (int q : b->nbr[p]) {
       if (q <= p) continue;
-      PairOut o = pair_main(b, p, q, b->gamma, b->Lx, b->Ly);
-      if (!o.contact) continue;
-      b->fx[p] += o.fx;
-      b->fy[p] += o.fy;
-      b->tau[p] += o.tau_p;
-      b->fx[q] -= o.fx;
-      b->fy[q] -= o.fy;
-      b->tau[q] += o.tau_q;
+      PairOut pc[2];
+      int k = pair_contacts(b, b->x.data(), b->y.data(), b->phi.data(), p, q, b->gamma, b->Lx,
+                            b->Ly, pc);
+      for (int j = 0; j < k; ++j) {
+        const PairOut& o = pc[j];
+        b->fx[p] += o.fx;
+        b->fy[p] += o.fy;
+        b->tau[p] += o.tau_p;
+        b->fx[q] -= o.fx;
+        b->fy[q] -= o.fy;
+        b->tau[q] += o.tau_q;
+      }
     }
 }
 
@@ -239,14 +404,17 @@ void forces_temp(orc_box* b, double Lx, double Ly) {
   for (int p = 0; p < b->n; ++p)
     for (int q : b->nbr[p]) {
       if (q <= p) continue;
-      PairOut o = pair_temp(b, p, q, gam, Lx, Ly);
-      if (!o.contact) continue;
-      b->tfx[p] += o.fx;
-      b->tfy[p] += o.fy;
-      b->ttau[p] += o.tau_p;
-      b->tfx[q] -= o.fx;
-      b->tfy[q] -= o.fy;
-      b->ttau[q] += o.tau_q;
+      PairOut pc[2];
+      int k = pair_contacts(b, b->tx.data(), b->ty.data(), b->tphi.data(), p, q, gam, Lx, Ly, pc);
+      for (int j = 0; j < k; ++j) {
+        const PairOut& o = pc[j];
+        b->tfx[p] += o.fx;
+        b->tfy[p] += o.fy;
+        b->ttau[p] += o.tau_p;
+        b->tfx[q] -= o.fx;
+        b->tfy[q] -= o.fy;
+        b->ttau[q] += o.tau_q;
+      }
     }
 }
 
@@ -262,31 +430,35 @@ void forces_se(orc_box* b) {
   for (int p = 0; p < b->n; ++p)
     for (int q : b->nbr[p]) {
       if (q <= p) continue;
-      PairOut o = pair_main(b, p, q, b->gamma, b->Lx, b->Ly);
-      if (!o.contact) continue;
-      b->ncontacts++;
-      b->fx[p] += o.fx;
-      b->fy[p] += o.fy;
-      b->tau[p] += o.tau_p;
-      b->fx[q] -= o.fx;
-      b->fy[q] -= o.fy;
-      b->tau[q] += o.tau_q;
-      double cxa = o.s * o.nxA - 0.5 * o.dx;
-      double cya = o.s * o.nyA - 0.5 * o.dy;
-      double cxb = o.t * o.nxB + 0.5 * o.dx;
-      double cyb_ref = o.t * o.nxB + 0.5 * o.dy;  // reference slip (uses nxB)
-      double cyb = o.t * o.nyB + 0.5 * o.dy;
-      b->E += o.dV * o.sigma * (1.0 - o.dist / o.sigma) / (2.0 * b->n);
-      b->Pxx += o.fx * cxa / area;
-      b->Pxx -= o.fx * cxb / area;
-      b->Pyy += o.fy * cya / area;
-      b->Pyy -= o.fy * cyb_ref / area;
-      b->Pxy += o.fy * cxa / area;
-      b->Pxx -= o.fy * cxb / area;  // reference slip: should update Pxy
-      b->cPxx += o.fx * (cxa - cxb) / area;
-      b->cPyy += o.fy * (cya - cyb) / area;
-      b->cPxy += o.fy * (cxa - cxb) / area;
-      b->cPyx += o.fx * (cya - cyb) / area;
+      PairOut pc[2];
+      int k = pair_contacts(b, b->x.data(), b->y.data(), b->phi.data(), p, q, b->gamma, b->Lx,
+                            b->Ly, pc);
+      if (k > 0) b->ncontacts++;
+      for (int j = 0; j < k; ++j) {
+        const PairOut& o = pc[j];
+        b->fx[p] += o.fx;
+        b->fy[p] += o.fy;
+        b->tau[p] += o.tau_p;
+        b->fx[q] -= o.fx;
+        b->fy[q] -= o.fy;
+        b->tau[q] += o.tau_q;
+        double cxa = o.s * o.nxA - 0.5 * o.dx;
+        double cya = o.s * o.nyA - 0.5 * o.dy;
+        double cxb = o.t * o.nxB + 0.5 * o.dx;
+        double cyb_ref = o.t * o.nxB + 0.5 * o.dy;  // reference slip (uses nxB)
+        double cyb = o.t * o.nyB + 0.5 * o.dy;
+        b->E += o.dV * o.sigma * (1.0 - o.dist / o.sigma) / (2.0 * b->n);
+        b->Pxx += o.fx * cxa / area;
+        b->Pxx -= o.fx * cxb / area;
+        b->Pyy += o.fy * cya / area;
+        b->Pyy -= o.fy * cyb_ref / area;
+        b->Pxy += o.fy * cxa / area;
+        b->Pxx -= o.fy * cxb / area;  // reference slip: should update Pxy
+        b->cPxx += o.fx * (cxa - cxb) / area;
+        b->cPyy += o.fy * (cya - cyb) / area;
+        b->cPxy += o.fy * (cxa - cxb) / area;
+        b->cPyx += o.fx * (cya - cyb) / area;
+      }
     }
 }
 
@@ -581,6 +753,8 @@ orc_box* orc_new(double L, int n, const double* x, const double* y, const double
 }
 
 void orc_free(orc_box* b) { delete b; }
+
+void orc_set_contact_model(orc_box* b, int model) { b->contact_model = model == 1 ? 1 : 0; }
 
 void orc_set_params(orc_box* b, double gamma, double gamma_total, double strain_rate,
                     double resize_rate, double step) {

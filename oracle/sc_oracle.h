@@ -31,6 +31,9 @@ int orc_run_steps(orc_box* b, int nsteps, int se_every, double* se_out);  // :28
 void orc_resize_step(orc_box* b, int shrink, int corrected_image);  // resize_dynamics.cpp:164-243
 void orc_gd_step(orc_box* b);          // relaxation_dynamics.cpp:11-31
 void orc_set_back_gamma(orc_box* b);   // spherocyl_box.cpp:524-545
+// 0: closest-point contact (src/cpp); 1: two-point contact model
+// (src/cuda/2point_dynamics.cu:391-546), each pair evaluated once, lower id first.
+void orc_set_contact_model(orc_box* b, int model);
 
 // Per-particle state, reference particle order.
 void orc_get_particles(const orc_box* b, double* x, double* y, double* phi, double* R, double* A,

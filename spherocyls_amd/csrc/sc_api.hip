@@ -16,9 +16,9 @@
 
 namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
-template <int MODE>
+template <int MODE, bool TWO_POINT>
 __global__ void k_tile(TileArgs a);
-template <int MODE, int LANES>
+template <int MODE, int LANES, bool TWO_POINT>
 __global__ void k_list(ListArgs a);
 __global__ void k_reduce(ReduceArgs a);
 __global__ void k_segvec(Bank b, int ntot);
@@ -106,6 +106,7 @@ struct sc_system {
   int tile_cols = 12, block = 128, list_block = 256, list_lanes = 1;
   int lds_max = 64 * 1024;
   int prec = SC_PREC_FP64;
+  int contact_model = SC_CONTACT_CLOSEST;
   int ncells = 0, ntiles = 0, npblocks = 0;
   std::vector<int4> tiles;
   std::vector<int2> pblocks, tile_ranges, list_ranges;
@@ -436,13 +437,17 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.resize = ps.resize;
     a.shrink = ps.shrink;
     a.corrected_image = ps.corrected_image;
+    a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     a.npack = s->npack;
     a.list_groups = s->block / kLanes;
     a.stage_cap = 3 * (s->tile_cols + 2) * kCap;
     size_t lds = tile_lds_bytes(a.stage_cap, a.list_groups, s->block);
     if (lds > (size_t)s->lds_max) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
-    auto fn = ps.mode == kModeForces ? &k_tile<kModeForces>
-                                     : (ps.mode == kModePred ? &k_tile<kModePred> : &k_tile<kModeCorr>);
+    using TFn = void (*)(TileArgs);
+    static const TFn ttable[2][3] = {
+        {&k_tile<kModeForces, false>, &k_tile<kModePred, false>, &k_tile<kModeCorr, false>},
+        {&k_tile<kModeForces, true>, &k_tile<kModePred, true>, &k_tile<kModeCorr, true>}};
+    TFn fn = ttable[a.two_point ? 1 : 0][ps.mode];
     if (lds > 64 * 1024)
       HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -495,16 +500,18 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.resize = ps.resize;
     a.shrink = ps.shrink;
     a.corrected_image = ps.corrected_image;
+    a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     using Fn = void (*)(ListArgs);
-    static const Fn table[3][4] = {
-        {&k_list<kModeForces, 1>, &k_list<kModeForces, 2>, &k_list<kModeForces, 4>,
-         &k_list<kModeForces, 8>},
-        {&k_list<kModePred, 1>, &k_list<kModePred, 2>, &k_list<kModePred, 4>,
-         &k_list<kModePred, 8>},
-        {&k_list<kModeCorr, 1>, &k_list<kModeCorr, 2>, &k_list<kModeCorr, 4>,
-         &k_list<kModeCorr, 8>}};
+#define SC_LIST_ROW(M, T) \
+  {&k_list<M, 1, T>, &k_list<M, 2, T>, &k_list<M, 4, T>, &k_list<M, 8, T>}
+    static const Fn table[2][3][4] = {
+        {SC_LIST_ROW(kModeForces, false), SC_LIST_ROW(kModePred, false),
+         SC_LIST_ROW(kModeCorr, false)},
+        {SC_LIST_ROW(kModeForces, true), SC_LIST_ROW(kModePred, true),
+         SC_LIST_ROW(kModeCorr, true)}};
+#undef SC_LIST_ROW
     const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
-    Fn fn = table[ps.mode][li];
+    Fn fn = table[a.two_point ? 1 : 0][ps.mode][li];
     KernelTimer t(s, ps.kernel_id);
     hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block), 0, s->stream, a);
     s->kseq++;
@@ -872,6 +879,18 @@ int sc_set_precision(sc_system* s, int mode) {
   HIPCK(hipStreamSynchronize(s->stream));
   return SC_OK;
   API_END
+}
+
+int sc_set_contact_model(sc_system* s, int model) {
+  if (!s || (model != SC_CONTACT_CLOSEST && model != SC_CONTACT_TWO_POINT)) return SC_ERR_ARG;
+  s->contact_model = model;
+  return SC_OK;
+}
+
+int sc_get_contact_model(const sc_system* s, int* model) {
+  if (!s || !model) return SC_ERR_ARG;
+  *model = s->contact_model;
+  return SC_OK;
 }
 
 int sc_set_path(sc_system* s, int path) {
@@ -1406,6 +1425,37 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
 }
 
 }  // extern "C"
+
+namespace sc {
+int system_view(sc_system* s, SystemView* v) {
+  if (!s || !v) return SC_ERR_ARG;
+  v->device = s->device;
+  v->stream = s->stream;
+  v->ntot = s->ntot;
+  v->npack = s->npack;
+  v->ncells = s->ncells;
+  v->b = s->B();
+  v->packs = s->d_packs;
+  v->scal = s->d_scal[s->cur_scal];
+  v->cnt = s->d_cnt[s->cur_buf];
+  v->slot = s->d_slot[s->cur_buf];
+  v->pack_of = s->d_pack_of;
+  return SC_OK;
+}
+
+int system_positions_changed(sc_system* s) {
+  if (!s) return SC_ERR_ARG;
+  API_BEGIN
+  s->cur_buf = 0;
+  bin_all(s, 0, 1);
+  segvec(s);
+  resort(s);
+  return check_errors(s);
+  API_END
+}
+}  // namespace sc
+
+
 
 namespace {
 // reconfigure_cells (src/cpp/resize_dynamics.cpp:106-121): when the box length

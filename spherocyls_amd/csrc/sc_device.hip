@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
 // ---------------------------------------------------------------------------
 // the tile force kernel
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, bool TWO_POINT>
 __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
 #pragma clang fp contract(off)
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -394,45 +394,36 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
       double ddy = min_image(L.y[p] - L.y[q], Ly);
       ddx += gam * ddy;
       const double sig = L.R[p] + L.R[q];
-      PairOut<double> r;
-      bool hit;
-      if (fp64) {
-        hit = pair_contact_f64(ddx, ddy, L.nx[p], L.ny[p], L.nx[q], L.ny[q], L.A[p], L.A[q], sig, r);
-      } else {
-        PairOut<float> rf;
-        hit = pair_contact_f32((float)ddx, (float)ddy, (float)L.nx[p], (float)L.ny[p],
-                               (float)L.nx[q], (float)L.ny[q], (float)L.A[p], (float)L.A[q],
-                               (float)sig, rf);
-        r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
-        r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
-      }
-      if (!hit) continue;
-      if (iP) {
-        Fx += r.fx;
-        Fy += r.fy;
-        Tq += r.tp;
-        if (do_se) {
-          // contact arms, src/cpp/spherocyl_box.cpp:483-486
-          double nxA = L.nx[p], nyA = L.ny[p], nxB = L.nx[q], nyB = L.ny[q];
-          double cxa = r.s * nxA - 0.5 * r.dx;
-          double cya = r.s * nyA - 0.5 * r.dy;
-          double cxb = r.t * nxB + 0.5 * r.dx;
-          double cyb_ref = r.t * nxB + 0.5 * r.dy;
-          double cyb = r.t * nyB + 0.5 * r.dy;
-          acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
-          acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
-          acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
-          acc[kSePxy] += r.fy * cxa;
-          acc[kSeCPxx] += r.fx * (cxa - cxb);
-          acc[kSeCPyy] += r.fy * (cya - cyb);
-          acc[kSeCPxy] += r.fy * (cxa - cxb);
-          acc[kSeCPyx] += r.fx * (cya - cyb);
-          acc[kSeNc] += 1.0;
+      PairOut<double> r0, r1;
+      bool h0, h1;
+      pair_contacts<TWO_POINT>(ddx, ddy, L.nx[p], L.ny[p], L.nx[q], L.ny[q], L.A[p], L.A[q], sig,
+                               fp64, r0, h0, r1, h1);
+      if (h0) {
+        if (iP) {
+          Fx += r0.fx;
+          Fy += r0.fy;
+          Tq += r0.tp;
+          if (do_se) {
+            add_contact_sums(r0, L.nx[p], L.ny[p], L.nx[q], L.ny[q], sig, acc);
+            acc[kSeNc] += 1.0;
+          }
+        } else {
+          Fx -= r0.fx;
+          Fy -= r0.fy;
+          Tq += r0.tq;
         }
-      } else {
-        Fx -= r.fx;
-        Fy -= r.fy;
-        Tq += r.tq;
+      }
+      if (TWO_POINT && h1) {
+        if (iP) {
+          Fx += r1.fx;
+          Fy += r1.fy;
+          Tq += r1.tp;
+          if (do_se) add_contact_sums(r1, L.nx[p], L.ny[p], L.nx[q], L.ny[q], sig, acc);
+        } else {
+          Fx -= r1.fx;
+          Fy -= r1.fy;
+          Tq += r1.tq;
+        }
       }
     }
     for (int off = LN >> 1; off > 0; off >>= 1) {
@@ -574,9 +565,12 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
   if (tid < kNSum) a.tile_part[(size_t)blockIdx.x * kNSum + tid] = L.red[tid];
 }
 
-template __global__ void k_tile<kModeForces>(TileArgs);
-template __global__ void k_tile<kModePred>(TileArgs);
-template __global__ void k_tile<kModeCorr>(TileArgs);
+template __global__ void k_tile<kModeForces, false>(TileArgs);
+template __global__ void k_tile<kModePred, false>(TileArgs);
+template __global__ void k_tile<kModeCorr, false>(TileArgs);
+template __global__ void k_tile<kModeForces, true>(TileArgs);
+template __global__ void k_tile<kModePred, true>(TileArgs);
+template __global__ void k_tile<kModeCorr, true>(TileArgs);
 
 // ---------------------------------------------------------------------------
 // particle-parallel kernels

@@ -109,6 +109,7 @@ struct TileArgs {
   int npack;
   int stage_cap;          // LDS staging capacity (entries)
   int list_groups;        // LDS contact-list slots
+  int two_point;          // two-point contact model instead of the closest point
 };
 
 // Neighbour-list force path.  Each particle keeps a list of candidate partners
@@ -145,6 +146,7 @@ struct ListArgs {
   int ntot, npack;
   int se_every, se_always, do_fire, do_gd, do_resid;
   int resize, shrink, corrected_image;
+  int two_point;           // two-point contact model instead of the closest point
 };
 
 // Arguments of the gradient-descent / FIRE update kernel.
@@ -179,4 +181,28 @@ struct ReduceArgs {
   int npack;
 };
 
+// Device view of a system's current state, for other device code of this
+// library (the message-passing warm start).  Valid until the next call that
+// moves particles or re-sorts the bank.
+struct SystemView {
+  int device;
+  hipStream_t stream;
+  int ntot, npack, ncells;
+  Bank b;                  // particle arrays in storage (cell-sorted) order; b.id = caller index
+  const PackInfo* packs;
+  const PackScal* scal;    // current per-packing scalars
+  const int* cnt;          // exact cell buckets of the current state
+  const int* slot;
+  const int* pack_of;      // storage index -> packing
+};
+
 }  // namespace sc
+
+struct sc_system;
+namespace sc {
+int system_view(sc_system* s, SystemView* v);
+// After device code changed x, y or phi of the current bank: wrap, re-bin,
+// refresh segment vectors, re-sort and schedule a list rebuild.
+int system_positions_changed(sc_system* s);
+}  // namespace sc
+

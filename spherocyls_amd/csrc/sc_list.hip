@@ -98,7 +98,7 @@ __device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double
 // (Fx, Fy, Tq) and, for the lower-id side of the pair, into the stress/energy
 // sums.  The pair is evaluated in the reference's orientation (lower particle
 // id first, src/cpp/spherocyl_box.cpp:514-520) so both sides agree bitwise.
-template <bool SUMS>
+template <bool SUMS, bool TWO_POINT>
 __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, double nyi, double Ri,
                                           double Ai, int idi, double xq, double yq, double nxq,
                                           double nyq, double Rq, double Aq, int idq, double Lx,
@@ -113,43 +113,35 @@ __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, doub
   const double nxA = iP ? nxi : nxq, nyA = iP ? nyi : nyq;
   const double nxB = iP ? nxq : nxi, nyB = iP ? nyq : nyi;
   const double Ap = iP ? Ai : Aq, Aqq = iP ? Aq : Ai;
-  PairOut<double> r;
-  bool hit;
-  if (fp64) {
-    hit = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, r);
-  } else {
-    PairOut<float> rf;
-    hit = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB, (float)nyB,
-                           (float)Ap, (float)Aqq, (float)sig, rf);
-    r.fx = rf.fx; r.fy = rf.fy; r.tp = rf.tp; r.tq = rf.tq; r.s = rf.s; r.t = rf.t;
-    r.dx = rf.dx; r.dy = rf.dy; r.dist = rf.dist; r.dV = rf.dV;
-  }
-  if (!hit) return;
-  if (iP) {
-    Fx += r.fx;
-    Fy += r.fy;
-    Tq += r.tp;
-    if (SUMS && do_se) {
-      // contact arms, src/cpp/spherocyl_box.cpp:483-486
-      double cxa = r.s * nxA - 0.5 * r.dx;
-      double cya = r.s * nyA - 0.5 * r.dy;
-      double cxb = r.t * nxB + 0.5 * r.dx;
-      double cyb_ref = r.t * nxB + 0.5 * r.dy;
-      double cyb = r.t * nyB + 0.5 * r.dy;
-      acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
-      acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
-      acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
-      acc[kSePxy] += r.fy * cxa;
-      acc[kSeCPxx] += r.fx * (cxa - cxb);
-      acc[kSeCPyy] += r.fy * (cya - cyb);
-      acc[kSeCPxy] += r.fy * (cxa - cxb);
-      acc[kSeCPyx] += r.fx * (cya - cyb);
-      acc[kSeNc] += 1.0;
+  PairOut<double> r0, r1;
+  bool h0, h1;
+  pair_contacts<TWO_POINT>(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, fp64, r0, h0, r1, h1);
+  if (h0) {
+    if (iP) {
+      Fx += r0.fx;
+      Fy += r0.fy;
+      Tq += r0.tp;
+      if (SUMS && do_se) {
+        add_contact_sums(r0, nxA, nyA, nxB, nyB, sig, acc);
+        acc[kSeNc] += 1.0;
+      }
+    } else {
+      Fx -= r0.fx;
+      Fy -= r0.fy;
+      Tq += r0.tq;
     }
-  } else {
-    Fx -= r.fx;
-    Fy -= r.fy;
-    Tq += r.tq;
+  }
+  if (TWO_POINT && h1) {
+    if (iP) {
+      Fx += r1.fx;
+      Fy += r1.fy;
+      Tq += r1.tp;
+      if (SUMS && do_se) add_contact_sums(r1, nxA, nyA, nxB, nyB, sig, acc);
+    } else {
+      Fx -= r1.fx;
+      Fy -= r1.fy;
+      Tq += r1.tq;
+    }
   }
 }
 
@@ -159,7 +151,7 @@ __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, doub
 // combined with xor shuffles (fixed order for a given LANES).
 constexpr int kGather = 4;
 
-template <int MODE, int LANES>
+template <int MODE, int LANES, bool TWO_POINT>
 __global__ void __launch_bounds__(256) k_list(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double s_red[4 * kNSum];
@@ -263,8 +255,9 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
         const double reach = reach_i + Rq[j] + Aq[j];
         if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
         const int q = qs[j];
-        pair_into<kSums>(xi, yi, nxi, nyi, Ri, Ai, idi, xq[j], yq[j], a.enx[q], a.eny[q], Rq[j],
-                         Aq[j], a.b.id[q], Lx, Ly, gam, fp64, do_se, Fx, Fy, Tq, acc);
+        pair_into<kSums, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, xq[j], yq[j], a.enx[q],
+                                    a.eny[q], Rq[j], Aq[j], a.b.id[q], Lx, Ly, gam, fp64, do_se,
+                                    Fx, Fy, Tq, acc);
       }
     }
     if (LANES > 1) {
@@ -424,14 +417,18 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   }
 }
 
-#define SC_LIST_INST(L)                                       \
-  template __global__ void k_list<kModeForces, L>(ListArgs); \
-  template __global__ void k_list<kModePred, L>(ListArgs);   \
-  template __global__ void k_list<kModeCorr, L>(ListArgs);
-SC_LIST_INST(1)
-SC_LIST_INST(2)
-SC_LIST_INST(4)
-SC_LIST_INST(8)
+#define SC_LIST_INST(L, T)                                       \
+  template __global__ void k_list<kModeForces, L, T>(ListArgs); \
+  template __global__ void k_list<kModePred, L, T>(ListArgs);   \
+  template __global__ void k_list<kModeCorr, L, T>(ListArgs);
+SC_LIST_INST(1, false)
+SC_LIST_INST(2, false)
+SC_LIST_INST(4, false)
+SC_LIST_INST(8, false)
+SC_LIST_INST(1, true)
+SC_LIST_INST(2, true)
+SC_LIST_INST(4, true)
+SC_LIST_INST(8, true)
 #undef SC_LIST_INST
 
 // (nx, ny) = A (cos phi, sin phi) for every particle of the bank.

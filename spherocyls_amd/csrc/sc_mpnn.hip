@@ -1,0 +1,1026 @@
+// Message-passing network over the contact graph: warm starts for the
+// minimiser (no reference counterpart).
+//
+// Network (hidden width 64, three message-passing layers, SiLU):
+//   node input  x_i = (cos 2phi, sin 2phi, R, A)
+//   edge input  e_ij = (1 - d/sigma, D_x/d, D_y/d, b_x, b_y) for every contact
+//               (closest-point distance d < sigma), D the closest-approach
+//               vector from j to i and b = r_j - r_i (sheared coordinates,
+//               minimum image); both directions i<-j and j<-i are edges
+//   h0_i   = silu(W_e [x_i, 1])
+//   layer: z_ij = W_m [h_i, h_j, e_ij, 1];  M_i = sum_j silu(z_ij)
+//          h_i <- h_i + silu(W_u [h_i, M_i, 1])
+//   output y_i = W_o [h_i, 1]  (first three components), the displacement
+//          (dx, dy, dphi) = scale * y_i in sheared coordinates.
+// Every affine map carries its bias as the last weight row against a constant
+// 1 column of its input, so the forward and backward passes are plain GEMMs:
+//   forward    C[M x N] = A[M x K] . W[K x N]       (k_gemm_nt with Bt = W^T)
+//   input grad dA       = dC . W^T                  (k_gemm_nt with Bt = W)
+//   weight grad dW      = A^T . dC                   (k_gemm_tn, per-chunk
+//                                                     partials summed in order)
+// GEMM operands are BF16 and accumulate in FP32 on the MFMA units
+// (v_mfma_f32_16x16x16bf16_1k: a wavefront computes 16-row strips, one
+// 16 x 16 tile per MFMA); the residual stream h, pre-activations, gradients,
+// FP32 master weights and Adam moments stay FP32.  Edges are stored grouped by
+// receiver (CSR), with the index of each edge's reverse, so every sum over
+// edges runs in a fixed order: forward and backward are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spherocyls.h"
+#include "../../include/spherocyls_mpnn.h"
+#include "sc_internal.h"
+#include "sc_physics.h"
+
+namespace sc {
+namespace mp {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+constexpr int kH = 64;        // hidden width
+constexpr int kLayers = 3;
+constexpr int kKx = 16;       // [x(4), 1, 0..]
+constexpr int kKe = 144;      // [h_i, h_j, e(5), 1, 0..]
+constexpr int kKu = 144;      // [h, M, 1, 0..]
+constexpr int kKo = 80;       // [h, 1, 0..]
+constexpr int kNo = 16;       // output width (3 used)
+constexpr int kEdgeFeat = 5;
+constexpr int kNodeFeat = 4;
+constexpr int kChunk = 2048;  // rows per weight-gradient partial
+
+// parameter blocks (FP32 master layout, each W row-major [K][N])
+struct Layout {
+  int we, wm[kLayers], wu[kLayers], wo, total;
+  Layout() {
+    int o = 0;
+    we = o; o += kKx * kH;
+    for (int l = 0; l < kLayers; ++l) { wm[l] = o; o += kKe * kH; wu[l] = o; o += kKu * kH; }
+    wo = o; o += kKo * kNo;
+    total = o;
+  }
+};
+
+__device__ __forceinline__ bf16 to_bf16(float v) {
+  // round to nearest even
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// ---------------------------------------------------------------------------
+// GEMMs
+// ---------------------------------------------------------------------------
+enum Epi : int {
+  kEpiStore = 0,     // C32 = acc
+  kEpiSiluH = 1,     // C32 = acc (pre-activation); H32 = silu(acc)
+  kEpiResid = 2,     // C32 = acc; H32 = Hin32 + silu(acc)
+};
+
+// C[M x NT*16] = A[M x K] . Bt^T, A row-major (lda), Bt row-major [N][ldb].
+// One wavefront per 16-row strip, four per workgroup.  K % 16 == 0,
+// lda % 4 == 0, ldb % 4 == 0 (8-byte operand loads).
+template <int NT, int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int lda,
+                                                  const bf16* __restrict__ Bt, int ldb, int M, int K,
+                                                  float* __restrict__ C, int ldc,
+                                                  const float* __restrict__ Hin,
+                                                  float* __restrict__ Hout, int ldh) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int r0 = (blockIdx.x * 4 + wave) * 16;
+  if (r0 >= M) return;
+  const int ar = r0 + (lane & 15);
+  const int kq = 4 * (lane >> 4);
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4 a = bf16x4{0, 0, 0, 0};
+    if (ar < M) a = *reinterpret_cast<const bf16x4*>(A + (size_t)ar * lda + k0 + kq);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x4 b =
+          *reinterpret_cast<const bf16x4*>(Bt + (size_t)(t * 16 + (lane & 15)) * ldb + k0 + kq);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  // acc[t][i] = C[r0 + 4 (lane / 16) + i][t * 16 + lane % 16]
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int col = t * 16 + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = r0 + kq + i;
+      if (row >= M) continue;
+      const float v = acc[t][i];
+      C[(size_t)row * ldc + col] = v;
+      if (EPI == kEpiSiluH) Hout[(size_t)row * ldh + col] = silu(v);
+      if (EPI == kEpiResid)
+        Hout[(size_t)row * ldh + col] = Hin[(size_t)row * ldh + col] + silu(v);
+    }
+  }
+}
+
+// Partial weight gradients: P[chunk][K][N] = A[rows of chunk]^T . D[rows of
+// chunk], A row-major [M][lda] (K columns), D row-major [M][ldd] (N = NTN*16
+// columns).  One workgroup per chunk of kChunk rows; its four wavefronts share
+// the (K/16) x NTN output tiles.
+template <int NTN>
+__global__ void __launch_bounds__(256) k_gemm_tn(const bf16* __restrict__ A, int lda,
+                                                  const bf16* __restrict__ D, int ldd, int M, int K,
+                                                  float* __restrict__ P) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int m_begin = blockIdx.x * kChunk;
+  const int m_end = min(M, m_begin + kChunk);
+  const int KT = K / 16;
+  const int ntiles = KT * NTN;
+  float* Pc = P + (size_t)blockIdx.x * K * NTN * 16;
+  const int kq = 4 * (lane >> 4);
+  for (int tile = wave; tile < ntiles; tile += 4) {
+    const int kt = tile / NTN, nt = tile % NTN;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m0 = m_begin; m0 < m_end; m0 += 16) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + kq + i;
+        const bool ok = m < m_end;
+        // A operand = A^T tile: row (lane % 16) of the K block, column m
+        a[i] = ok ? (short)A[(size_t)m * lda + kt * 16 + (lane & 15)] : (short)0;
+        // B operand = D tile: row m, column (lane % 16) of the N block
+        b[i] = ok ? (short)D[(size_t)m * ldd + nt * 16 + (lane & 15)] : (short)0;
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      Pc[(size_t)(kt * 16 + kq + i) * (NTN * 16) + nt * 16 + (lane & 15)] = acc[i];
+  }
+}
+
+// G[i] (+)= sum over chunks of P[chunk][i], chunks in order.
+__global__ void k_sum_partials(const float* __restrict__ P, int nchunks, int size,
+                               float* __restrict__ G, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= size) return;
+  float s = accumulate ? G[i] : 0.0f;
+  for (int c = 0; c < nchunks; ++c) s += P[(size_t)c * size + i];
+  G[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// graph input builders (BF16 GEMM operands with the constant-1 column)
+// ---------------------------------------------------------------------------
+__global__ void k_build_x(const float* __restrict__ nf, int n, bf16* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bf16* r = X + (size_t)i * kKx;
+  for (int c = 0; c < kKx; ++c) {
+    float v = c < kNodeFeat ? nf[(size_t)i * kNodeFeat + c] : (c == kNodeFeat ? 1.0f : 0.0f);
+    r[c] = to_bf16(v);
+  }
+}
+
+// Ein[e] = [h_dst, h_src, e_feat, 1, 0...]; one thread per (edge, column).
+__global__ void k_build_edge_in(const float* __restrict__ H, const int* __restrict__ src,
+                                const int* __restrict__ dst, const float* __restrict__ ef, int ne,
+                                bf16* __restrict__ Ein) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ne * kKe) return;
+  const int e = (int)(t / kKe), c = (int)(t % kKe);
+  float v;
+  if (c < kH) v = H[(size_t)dst[e] * kH + c];
+  else if (c < 2 * kH) v = H[(size_t)src[e] * kH + (c - kH)];
+  else if (c < 2 * kH + kEdgeFeat) v = ef[(size_t)e * kEdgeFeat + (c - 2 * kH)];
+  else v = c == 2 * kH + kEdgeFeat ? 1.0f : 0.0f;
+  Ein[t] = to_bf16(v);
+}
+
+// Uin[i] = [h_i, sum over in-edges of silu(z), 1, 0...]; one thread per (node, column).
+__global__ void k_build_node_in(const float* __restrict__ H, const float* __restrict__ Z,
+                                const int* __restrict__ off, int n, bf16* __restrict__ Uin) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * kKu) return;
+  const int i = (int)(t / kKu), c = (int)(t % kKu);
+  float v;
+  if (c < kH) {
+    v = H[(size_t)i * kH + c];
+  } else if (c < 2 * kH) {
+    v = 0.0f;
+    for (int e = off[i]; e < off[i + 1]; ++e) v += silu(Z[(size_t)e * kH + (c - kH)]);
+  } else {
+    v = c == 2 * kH ? 1.0f : 0.0f;
+  }
+  Uin[t] = to_bf16(v);
+}
+
+__global__ void k_build_out_in(const float* __restrict__ H, int n, bf16* __restrict__ Hin) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * kKo) return;
+  const int i = (int)(t / kKo), c = (int)(t % kKo);
+  float v = c < kH ? H[(size_t)i * kH + c] : (c == kH ? 1.0f : 0.0f);
+  Hin[t] = to_bf16(v);
+}
+
+// ---------------------------------------------------------------------------
+// backward helpers
+// ---------------------------------------------------------------------------
+// loss = mean over nodes and the 3 outputs of (y - t)^2; dY (BF16, 16 wide).
+// Per-block partial losses, summed in order by k_sum_partials.
+__global__ void k_loss(const float* __restrict__ Y, const float* __restrict__ T, int n,
+                       bf16* __restrict__ dY, float* __restrict__ part) {
+  __shared__ float sh[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < n) {
+    const float inv = 1.0f / (3.0f * (float)n);
+    for (int c = 0; c < kNo; ++c) {
+      float g = 0.0f;
+      if (c < 3) {
+        float d = Y[(size_t)i * kNo + c] - T[(size_t)i * 3 + c];
+        l += d * d * inv;
+        g = 2.0f * d * inv;
+      }
+      dY[(size_t)i * kNo + c] = to_bf16(g);
+    }
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// dPre = dH (given as columns [col0, col0+64) of dIn, row stride ldin) * silu'(Pre)
+__global__ void k_act_grad(const float* __restrict__ dIn, int ldin, int col0,
+                           const float* __restrict__ Pre, int rows, bf16* __restrict__ dPre) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)rows * kH) return;
+  const int r = (int)(t / kH), c = (int)(t % kH);
+  dPre[t] = to_bf16(dIn[(size_t)r * ldin + col0 + c] * silu_grad(Pre[t]));
+}
+
+// dH[i][c] = dIn[i][c], c < 64 (row stride ld)
+__global__ void k_take_cols(const float* __restrict__ dIn, int ld, int n, float* __restrict__ dH) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * kH) return;
+  dH[t] = dIn[(t / kH) * ld + t % kH];
+}
+
+// Edge pre-activation gradient: dZ[e] = dM[dst(e)] * silu'(Z[e]); dM is
+// columns [64, 128) of dUin (row stride kKu).
+__global__ void k_edge_grad(const float* __restrict__ dUin, const float* __restrict__ Z,
+                            const int* __restrict__ dst, int ne, bf16* __restrict__ dZ) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ne * kH) return;
+  const int e = (int)(t / kH), c = (int)(t % kH);
+  dZ[t] = to_bf16(dUin[(size_t)dst[e] * kKu + kH + c] * silu_grad(Z[t]));
+}
+
+// dH_prev[i] = dH[i] + dUin[i][0:64] + sum over in-edges e of dEin[e][0:64]
+//             + sum over out-edges (= reverses of in-edges) of dEin[rev e][64:128]
+__global__ void k_node_grad(const float* __restrict__ dH, const float* __restrict__ dUin,
+                            const float* __restrict__ dEin, const int* __restrict__ off,
+                            const int* __restrict__ rev, int n, float* __restrict__ dHprev) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * kH) return;
+  const int i = (int)(t / kH), c = (int)(t % kH);
+  float v = dH[t] + dUin[(size_t)i * kKu + c];
+  for (int e = off[i]; e < off[i + 1]; ++e)
+    v += dEin[(size_t)e * kKe + c] + dEin[(size_t)rev[e] * kKe + kH + c];
+  dHprev[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+// BF16 copies of every W: natural [K][N] (input gradients) and transposed
+// [N][K] (forward).
+__global__ void k_refresh(const float* __restrict__ W, int K, int N, bf16* __restrict__ Wn,
+                          bf16* __restrict__ Wt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= K * N) return;
+  const int k = t / N, n = t % N;
+  const bf16 v = to_bf16(W[t]);
+  Wn[t] = v;
+  Wt[(size_t)n * K + k] = v;
+}
+
+__global__ void k_adam(float* __restrict__ W, const float* __restrict__ G, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = G[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * g;
+  const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  W[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// graph of a system's current state
+// ---------------------------------------------------------------------------
+// Closest-point contact of storage particles g and q evaluated with the lower
+// caller id first (as the force kernels do), so both directions of a pair see
+// the same contact; r.dx, r.dy are turned to point from q to g.
+__device__ bool pair_canonical(const SystemView& v, int g, int q, double Lx, double Ly,
+                               double gam, PairOut<double>& r) {
+  const bool gFirst = v.b.id[g] < v.b.id[q];
+  const int p = gFirst ? g : q, o = gFirst ? q : g;
+  double dx = min_image(v.b.x[p] - v.b.x[o], Lx);
+  double dy = min_image(v.b.y[p] - v.b.y[o], Ly);
+  dx += gam * dy;
+  if (!pair_contact_f64(dx, dy, v.b.nx[p], v.b.ny[p], v.b.nx[o], v.b.ny[o], v.b.A[p], v.b.A[o],
+                        v.b.R[p] + v.b.R[o], r))
+    return false;
+  if (!gFirst) {
+    r.dx = -r.dx;
+    r.dy = -r.dy;
+  }
+  return true;
+}
+
+// Contacts of storage particle g (closest-point distance < sigma) over the 3 x 3
+// cells around it, partner storage indices sorted by caller id; count only when
+// out is null.
+__device__ int contacts_of(const SystemView& v, int g, int* out, int cap) {
+  const int pk = v.pack_of[g];
+  const PackInfo P = v.packs[pk];
+  const PackScal S = v.scal[pk];
+  const double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
+  double xw = v.b.x[g], yw = v.b.y[g];
+  wrap_coord(xw, Lx);
+  wrap_coord(yw, Ly);
+  const int col = min(max((int)(xw / (Lx / P.cols)), 0), P.cols - 1);
+  const int row = min(max((int)(yw / (Ly / P.rows)), 0), P.rows - 1);
+  int rs[3], cs[3], nr = 0, nc = 0;
+  for (int d = -1; d <= 1; ++d) {
+    int r = (row + d + P.rows) % P.rows, c = (col + d + P.cols) % P.cols;
+    bool dr = false, dc = false;
+    for (int k = 0; k < nr; ++k) dr |= rs[k] == r;
+    for (int k = 0; k < nc; ++k) dc |= cs[k] == c;
+    if (!dr) rs[nr++] = r;
+    if (!dc) cs[nc++] = c;
+  }
+  int n = 0;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) {
+      const int cell = P.coff + rs[i] * P.cols + cs[j];
+      const int m = min(v.cnt[cell], kCap);
+      for (int k = 0; k < m; ++k) {
+        const int q = v.slot[(size_t)cell * kCap + k];
+        if (q == g) continue;
+        PairOut<double> r;
+        if (!pair_canonical(v, g, q, Lx, Ly, gam, r)) continue;
+        if (out) {
+          // insertion by caller id keeps every node's edge order fixed
+          int pos = min(n, cap - 1);
+          const int idq = v.b.id[q];
+          while (pos > 0 && v.b.id[out[pos - 1]] > idq) {
+            if (pos < cap) out[pos] = out[pos - 1];
+            --pos;
+          }
+          if (n < cap) out[pos] = q;
+        }
+        ++n;
+      }
+    }
+  return n;
+}
+
+__global__ void k_graph_count(SystemView v, int* deg) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= v.ntot) return;
+  deg[g] = contacts_of(v, g, nullptr, 0);
+}
+
+// Nodes in storage order; in-edges of node g (receiver g, senders = contacts)
+// at off[g] .. off[g+1]-1.  Writes node features, edge src / dst and features.
+__global__ void k_graph_fill(SystemView v, const int* __restrict__ off, int* __restrict__ src,
+                             int* __restrict__ dst, float* __restrict__ nf,
+                             float* __restrict__ ef) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= v.ntot) return;
+  const double phi = v.b.phi[g];
+  nf[(size_t)g * kNodeFeat + 0] = (float)cos(2.0 * phi);
+  nf[(size_t)g * kNodeFeat + 1] = (float)sin(2.0 * phi);
+  nf[(size_t)g * kNodeFeat + 2] = (float)v.b.R[g];
+  nf[(size_t)g * kNodeFeat + 3] = (float)v.b.A[g];
+  const int o = off[g], cap = off[g + 1] - o;
+  if (cap == 0) return;
+  contacts_of(v, g, src + o, cap);
+  const int pk = v.pack_of[g];
+  const PackScal S = v.scal[pk];
+  for (int k = 0; k < cap; ++k) {
+    const int e = o + k, q = src[e];
+    dst[e] = g;
+    const double bx = min_image(v.b.x[q] - v.b.x[g], S.Lx);
+    const double by = min_image(v.b.y[q] - v.b.y[g], S.Ly);
+    PairOut<double> r;
+    pair_canonical(v, g, q, S.Lx, S.Ly, S.gamma, r);
+    const double sig = v.b.R[g] + v.b.R[q];
+    const double d = r.dist > 0 ? r.dist : 1e-300;
+    float* f = ef + (size_t)e * kEdgeFeat;
+    f[0] = (float)(1.0 - r.dist / sig);
+    f[1] = (float)(r.dx / d);
+    f[2] = (float)(r.dy / d);
+    f[3] = (float)bx;
+    f[4] = (float)by;
+  }
+}
+
+// x, y, phi of storage particle g += scale * y_g
+__global__ void k_apply(Bank b, const float* __restrict__ Y, float sx, float sy, float sp,
+                        int n) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  b.x[g] += (double)(sx * Y[(size_t)g * kNo + 0]);
+  b.y[g] += (double)(sy * Y[(size_t)g * kNo + 1]);
+  b.phi[g] += (double)(sp * Y[(size_t)g * kNo + 2]);
+}
+
+}  // namespace mp
+}  // namespace sc
+
+using namespace sc;
+using namespace sc::mp;
+
+struct sc_mpnn {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Layout lay;
+  float scale[3] = {1.0f, 1.0f, 1.0f};
+  std::string err;
+  std::vector<void*> allocs;
+  // parameters
+  float *W = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr;
+  bf16 *Wn = nullptr, *Wt = nullptr;  // same offsets as W
+  // graph (capacity)
+  int n = 0, e = 0, ncap = 0, ecap = 0;
+  bool has_forward = false;
+  int *off = nullptr, *src = nullptr, *dst = nullptr, *rev = nullptr;
+  float *nf = nullptr, *ef = nullptr;
+  // activations
+  bf16* X = nullptr;
+  float* Ze = nullptr;                 // encoder pre-activation [n][64]
+  float* Hs[kLayers + 1] = {};         // residual stream per layer [n][64]
+  bf16* Ein[kLayers] = {};             // [e][144]
+  float* Z[kLayers] = {};              // [e][64]
+  bf16* Uin[kLayers] = {};             // [n][144]
+  float* U[kLayers] = {};              // [n][64]
+  bf16* Hin = nullptr;                 // [n][80]
+  float* Y = nullptr;                  // [n][16]
+  float* T = nullptr;                  // [n][3] targets
+  // backward scratch
+  bf16 *dY = nullptr, *dPre = nullptr, *dZe = nullptr;
+  float *dA = nullptr, *dH = nullptr, *dH2 = nullptr, *dEin = nullptr, *P = nullptr;
+  float* part = nullptr;
+  size_t pcap = 0;
+  int* d_deg = nullptr;
+
+  template <typename T>
+  T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+      throw std::string("hipMalloc failed");
+    allocs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  void dfree(void*& p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+  }
+};
+
+namespace {
+
+#define MPCK(expr)                                                      \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) throw std::string(#expr) + ": " + hipGetErrorString(e_); \
+  } while (0)
+
+int fail(sc_mpnn* m, int st, const std::string& msg) {
+  if (m) m->err = msg;
+  return st;
+}
+
+inline int blocks(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+template <class T>
+void realloc_dev(sc_mpnn* m, T*& p, size_t count) {
+  void* q = p;
+  m->dfree(q);
+  p = m->template dalloc<T>(count);
+}
+
+void ensure_capacity(sc_mpnn* m, int n, int e) {
+  if (n <= m->ncap && e <= m->ecap) return;
+  const int nc = std::max(n, m->ncap), ec = std::max(e, m->ecap);
+  realloc_dev(m, m->off, nc + 1);
+  realloc_dev(m, m->src, ec);
+  realloc_dev(m, m->dst, ec);
+  realloc_dev(m, m->rev, ec);
+  realloc_dev(m, m->nf, (size_t)nc * kNodeFeat);
+  realloc_dev(m, m->ef, (size_t)ec * kEdgeFeat);
+  realloc_dev(m, m->X, (size_t)nc * kKx);
+  realloc_dev(m, m->Ze, (size_t)nc * kH);
+  for (int l = 0; l <= kLayers; ++l) realloc_dev(m, m->Hs[l], (size_t)nc * kH);
+  for (int l = 0; l < kLayers; ++l) {
+    realloc_dev(m, m->Ein[l], (size_t)ec * kKe);
+    realloc_dev(m, m->Z[l], (size_t)ec * kH);
+    realloc_dev(m, m->Uin[l], (size_t)nc * kKu);
+    realloc_dev(m, m->U[l], (size_t)nc * kH);
+  }
+  realloc_dev(m, m->Hin, (size_t)nc * kKo);
+  realloc_dev(m, m->Y, (size_t)nc * kNo);
+  realloc_dev(m, m->T, (size_t)nc * 3);
+  realloc_dev(m, m->dY, (size_t)nc * kNo);
+  realloc_dev(m, m->dPre, (size_t)std::max(nc, ec) * kH);
+  realloc_dev(m, m->dZe, (size_t)nc * kH);
+  realloc_dev(m, m->dA, (size_t)nc * kKu);
+  realloc_dev(m, m->dH, (size_t)nc * kH);
+  realloc_dev(m, m->dH2, (size_t)nc * kH);
+  realloc_dev(m, m->dEin, (size_t)ec * kKe);
+  realloc_dev(m, m->d_deg, nc + 1);
+  const size_t chunks = (size_t)std::max(nc, ec) / kChunk + 1;
+  m->pcap = chunks * kKe * kH;
+  realloc_dev(m, m->P, m->pcap);
+  realloc_dev(m, m->part, (size_t)blocks(nc) + 1);
+  m->ncap = nc;
+  m->ecap = ec;
+}
+
+void refresh_bf16(sc_mpnn* m) {
+  const Layout& L = m->lay;
+  auto one = [&](int o, int K, int N) {
+    hipLaunchKernelGGL(k_refresh, dim3(blocks((size_t)K * N)), dim3(256), 0, m->stream, m->W + o,
+                       K, N, m->Wn + o, m->Wt + o);
+  };
+  one(L.we, kKx, kH);
+  for (int l = 0; l < kLayers; ++l) {
+    one(L.wm[l], kKe, kH);
+    one(L.wu[l], kKu, kH);
+  }
+  one(L.wo, kKo, kNo);
+}
+
+template <int NT, int EPI>
+void gemm_nt(sc_mpnn* m, const bf16* A, int lda, const bf16* Bt, int ldb, int M, int K, float* C,
+             int ldc, const float* Hin = nullptr, float* Hout = nullptr, int ldh = 0) {
+  if (M <= 0) return;
+  hipLaunchKernelGGL((k_gemm_nt<NT, EPI>), dim3(blocks((size_t)M, 64)), dim3(256), 0, m->stream,
+                     A, lda, Bt, ldb, M, K, C, ldc, Hin, Hout, ldh);
+}
+
+template <int NTN>
+void gemm_tn(sc_mpnn* m, const bf16* A, int lda, const bf16* D, int ldd, int M, int K,
+             float* Gout) {
+  const int size = K * NTN * 16;
+  if (M <= 0) return;
+  const int chunks = (M + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL((k_gemm_tn<NTN>), dim3(chunks), dim3(256), 0, m->stream, A, lda, D, ldd, M, K,
+                     m->P);
+  hipLaunchKernelGGL(k_sum_partials, dim3(blocks((size_t)size)), dim3(256), 0, m->stream, m->P,
+                     chunks, size, Gout, 0);
+}
+
+void forward(sc_mpnn* m) {
+  const Layout& L = m->lay;
+  const int n = m->n, e = m->e;
+  hipStream_t st = m->stream;
+  hipLaunchKernelGGL(k_build_x, dim3(blocks(n)), dim3(256), 0, st, m->nf, n, m->X);
+  gemm_nt<4, kEpiSiluH>(m, m->X, kKx, m->Wt + L.we, kKx, n, kKx, m->Ze, kH, nullptr, m->Hs[0], kH);
+  for (int l = 0; l < kLayers; ++l) {
+    if (e > 0) {
+      hipLaunchKernelGGL(k_build_edge_in, dim3(blocks((size_t)e * kKe)), dim3(256), 0, st,
+                         m->Hs[l], m->src, m->dst, m->ef, e, m->Ein[l]);
+      gemm_nt<4, kEpiStore>(m, m->Ein[l], kKe, m->Wt + L.wm[l], kKe, e, kKe, m->Z[l], kH);
+    }
+    hipLaunchKernelGGL(k_build_node_in, dim3(blocks((size_t)n * kKu)), dim3(256), 0, st, m->Hs[l],
+                       m->Z[l], m->off, n, m->Uin[l]);
+    gemm_nt<4, kEpiResid>(m, m->Uin[l], kKu, m->Wt + L.wu[l], kKu, n, kKu, m->U[l], kH, m->Hs[l],
+                          m->Hs[l + 1], kH);
+  }
+  hipLaunchKernelGGL(k_build_out_in, dim3(blocks((size_t)n * kKo)), dim3(256), 0, st,
+                     m->Hs[kLayers], n, m->Hin);
+  gemm_nt<1, kEpiStore>(m, m->Hin, kKo, m->Wt + L.wo, kKo, n, kKo, m->Y, kNo);
+  m->has_forward = true;
+}
+
+float backward(sc_mpnn* m) {
+  const Layout& L = m->lay;
+  const int n = m->n, e = m->e;
+  hipStream_t st = m->stream;
+  const int nb = blocks(n);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, m->Y, m->T, n, m->dY, m->part);
+  float* dloss = m->P;  // scratch for the reduced loss
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1), 0, st, m->part, nb, 1, dloss, 0);
+  float loss = 0.0f;
+  MPCK(hipMemcpyAsync(&loss, dloss, sizeof(float), hipMemcpyDeviceToHost, st));
+  MPCK(hipStreamSynchronize(st));
+  // decoder
+  gemm_tn<1>(m, m->Hin, kKo, m->dY, kNo, n, kKo, m->G + L.wo);
+  gemm_nt<5, kEpiStore>(m, m->dY, kNo, m->Wn + L.wo, kNo, n, kNo, m->dA, kKo);
+  // gradient of the last hidden state: the h columns of dHin
+  hipLaunchKernelGGL(k_take_cols, dim3(blocks((size_t)n * kH)), dim3(256), 0, st, m->dA, kKo, n,
+                     m->dH);
+  float* dH = m->dH;
+  float* dH2 = m->dH2;
+  for (int l = kLayers - 1; l >= 0; --l) {
+    // node update: U = Uin . Wu, h_{l+1} = h_l + silu(U)
+    hipLaunchKernelGGL(k_act_grad, dim3(blocks((size_t)n * kH)), dim3(256), 0, st, dH, kH, 0,
+                       m->U[l], n, m->dPre);
+    gemm_tn<4>(m, m->Uin[l], kKu, m->dPre, kH, n, kKu, m->G + L.wu[l]);
+    gemm_nt<9, kEpiStore>(m, m->dPre, kH, m->Wn + L.wu[l], kH, n, kH, m->dA, kKu);
+    if (e > 0) {
+      hipLaunchKernelGGL(k_edge_grad, dim3(blocks((size_t)e * kH)), dim3(256), 0, st, m->dA,
+                         m->Z[l], m->dst, e, m->dPre);
+      gemm_tn<4>(m, m->Ein[l], kKe, m->dPre, kH, e, kKe, m->G + L.wm[l]);
+      gemm_nt<9, kEpiStore>(m, m->dPre, kH, m->Wn + L.wm[l], kH, e, kH, m->dEin, kKe);
+    } else {
+      MPCK(hipMemsetAsync(m->G + L.wm[l], 0, sizeof(float) * kKe * kH, st));
+    }
+    hipLaunchKernelGGL(k_node_grad, dim3(blocks((size_t)n * kH)), dim3(256), 0, st, dH, m->dA,
+                       m->dEin, m->off, m->rev, n, dH2);
+    std::swap(dH, dH2);
+  }
+  // encoder
+  hipLaunchKernelGGL(k_act_grad, dim3(blocks((size_t)n * kH)), dim3(256), 0, st, dH, kH, 0, m->Ze,
+                     n, m->dZe);
+  gemm_tn<4>(m, m->X, kKx, m->dZe, kH, n, kKx, m->G + L.we);
+  return loss;
+}
+
+void init_params(sc_mpnn* m, unsigned long long seed) {
+  // SplitMix64 stream, uniform(-b, b) with b = sqrt(3 / fan_in) on real inputs;
+  // bias rows and padding rows start at zero
+  uint64_t z = seed;
+  auto next = [&]() {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+    r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+    return r ^ (r >> 31);
+  };
+  auto uni = [&]() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); };
+  const Layout& L = m->lay;
+  std::vector<float> h(L.total, 0.0f);
+  auto fill = [&](int o, int K, int N, int fan_in, double gain) {
+    double b = gain * std::sqrt(3.0 / fan_in);
+    for (int k = 0; k < fan_in; ++k)
+      for (int c = 0; c < N; ++c) h[o + k * N + c] = (float)((2.0 * uni() - 1.0) * b);
+    (void)K;
+  };
+  fill(L.we, kKx, kH, kNodeFeat, 1.0);
+  for (int l = 0; l < kLayers; ++l) {
+    fill(L.wm[l], kKe, kH, 2 * kH + kEdgeFeat, 1.0);
+    fill(L.wu[l], kKu, kH, 2 * kH, 0.5);
+  }
+  fill(L.wo, kKo, kNo, kH, 0.1);
+  for (int k = 0; k < kKo; ++k)
+    for (int c = 3; c < kNo; ++c) h[L.wo + k * kNo + c] = 0.0f;
+  MPCK(hipMemcpy(m->W, h.data(), sizeof(float) * L.total, hipMemcpyHostToDevice));
+  MPCK(hipMemset(m->G, 0, sizeof(float) * L.total));
+  MPCK(hipMemset(m->Am, 0, sizeof(float) * L.total));
+  MPCK(hipMemset(m->Av, 0, sizeof(float) * L.total));
+  refresh_bf16(m);
+  MPCK(hipStreamSynchronize(m->stream));
+}
+
+// Host-side CSR bookkeeping shared by load_graph and graph_from_system:
+// rev[e] = index of the edge dst->src.  Edges grouped by dst, ordered by src
+// within a group.
+int build_rev(int n, const std::vector<int>& off, const std::vector<int>& src,
+              const std::vector<int>& dst, std::vector<int>& rev) {
+  const int e = (int)src.size();
+  rev.assign(e, -1);
+  for (int k = 0; k < e; ++k) {
+    const int i = dst[k], j = src[k];
+    if (i < 0 || i >= n || j < 0 || j >= n) return SC_ERR_ARG;
+    // reverse edge lives in group j, with src == i (binary search: sorted by src)
+    int lo = off[j], hi = off[j + 1];
+    while (lo < hi) {
+      int mid = (lo + hi) / 2;
+      if (src[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= off[j + 1] || src[lo] != i) return SC_ERR_ARG;
+    rev[k] = lo;
+  }
+  return SC_OK;
+}
+
+}  // namespace
+
+#define MP_BEGIN                 \
+  if (!m) return SC_ERR_ARG;     \
+  try {                          \
+    MPCK(hipSetDevice(m->device));
+#define MP_END                              \
+  }                                         \
+  catch (const std::string& s) {            \
+    return fail(m, SC_ERR_HIP, s);          \
+  }
+
+extern "C" {
+
+int sc_mpnn_create(sc_mpnn** out, int device, unsigned long long seed) {
+  if (!out) return SC_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return SC_ERR_NO_DEVICE;
+  if (device < 0 || device >= count) return SC_ERR_ARG;
+  sc_mpnn* m = new sc_mpnn();
+  m->device = device;
+  try {
+    MPCK(hipSetDevice(device));
+    MPCK(hipStreamCreate(&m->stream));
+    const int P = m->lay.total;
+    m->W = m->dalloc<float>(P);
+    m->G = m->dalloc<float>(P);
+    m->Am = m->dalloc<float>(P);
+    m->Av = m->dalloc<float>(P);
+    m->Wn = m->dalloc<bf16>(P);
+    m->Wt = m->dalloc<bf16>(P);
+    init_params(m, seed);
+  } catch (const std::string& s) {
+    sc_mpnn_destroy(m);
+    return SC_ERR_HIP;
+  }
+  *out = m;
+  return SC_OK;
+}
+
+int sc_mpnn_destroy(sc_mpnn* m) {
+  if (!m) return SC_ERR_ARG;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (void* p : m->allocs) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return SC_OK;
+}
+
+const char* sc_mpnn_last_error(const sc_mpnn* m) { return m ? m->err.c_str() : ""; }
+
+int sc_mpnn_num_params(const sc_mpnn* m, int* count) {
+  if (!m || !count) return SC_ERR_ARG;
+  *count = m->lay.total;
+  return SC_OK;
+}
+
+int sc_mpnn_get_params(sc_mpnn* m, float* out) {
+  if (!out) return SC_ERR_ARG;
+  MP_BEGIN
+  MPCK(hipStreamSynchronize(m->stream));
+  MPCK(hipMemcpy(out, m->W, sizeof(float) * m->lay.total, hipMemcpyDeviceToHost));
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_set_params(sc_mpnn* m, const float* in) {
+  if (!in) return SC_ERR_ARG;
+  MP_BEGIN
+  MPCK(hipStreamSynchronize(m->stream));
+  MPCK(hipMemcpy(m->W, in, sizeof(float) * m->lay.total, hipMemcpyHostToDevice));
+  refresh_bf16(m);
+  MPCK(hipStreamSynchronize(m->stream));
+  m->has_forward = false;
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_get_grads(sc_mpnn* m, float* out) {
+  if (!out) return SC_ERR_ARG;
+  MP_BEGIN
+  MPCK(hipStreamSynchronize(m->stream));
+  MPCK(hipMemcpy(out, m->G, sizeof(float) * m->lay.total, hipMemcpyDeviceToHost));
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_set_output_scale(sc_mpnn* m, const float* scale3) {
+  if (!m || !scale3) return SC_ERR_ARG;
+  for (int i = 0; i < 3; ++i) {
+    if (!(scale3[i] > 0) || !std::isfinite(scale3[i])) return SC_ERR_ARG;
+    m->scale[i] = scale3[i];
+  }
+  return SC_OK;
+}
+
+int sc_mpnn_get_output_scale(const sc_mpnn* m, float* scale3) {
+  if (!m || !scale3) return SC_ERR_ARG;
+  for (int i = 0; i < 3; ++i) scale3[i] = m->scale[i];
+  return SC_OK;
+}
+
+int sc_mpnn_load_graph(sc_mpnn* m, int n, const float* node_feat, int e, const int* offsets,
+                       const int* src, const float* edge_feat) {
+  if (!m || n <= 0 || e < 0 || !node_feat || !offsets || (e > 0 && (!src || !edge_feat)))
+    return SC_ERR_ARG;
+  std::vector<int> off(offsets, offsets + n + 1), s(src, src + e), d(e);
+  if (off[0] != 0 || off[n] != e) return fail(m, SC_ERR_ARG, "offsets must run from 0 to e");
+  for (int i = 0; i < n; ++i) {
+    if (off[i + 1] < off[i]) return fail(m, SC_ERR_ARG, "offsets must not decrease");
+    for (int k = off[i]; k < off[i + 1]; ++k) {
+      d[k] = i;
+      if (k > off[i] && s[k] <= s[k - 1])
+        return fail(m, SC_ERR_ARG, "senders must increase within a receiver's edges");
+    }
+  }
+  std::vector<int> rev;
+  if (build_rev(n, off, s, d, rev) != SC_OK)
+    return fail(m, SC_ERR_ARG, "edges must be symmetric (every i<-j has j<-i)");
+  for (size_t i = 0; i < (size_t)n * kNodeFeat; ++i)
+    if (!std::isfinite(node_feat[i])) return fail(m, SC_ERR_ARG, "non-finite node feature");
+  for (size_t i = 0; i < (size_t)e * kEdgeFeat; ++i)
+    if (!std::isfinite(edge_feat[i])) return fail(m, SC_ERR_ARG, "non-finite edge feature");
+  MP_BEGIN
+  ensure_capacity(m, n, e);
+  m->n = n;
+  m->e = e;
+  hipStream_t st = m->stream;
+  MPCK(hipMemcpyAsync(m->off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+  MPCK(hipMemcpyAsync(m->nf, node_feat, sizeof(float) * n * kNodeFeat, hipMemcpyHostToDevice, st));
+  if (e > 0) {
+    MPCK(hipMemcpyAsync(m->src, s.data(), sizeof(int) * e, hipMemcpyHostToDevice, st));
+    MPCK(hipMemcpyAsync(m->dst, d.data(), sizeof(int) * e, hipMemcpyHostToDevice, st));
+    MPCK(hipMemcpyAsync(m->rev, rev.data(), sizeof(int) * e, hipMemcpyHostToDevice, st));
+    MPCK(hipMemcpyAsync(m->ef, edge_feat, sizeof(float) * e * kEdgeFeat, hipMemcpyHostToDevice,
+                        st));
+  }
+  MPCK(hipStreamSynchronize(st));
+  m->has_forward = false;
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_forward(sc_mpnn* m, float* out) {
+  MP_BEGIN
+  if (m->n <= 0) return fail(m, SC_ERR_STATE, "no graph loaded");
+  forward(m);
+  if (out) {
+    std::vector<float> y((size_t)m->n * kNo);
+    MPCK(hipMemcpyAsync(y.data(), m->Y, sizeof(float) * y.size(), hipMemcpyDeviceToHost,
+                        m->stream));
+    MPCK(hipStreamSynchronize(m->stream));
+    for (int i = 0; i < m->n; ++i)
+      for (int c = 0; c < 3; ++c) out[(size_t)i * 3 + c] = y[(size_t)i * kNo + c];
+  } else {
+    MPCK(hipStreamSynchronize(m->stream));
+  }
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_backward(sc_mpnn* m, const float* target, float* loss) {
+  if (!target) return SC_ERR_ARG;
+  MP_BEGIN
+  if (!m->has_forward) return fail(m, SC_ERR_STATE, "backward needs a forward pass first");
+  MPCK(hipMemcpyAsync(m->T, target, sizeof(float) * m->n * 3, hipMemcpyHostToDevice, m->stream));
+  float l = backward(m);
+  MPCK(hipStreamSynchronize(m->stream));
+  if (loss) *loss = l;
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_adam(sc_mpnn* m, float lr, float beta1, float beta2, float eps, int step) {
+  if (!m || !(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || step < 1)
+    return SC_ERR_ARG;
+  MP_BEGIN
+  const int P = m->lay.total;
+  const float c1 = 1.0f - std::pow(beta1, (float)step), c2 = 1.0f - std::pow(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks(P)), dim3(256), 0, m->stream, m->W, m->G, m->Am, m->Av, P,
+                     lr, beta1, beta2, eps, c1, c2);
+  refresh_bf16(m);
+  MPCK(hipStreamSynchronize(m->stream));
+  m->has_forward = false;
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_graph_from_system(sc_mpnn* m, sc_system* s, int* n_out, int* e_out) {
+  if (!s) return SC_ERR_ARG;
+  MP_BEGIN
+  SystemView v;
+  if (system_view(s, &v) != SC_OK) return fail(m, SC_ERR_ARG, "no system");
+  if (v.device != m->device) return fail(m, SC_ERR_ARG, "system and network on different devices");
+  MPCK(hipStreamSynchronize(v.stream));
+  const int n = v.ntot;
+  ensure_capacity(m, n, m->ecap);
+  hipLaunchKernelGGL(k_graph_count, dim3(blocks(n)), dim3(256), 0, m->stream, v, m->d_deg);
+  std::vector<int> deg(n), off(n + 1, 0);
+  MPCK(hipMemcpyAsync(deg.data(), m->d_deg, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
+  MPCK(hipStreamSynchronize(m->stream));
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + deg[i];
+  const int e = off[n];
+  ensure_capacity(m, n, e);
+  MPCK(hipMemcpyAsync(m->off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+                      m->stream));
+  hipLaunchKernelGGL(k_graph_fill, dim3(blocks(n)), dim3(256), 0, m->stream, v, m->off, m->src,
+                     m->dst, m->nf, m->ef);
+  // reverse-edge index (host): senders within a group are ordered by caller id,
+  // so look them up through the storage -> caller id map
+  std::vector<int> src(e), dst(e), id(n);
+  MPCK(hipMemcpyAsync(src.data(), m->src, sizeof(int) * e, hipMemcpyDeviceToHost, m->stream));
+  MPCK(hipMemcpyAsync(id.data(), v.b.id, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
+  MPCK(hipStreamSynchronize(m->stream));
+  std::vector<int> rev(e, -1);
+  for (int i = 0; i < n; ++i)
+    for (int k = off[i]; k < off[i + 1]; ++k) {
+      const int j = src[k];
+      for (int r = off[j]; r < off[j + 1]; ++r)
+        if (src[r] == i) {
+          rev[k] = r;
+          break;
+        }
+      if (rev[k] < 0) return fail(m, SC_ERR_STATE, "contact graph is not symmetric");
+    }
+  MPCK(hipMemcpyAsync(m->rev, rev.data(), sizeof(int) * e, hipMemcpyHostToDevice, m->stream));
+  MPCK(hipStreamSynchronize(m->stream));
+  m->n = n;
+  m->e = e;
+  m->has_forward = false;
+  if (n_out) *n_out = n;
+  if (e_out) *e_out = e;
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_export_graph(sc_mpnn* m, sc_system* s, float* node_feat, int* offsets, int* src,
+                         float* edge_feat) {
+  if (!s || !node_feat || !offsets) return SC_ERR_ARG;
+  MP_BEGIN
+  SystemView v;
+  if (system_view(s, &v) != SC_OK || v.ntot != m->n)
+    return fail(m, SC_ERR_STATE, "graph was not built from this system");
+  const int n = m->n, e = m->e;
+  std::vector<int> off(n + 1), sr(e), id(n);
+  std::vector<float> nf((size_t)n * kNodeFeat), ef((size_t)e * kEdgeFeat);
+  MPCK(hipMemcpy(off.data(), m->off, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(sr.data(), m->src, sizeof(int) * e, hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(id.data(), v.b.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(nf.data(), m->nf, sizeof(float) * nf.size(), hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(ef.data(), m->ef, sizeof(float) * ef.size(), hipMemcpyDeviceToHost));
+  // caller order: node id[g] <- storage g; a receiver's edges keep their order
+  // (senders ascending by caller id)
+  std::vector<int> deg(n);
+  for (int g = 0; g < n; ++g) deg[id[g]] = off[g + 1] - off[g];
+  offsets[0] = 0;
+  for (int i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + deg[i];
+  for (int g = 0; g < n; ++g) {
+    const int i = id[g];
+    for (int c = 0; c < kNodeFeat; ++c) node_feat[(size_t)i * kNodeFeat + c] = nf[(size_t)g * kNodeFeat + c];
+    for (int k = 0; k < deg[i]; ++k) {
+      const int eo = offsets[i] + k, es = off[g] + k;
+      if (src) src[eo] = id[sr[es]];
+      if (edge_feat)
+        for (int c = 0; c < kEdgeFeat; ++c)
+          edge_feat[(size_t)eo * kEdgeFeat + c] = ef[(size_t)es * kEdgeFeat + c];
+    }
+  }
+  return SC_OK;
+  MP_END
+}
+
+int sc_mpnn_warm_start(sc_mpnn* m, sc_system* s) {
+  if (!s) return SC_ERR_ARG;
+  int st = sc_mpnn_graph_from_system(m, s, nullptr, nullptr);
+  if (st != SC_OK) return st;
+  MP_BEGIN
+  forward(m);
+  SystemView v;
+  system_view(s, &v);
+  hipLaunchKernelGGL(k_apply, dim3(blocks(v.ntot)), dim3(256), 0, m->stream, v.b, m->Y,
+                     m->scale[0], m->scale[1], m->scale[2], v.ntot);
+  MPCK(hipStreamSynchronize(m->stream));
+  return system_positions_changed(s);
+  MP_END
+}
+
+}  // extern "C"

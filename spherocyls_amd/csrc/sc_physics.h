@@ -30,7 +30,8 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   double d = nxA * ddx + nyA * ddy;
   double e = -nxB * ddx - nyB * ddy;
   double det = a * c - b * b;
-  double t = fmin(fmax((b * d - a * e) / det, -1.0), 1.0);
+  double tu = (b * d - a * e) / det;
+  double t = fmin(fmax(tu, -1.0), 1.0);
   double s = -(b * t + d) / a;
   double sabs = fabs(s);
   s = fmin(fmax(s, -1.0), 1.0);
@@ -39,6 +40,21 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   double Dy = ddy + s * nyA - t * nyB;
   double d2 = Dx * Dx + Dy * Dy;
   if (!(d2 < sig * sig)) return false;
+  o.s = s;
+  o.t = t;
+  o.dx = Dx;
+  o.dy = Dy;
+  if ((det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0) || d2 == 0.0) {
+    // crossed segments: the lines meet inside both segments, so d = 0 and
+    // stays 0 under any small displacement -- a plateau of the energy with
+    // zero gradient.  (D is rounding noise here; normalising it would give a
+    // full-size force in a random direction, or 0/0.)  An exact D = 0 in any
+    // other geometry (touching ends, collinear overlap) is treated alike.
+    o.fx = o.fy = o.tp = o.tq = 0.0;
+    o.dist = 0.0;
+    o.dV = 1.0 / sig;
+    return true;
+  }
   double dist = sqrt(d2);
   double dV = (1.0 - dist / sig) / sig;
   double fx = Dx * dV / dist;
@@ -47,10 +63,6 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   o.fy = fy;
   o.tp = s * nxA * fy - s * nyA * fx;
   o.tq = t * nyB * fx - t * nxB * fy;
-  o.s = s;
-  o.t = t;
-  o.dx = Dx;
-  o.dy = Dy;
   o.dist = dist;
   o.dV = dV;
   return true;
@@ -67,7 +79,8 @@ __device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA
   float d = nxA * ddx + nyA * ddy;
   float e = -nxB * ddx - nyB * ddy;
   float det = a * c - b * b;
-  float t = fminf(fmaxf((b * d - a * e) / det, -1.0f), 1.0f);
+  float tu = (b * d - a * e) / det;
+  float t = fminf(fmaxf(tu, -1.0f), 1.0f);
   float s = -(b * t + d) / a;
   float sabs = fabsf(s);
   s = fminf(fmaxf(s, -1.0f), 1.0f);
@@ -76,6 +89,16 @@ __device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA
   float Dy = ddy + s * nyA - t * nyB;
   float d2 = Dx * Dx + Dy * Dy;
   if (!(d2 < sig * sig)) return false;
+  o.s = s;
+  o.t = t;
+  o.dx = Dx;
+  o.dy = Dy;
+  if ((det > 0.0f && fabsf(tu) < 1.0f && sabs < 1.0f) || d2 == 0.0f) {  // crossed: plateau
+    o.fx = o.fy = o.tp = o.tq = 0.0f;
+    o.dist = 0.0f;
+    o.dV = 1.0f / sig;
+    return true;
+  }
   float dist = sqrtf(d2);
   float dV = (1.0f - dist / sig) / sig;
   float fx = Dx * dV / dist;
@@ -84,13 +107,198 @@ __device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA
   o.fy = fy;
   o.tp = s * nxA * fy - s * nyA * fx;
   o.tq = t * nyB * fx - t * nxB * fy;
+  o.dist = dist;
+  o.dV = dV;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// two-point contact model
+// ---------------------------------------------------------------------------
+// Force fields of one contact point from its D vector (harmonic, as above).
+template <typename T>
+__device__ __forceinline__ bool contact_point(T Dx, T Dy, T s, T t, T nxA, T nyA, T nxB, T nyB,
+                                              T sig, bool crossed, PairOut<T>& o) {
+#pragma clang fp contract(off)
+  T d2 = Dx * Dx + Dy * Dy;
+  if (!(d2 < sig * sig)) return false;
   o.s = s;
   o.t = t;
   o.dx = Dx;
   o.dy = Dy;
+  if (crossed || d2 == T(0)) {  // energy plateau, as in pair_contact_f64
+    o.fx = o.fy = o.tp = o.tq = T(0);
+    o.dist = T(0);
+    o.dV = T(1) / sig;
+    return true;
+  }
+  T dist = sqrt(d2);
+  T dV = (T(1) - dist / sig) / sig;
+  T fx = Dx * dV / dist;
+  T fy = Dy * dV / dist;
+  o.fx = fx;
+  o.fy = fy;
+  o.tp = s * nxA * fy - s * nyA * fx;
+  o.tq = t * nyB * fx - t * nxB * fy;
   o.dist = dist;
   o.dV = dV;
   return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T clamp1(T v) {
+  return fmin(fmax(v, T(-1)), T(1));
+}
+
+// Two-point contact model (src/cuda/2point_dynamics.cu:391-546).  Besides the
+// closest pair of points (s1, t1), a second pair (s2, t2) -- for nearly parallel
+// rods the other end of their overlap -- carries its own harmonic contact when
+// it overlaps as well, so two overlapping, nearly aligned rods have no energy
+// kink at alignment.  The second pair starts at the opposite end of whichever
+// segment fixed the first (s2 = -s1 or t2 = -t1) projected onto the other
+// segment; a third candidate from the other segment's matching end replaces it
+// when it is closer.  o0 / h0: the closest-point contact; o1 / h1: the second.
+template <typename T>
+__device__ __forceinline__ void pair_contacts_2p(T ddx, T ddy, T nxA, T nyA, T nxB, T nyB, T Ap,
+                                                 T Aq, T sig, PairOut<T>& o0, bool& h0,
+                                                 PairOut<T>& o1, bool& h1) {
+#pragma clang fp contract(off)
+  const T a = Ap * Ap;
+  const T b = -(nxA * nxB + nyA * nyB);
+  const T c = Aq * Aq;
+  const T d = nxA * ddx + nyA * ddy;
+  const T e = -nxB * ddx - nyB * ddy;
+  const T det = a * c - b * b;
+  T s1, t1, s2, t2, s3, t3;
+  bool crossed = false;
+  if (det <= T(0)) {  // parallel: the two ends of the overlap
+    s1 = clamp1(-(b + d) / a);
+    s2 = clamp1(-(d - b) / a);
+    if (fabs(s1) == T(1)) t1 = clamp1(-(b * s1 + e) / c);
+    else t1 = b < T(0) ? clamp1(-(b + e) / c) : clamp1(-(e - b) / c);
+    if (fabs(s2) == T(1)) t2 = clamp1(-(b * s2 + e) / c);
+    else t2 = b < T(0) ? clamp1(-(e - b) / c) : clamp1(-(b + e) / c);
+    s3 = s2;
+    t3 = t2;
+  } else {
+    const T tu = (b * d - a * e) / det;
+    t1 = clamp1(tu);
+    s1 = -(b * t1 + d) / a;
+    T sarg = fabs(s1);
+    s1 = clamp1(s1);
+    crossed = fabs(tu) < T(1) && sarg < T(1);
+    if (sarg >= T(1)) {
+      t1 = clamp1(-(b * s1 + e) / c);
+      s2 = -s1;
+      t2 = -(b * s2 + e) / c;
+      T targ = fabs(t2);
+      t2 = clamp1(t2);
+      if (targ > T(1)) {
+        s2 = clamp1(-(b * t2 + d) / a);
+        s3 = s2;
+        t3 = t2;
+      } else {
+        t3 = b < T(0) ? -s1 : s1;
+        s3 = -(b * t3 + d) / a;
+        sarg = fabs(s3);
+        s3 = clamp1(s3);
+        if (sarg >= T(1)) t3 = clamp1(-(b * s3 + e) / c);
+      }
+    } else {
+      t2 = -t1;
+      s2 = -(b * t2 + d) / a;
+      sarg = fabs(s2);
+      s2 = clamp1(s2);
+      if (sarg > T(1)) {
+        t2 = clamp1(-(b * s2 + e) / c);
+        s3 = s2;
+        t3 = t2;
+      } else {
+        s3 = b < T(0) ? -t1 : t1;
+        t3 = -(b * s3 + e) / c;
+        T targ = fabs(t3);
+        t3 = clamp1(t3);
+        if (targ >= T(1)) s3 = clamp1(-(b * t3 + d) / a);
+      }
+    }
+  }
+  h0 = contact_point(ddx + s1 * nxA - t1 * nxB, ddy + s1 * nyA - t1 * nyB, s1, t1, nxA, nyA, nxB,
+                     nyB, sig, crossed, o0);
+  h1 = false;
+  if (s1 != s2 && t1 != t2) {
+    T Dx2 = ddx + s2 * nxA - t2 * nxB;
+    T Dy2 = ddy + s2 * nyA - t2 * nyB;
+    if (s3 != s1 && s3 != s2) {
+      T Dx3 = ddx + s3 * nxA - t3 * nxB;
+      T Dy3 = ddy + s3 * nyA - t3 * nyB;
+      if (Dx3 * Dx3 + Dy3 * Dy3 < Dx2 * Dx2 + Dy2 * Dy2) {
+        s2 = s3;
+        t2 = t3;
+        Dx2 = Dx3;
+        Dy2 = Dy3;
+      }
+    }
+    h1 = contact_point(Dx2, Dy2, s2, t2, nxA, nyA, nxB, nyB, sig, false, o1);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void widen(const PairOut<T>& a, PairOut<double>& o) {
+  o.fx = a.fx; o.fy = a.fy; o.tp = a.tp; o.tq = a.tq; o.s = a.s; o.t = a.t;
+  o.dx = a.dx; o.dy = a.dy; o.dist = a.dist; o.dV = a.dV;
+}
+
+// Contacts of one pair in the orientation given (A = lower id): the
+// closest-point model of the CPU reference, or (TWO_POINT) the two-point model;
+// FP64 or FP32 pair math, FP64 results.  o0 / h0: closest-point contact;
+// o1 / h1: the second contact of the two-point model.
+template <bool TWO_POINT>
+__device__ __forceinline__ void pair_contacts(double ddx, double ddy, double nxA, double nyA,
+                                              double nxB, double nyB, double Ap, double Aq,
+                                              double sig, bool fp64, PairOut<double>& o0,
+                                              bool& h0, PairOut<double>& o1, bool& h1) {
+  if (!TWO_POINT) {
+    h1 = false;
+    if (fp64) {
+      h0 = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, o0);
+    } else {
+      PairOut<float> f0;
+      h0 = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
+                            (float)nyB, (float)Ap, (float)Aq, (float)sig, f0);
+      if (h0) widen(f0, o0);
+    }
+    return;
+  }
+  if (fp64) {
+    pair_contacts_2p<double>(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, o0, h0, o1, h1);
+  } else {
+    PairOut<float> f0, f1;
+    pair_contacts_2p<float>((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
+                            (float)nyB, (float)Ap, (float)Aq, (float)sig, f0, h0, f1, h1);
+    if (h0) widen(f0, o0);
+    if (h1) widen(f1, o1);
+  }
+}
+
+// Stress / energy sums of one contact point of pair (A, B) (A = lower id):
+// the reference's formulas with their two slips (src/cpp/spherocyl_box.cpp:
+// 483-495) and the corrected virial, unnormalised (k_reduce divides).
+__device__ __forceinline__ void add_contact_sums(const PairOut<double>& r, double nxA, double nyA,
+                                                 double nxB, double nyB, double sig, double* acc) {
+#pragma clang fp contract(off)
+  double cxa = r.s * nxA - 0.5 * r.dx;
+  double cya = r.s * nyA - 0.5 * r.dy;
+  double cxb = r.t * nxB + 0.5 * r.dx;
+  double cyb_ref = r.t * nxB + 0.5 * r.dy;
+  double cyb = r.t * nyB + 0.5 * r.dy;
+  acc[kSeE] += r.dV * sig * (1.0 - r.dist / sig);
+  acc[kSePxx] += r.fx * cxa - r.fx * cxb - r.fy * cxb;
+  acc[kSePyy] += r.fy * cya - r.fy * cyb_ref;
+  acc[kSePxy] += r.fy * cxa;
+  acc[kSeCPxx] += r.fx * (cxa - cxb);
+  acc[kSeCPyy] += r.fy * (cya - cyb);
+  acc[kSeCPxy] += r.fy * (cxa - cxb);
+  acc[kSeCPyx] += r.fx * (cya - cyb);
 }
 
 // Minimum image in sheared coordinates (src/cpp/strain_dynamics.cpp:28-29).

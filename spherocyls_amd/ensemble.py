@@ -1,0 +1,101 @@
+"""Minimisation workloads: ensembles of ragged packings brought to their
+pre-minimisation state on the GPU, then minimised to F_rms < 1e-10.
+
+A workload is a list of packings plus the Lees-Edwards strain of each; the
+pre-minimisation state of a packing is its seeded random configuration after a
+number of sheared Heun steps (all packings advanced together in one batched
+system).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from . import configs, native
+
+
+@dataclasses.dataclass
+class Workload:
+    packings: list
+    gammas: list
+
+    @property
+    def sizes(self):
+        return [p.n for p in self.packings]
+
+    @property
+    def particles(self):
+        return sum(self.sizes)
+
+
+def prepare(packings, heun_steps=2000, strain_rate=1e-4, dt=0.01, contact_model="two_point",
+            pad=0.1) -> Workload:
+    """Advance every packing by heun_steps sheared Heun steps (one batched
+    system) and return the resulting states."""
+    s = native.System(packings, pads=pad)
+    try:
+        s.set_contact_model(contact_model)
+        s.set_dynamics(strain_rate=strain_rate, step=dt)
+        if heun_steps:
+            s.strain_steps(heun_steps)
+        x, y, phi = s.positions()
+        gammas = [s.box(k)["gamma"] for k in range(s.npack)]
+    finally:
+        s.close()
+    out, o = [], 0
+    for p in packings:
+        sl = slice(o, o + p.n)
+        out.append(configs.Packing(p.L, x[sl].copy(), y[sl].copy(), phi[sl].copy(), p.A, p.R,
+                                   p.packing_fraction, p.name))
+        o += p.n
+    return Workload(out, gammas)
+
+
+def system_for(work: Workload, indices=None, precision="fp64", contact_model="two_point",
+               pad=0.1, autotune=True):
+    idx = list(range(len(work.packings))) if indices is None else list(indices)
+    s = native.System([work.packings[i] for i in idx], pads=pad)
+    s.set_contact_model(contact_model)
+    for k, i in enumerate(idx):
+        s.set_gamma(work.gammas[i], 0.0, k=k)
+    s.set_precision(precision)
+    if autotune:
+        s.autotune(reps=5)
+    return s
+
+
+def minimize_batched(work: Workload, params, precision="mixed", switch_frms=1e-6):
+    """All packings in one system; returns (seconds, iterations[], frms[], status)."""
+    s = system_for(work, precision=precision)
+    try:
+        s.synchronize()
+        t0 = time.perf_counter()
+        it, fr, st = s.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
+                            check=False)
+        t1 = time.perf_counter()
+    finally:
+        s.close()
+    return t1 - t0, it, fr, st
+
+
+def minimize_one_by_one(work: Workload, params, precision="fp64", limit=None):
+    """One system per packing, minimised in turn (systems are built and tuned
+    before the clock starts); returns (seconds, iterations[], frms[])."""
+    n = len(work.packings) if limit is None else min(limit, len(work.packings))
+    systems = [system_for(work, [i], precision=precision) for i in range(n)]
+    its, frs = [], []
+    try:
+        for s in systems:
+            s.synchronize()
+        t0 = time.perf_counter()
+        for s in systems:
+            it, fr, _ = s.fire(params, check=False)
+            its.append(int(it[0]))
+            frs.append(float(fr[0]))
+        t1 = time.perf_counter()
+    finally:
+        for s in systems:
+            s.close()
+    return t1 - t0, np.array(its), np.array(frs)

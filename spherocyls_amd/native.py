@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG, "libspherocyls.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "spherocyls.h")
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 _lib = None
 
@@ -60,6 +61,8 @@ def load():
         "sc_set_dynamics": ([sysp, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
         "sc_set_gamma": ([sysp, C.c_int, C.c_double, C.c_double], C.c_int),
         "sc_set_precision": ([sysp, C.c_int], C.c_int),
+        "sc_set_contact_model": ([sysp, C.c_int], C.c_int),
+        "sc_get_contact_model": ([sysp, _ip], C.c_int),
         "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_set_path": ([sysp, C.c_int], C.c_int),
         "sc_get_path": ([sysp, _ip], C.c_int),
@@ -92,6 +95,22 @@ def load():
                                C.c_int),
         "sc_read_positions": ([C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "sc_format_se_line": ([C.c_char_p, C.c_int, C.c_ulong, C.c_int, _dp, C.c_int], C.c_int),
+        "sc_mpnn_create": ([C.POINTER(sysp), C.c_int, C.c_ulonglong], C.c_int),
+        "sc_mpnn_destroy": ([sysp], C.c_int),
+        "sc_mpnn_last_error": ([sysp], C.c_char_p),
+        "sc_mpnn_num_params": ([sysp, _ip], C.c_int),
+        "sc_mpnn_get_params": ([sysp, _fp], C.c_int),
+        "sc_mpnn_set_params": ([sysp, _fp], C.c_int),
+        "sc_mpnn_get_grads": ([sysp, _fp], C.c_int),
+        "sc_mpnn_set_output_scale": ([sysp, _fp], C.c_int),
+        "sc_mpnn_get_output_scale": ([sysp, _fp], C.c_int),
+        "sc_mpnn_load_graph": ([sysp, C.c_int, _fp, C.c_int, _ip, _ip, _fp], C.c_int),
+        "sc_mpnn_forward": ([sysp, _fp], C.c_int),
+        "sc_mpnn_backward": ([sysp, _fp, _fp], C.c_int),
+        "sc_mpnn_adam": ([sysp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int], C.c_int),
+        "sc_mpnn_graph_from_system": ([sysp, sysp, _ip, _ip], C.c_int),
+        "sc_mpnn_export_graph": ([sysp, sysp, _fp, _ip, _ip, _fp], C.c_int),
+        "sc_mpnn_warm_start": ([sysp, sysp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -239,6 +258,16 @@ class System:
     def set_precision(self, mode):
         self._ck(self.lib.sc_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)))
 
+    def set_contact_model(self, model):
+        """"closest" (the CPU reference) or "two_point"."""
+        m = {"closest": SC_CONTACT_CLOSEST, "two_point": SC_CONTACT_TWO_POINT}.get(model, model)
+        self._ck(self.lib.sc_set_contact_model(self.h, m))
+
+    def contact_model(self):
+        m = C.c_int()
+        self._ck(self.lib.sc_get_contact_model(self.h, C.byref(m)))
+        return "two_point" if m.value == SC_CONTACT_TWO_POINT else "closest"
+
     def set_tiling(self, tile_cols, block):
         self._ck(self.lib.sc_set_tiling(self.h, tile_cols, block))
 
@@ -349,3 +378,132 @@ class System:
 
     def synchronize(self):
         self._ck(self.lib.sc_synchronize(self.h))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Graph:
+    """Contact graph in CSR over receivers (host arrays)."""
+
+    def __init__(self, node_feat, offsets, src, edge_feat):
+        self.node_feat = _f32(node_feat).reshape(-1, 4)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.src = np.ascontiguousarray(src, dtype=np.int32)
+        self.edge_feat = _f32(edge_feat).reshape(-1, 5)
+
+    @property
+    def n(self):
+        return self.node_feat.shape[0]
+
+    @property
+    def e(self):
+        return self.src.shape[0]
+
+    @staticmethod
+    def concat(graphs):
+        nf = np.concatenate([g.node_feat for g in graphs])
+        ef = np.concatenate([g.edge_feat for g in graphs])
+        src, offs, base, ebase = [], [np.zeros(1, dtype=np.int32)], 0, 0
+        for g in graphs:
+            src.append(g.src + base)
+            offs.append(g.offsets[1:] + ebase)
+            base += g.n
+            ebase += g.e
+        return Graph(nf, np.concatenate(offs), np.concatenate(src), ef)
+
+
+class Mpnn:
+    """The message-passing warm-start network (sc_mpnn)."""
+
+    def __init__(self, device=0, seed=1):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.sc_mpnn_create(C.byref(h), device, C.c_ulonglong(seed))
+        if st != 0:
+            raise SpherocylsError(st, "sc_mpnn_create")
+        self.h = h
+        n = C.c_int()
+        self._ck(self.lib.sc_mpnn_num_params(self.h, C.byref(n)))
+        self.num_params = n.value
+
+    def _ck(self, st):
+        if st != 0:
+            msg = self.lib.sc_mpnn_last_error(self.h) if getattr(self, "h", None) else b""
+            raise SpherocylsError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sc_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._ck(self.lib.sc_mpnn_get_params(self.h, _pf(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.shape == (self.num_params,)
+        self._ck(self.lib.sc_mpnn_set_params(self.h, _pf(p)))
+
+    def grads(self):
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._ck(self.lib.sc_mpnn_get_grads(self.h, _pf(out)))
+        return out
+
+    def set_output_scale(self, scale):
+        s = _f32(scale)
+        self._ck(self.lib.sc_mpnn_set_output_scale(self.h, _pf(s)))
+
+    def output_scale(self):
+        s = np.zeros(3, dtype=np.float32)
+        self._ck(self.lib.sc_mpnn_get_output_scale(self.h, _pf(s)))
+        return s
+
+    def load_graph(self, g):
+        self._ck(self.lib.sc_mpnn_load_graph(self.h, g.n, _pf(g.node_feat), g.e, _pi(g.offsets),
+                                             _pi(g.src), _pf(g.edge_feat)))
+        self._n = g.n
+
+    def forward(self):
+        out = np.empty((self._n, 3), dtype=np.float32)
+        self._ck(self.lib.sc_mpnn_forward(self.h, _pf(out)))
+        return out
+
+    def backward(self, target):
+        t = _f32(target).reshape(self._n, 3)
+        loss = C.c_float()
+        self._ck(self.lib.sc_mpnn_backward(self.h, _pf(t), C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._ck(self.lib.sc_mpnn_adam(self.h, lr, beta1, beta2, eps, step))
+
+    def graph_of(self, system):
+        """Contact graph of a System's current state, in caller order."""
+        n, e = C.c_int(), C.c_int()
+        self._ck(self.lib.sc_mpnn_graph_from_system(self.h, system.h, C.byref(n), C.byref(e)))
+        nf = np.empty((n.value, 4), dtype=np.float32)
+        off = np.empty(n.value + 1, dtype=np.int32)
+        src = np.empty(max(e.value, 1), dtype=np.int32)
+        ef = np.empty((max(e.value, 1), 5), dtype=np.float32)
+        self._ck(self.lib.sc_mpnn_export_graph(self.h, system.h, _pf(nf), _pi(off), _pi(src),
+                                               _pf(ef)))
+        self._n = n.value
+        return Graph(nf, off, src[:e.value], ef[:e.value])
+
+    def warm_start(self, system):
+        self._ck(self.lib.sc_mpnn_warm_start(self.h, system.h))

@@ -1,0 +1,207 @@
+"""Learned warm starts for the minimiser.
+
+A small message-passing network over the contact graph (native.Mpnn, kernels
+in csrc/sc_mpnn.hip) predicts each particle's relaxation displacement
+(dx, dy, dphi) from a pre-minimisation state.  The minimiser then starts from
+the displaced state; the plain minimiser is the fallback and the judge: a
+warm-started packing is accepted only when FIRE takes it to the same
+F_rms < ftol, otherwise that packing is minimised again from its original
+state.
+
+Training data are (pre-minimisation, minimised) pairs the simulator makes
+itself: ragged packings get sheared Heun steps (the pre state), then FIRE
+(the target).  Training runs on the GPU through the network's own forward,
+backward and Adam kernels.
+
+    python -m spherocyls_amd.warmstart train --out spherocyls_amd/data/mpnn_warmstart.npz
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import math
+import os
+import time
+
+import numpy as np
+
+from . import configs, ensemble, native
+
+REF_PI = 3.14159265358979
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA, "mpnn_warmstart.npz")
+
+
+def workload(count, n, seed, phi=(0.80, 0.86), aspect=2.0):
+    """Bidisperse (R 0.5 / 0.7) packings of n particles, phi uniform in the
+    range, from seeds seed*100003 + k."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        R, A = configs.bidisperse(n, aspect=aspect)
+        f = float(rng.uniform(*phi))
+        out.append(configs.make_packing(R, A, f, seed * 100003 + k, f"warm[{seed}:{k}] phi={f:.4f}"))
+    return out
+
+
+def split_graph(g, sizes):
+    """Split a concatenated graph (caller order) into per-packing graphs."""
+    out, o = [], 0
+    for n in sizes:
+        e0, e1 = int(g.offsets[o]), int(g.offsets[o + n])
+        out.append(native.Graph(g.node_feat[o:o + n], g.offsets[o:o + n + 1] - e0,
+                                g.src[e0:e1] - o, g.edge_feat[e0:e1]))
+        o += n
+    return out
+
+
+def displacement(pre, post_xyz, L):
+    x, y, phi = post_xyz
+    dx = x - pre.x
+    dy = y - pre.y
+    dx -= L * np.round(dx / L)
+    dy -= L * np.round(dy / L)
+    dp = phi - pre.phi
+    dp -= 2 * REF_PI * np.round(dp / (2 * REF_PI))
+    return np.column_stack([dx, dy, dp])
+
+
+@dataclasses.dataclass
+class Pairs:
+    graphs: list      # native.Graph per packing (pre state)
+    targets: list     # [n, 3] displacement to the minimised state
+    iterations: np.ndarray
+
+
+def make_pairs(packings, params, heun_steps=2000, mpnn=None):
+    """Pre states (sheared Heun steps), their contact graphs, and the FIRE
+    displacement of every particle (batched, FP64, two-point contacts)."""
+    work = ensemble.prepare(packings, heun_steps=heun_steps)
+    s = ensemble.system_for(work, precision="fp64", autotune=False)
+    own = mpnn is None
+    m = native.Mpnn() if own else mpnn
+    try:
+        g = m.graph_of(s)
+        it, fr, st = s.fire(params, check=False)
+        x, y, phi = s.positions()
+    finally:
+        s.close()
+        if own:
+            m.close()
+    graphs = split_graph(g, work.sizes)
+    targets, o = [], 0
+    for pk in work.packings:
+        sl = slice(o, o + pk.n)
+        targets.append(displacement(pk, (x[sl], y[sl], phi[sl]), pk.L).astype(np.float32))
+        o += pk.n
+    ok = fr < params[8]
+    keep = [i for i in range(len(graphs)) if ok[i]]
+    return Pairs([graphs[i] for i in keep], [targets[i] for i in keep], it[ok]), work
+
+
+def train(pairs, steps=2000, batch=8, lr=1e-3, seed=1, log_every=200, mpnn=None):
+    """Adam on the mean squared error of the scaled displacement; returns
+    (network, output scale, loss history)."""
+    tall = np.concatenate(pairs.targets)
+    scale = tall.std(axis=0).astype(np.float32) + 1e-12
+    m = native.Mpnn(seed=seed) if mpnn is None else mpnn
+    m.set_output_scale(scale)
+    rng = np.random.default_rng(seed)
+    hist = []
+    t0 = time.perf_counter()
+    for step in range(1, steps + 1):
+        idx = rng.choice(len(pairs.graphs), size=min(batch, len(pairs.graphs)), replace=False)
+        g = native.Graph.concat([pairs.graphs[i] for i in idx])
+        t = np.concatenate([pairs.targets[i] for i in idx]) / scale
+        m.load_graph(g)
+        m.forward()
+        loss = m.backward(t)
+        # cosine decay of the learning rate
+        m.adam(lr * 0.5 * (1 + math.cos(math.pi * (step - 1) / steps)), step)
+        hist.append(loss)
+        if log_every and step % log_every == 0:
+            print(f"step {step} loss {np.mean(hist[-log_every:]):.4f} "
+                  f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    return m, scale, np.array(hist)
+
+
+def save(m, path):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=m.params(), scale=m.output_scale())
+
+
+def load(path=DEFAULT_WEIGHTS, device=0):
+    with np.load(path, allow_pickle=False) as f:
+        m = native.Mpnn(device=device)
+        m.set_params(f["params"])
+        m.set_output_scale(f["scale"])
+    return m
+
+
+def minimize_with_warm_start(work, params, mpnn, precision="mixed", switch_frms=1e-6):
+    """Batched minimisation from warm starts with per-packing fallback.
+
+    Returns (seconds, iterations[], frms[], fallbacks): iterations count the
+    warm-started run plus, for a packing that did not reach ftol, its rerun
+    from the original state."""
+    s = ensemble.system_for(work, precision=precision)
+    fallbacks = []
+    try:
+        s.synchronize()
+        t0 = time.perf_counter()
+        mpnn.warm_start(s)
+        it, fr, st = s.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
+                            check=False)
+        bad = [k for k in range(len(work.packings)) if not fr[k] < params[8]]
+        if bad:
+            sub = ensemble.Workload([work.packings[k] for k in bad], [work.gammas[k] for k in bad])
+            s2 = ensemble.system_for(sub, precision=precision, autotune=False)
+            try:
+                it2, fr2, _ = s2.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
+                                      check=False)
+            finally:
+                s2.close()
+            for j, k in enumerate(bad):
+                it[k] += it2[j]
+                fr[k] = fr2[j]
+            fallbacks = bad
+        t1 = time.perf_counter()
+    finally:
+        s.close()
+    return t1 - t0, it, fr, fallbacks
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    t.add_argument("--packings", type=int, default=512)
+    t.add_argument("--n", type=int, default=4096)
+    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    params = native.fire_params(max_iter=100000)
+    t0 = time.perf_counter()
+    pairs = None
+    chunk = 64
+    for c0 in range(0, args.packings, chunk):
+        pk = workload(min(chunk, args.packings - c0), args.n, seed=1000 + args.seed + c0)
+        p, _ = make_pairs(pk, params)
+        if pairs is None:
+            pairs = p
+        else:
+            pairs.graphs += p.graphs
+            pairs.targets += p.targets
+            pairs.iterations = np.concatenate([pairs.iterations, p.iterations])
+        print(f"pairs {len(pairs.graphs)} ({time.perf_counter() - t0:.1f} s), "
+              f"FIRE iterations mean {pairs.iterations.mean():.0f}", flush=True)
+    m, scale, hist = train(pairs, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed)
+    save(m, args.out)
+    print(f"saved {args.out}: scale {scale}, final loss {hist[-100:].mean():.4f}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,135 @@
+"""Two-point contact model on the GPU against the oracle's restatement
+(tests/test_two_point_cpu.py pins the restatement itself)."""
+import numpy as np
+import pytest
+
+import oracle
+from spherocyls_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+FIRE = dict(dt0=0.05, dt_max=0.5, n_min=5, f_inc=1.1, f_dec=0.5, alpha0=0.1, f_alpha=0.99,
+            max_move=0.02, ftol=1e-10)
+
+
+def _packing(kind):
+    if kind == "aligned":
+        R, A = configs.bidisperse(400, aspect=3.0)
+        L = configs.box_side(R, A, 0.55)
+        x, y, phi = native.random_configuration(L, A, R, seed=9, config=native.SC_CONFIG_RANDOM_ALIGNED)
+        return configs.Packing(L, x, y, phi, A, R, 0.0, "aligned")
+    R, A = configs.bidisperse(600, aspect=4.0)
+    return configs.make_packing(R, A, 0.83, 61, "aspect 4", gamma=0.21)
+
+
+def _oracle(pk, gamma):
+    b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 0.1)
+    b.set_contact_model(1)
+    b.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    b.find_neighbors()
+    b.calc_se()
+    return b
+
+
+@pytest.mark.parametrize("path", ["list", "tile"])
+@pytest.mark.parametrize("kind", ["aligned", "aspect4"])
+def test_two_point_forces_stress(kind, path):
+    pk = _packing(kind)
+    gamma = 0.0 if kind == "aligned" else 0.21
+    print("two-point forces", kind, path, pk.n, flush=True)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_path(path)
+        s.set_contact_model("two_point")
+        s.set_gamma(gamma, 0.0)
+        s.calc_se()
+        got = s.forces()
+        bx = s.box()
+    finally:
+        s.close()
+    ob = _oracle(pk, gamma)
+    p = ob.particles()
+    sc = ob.force_scale()
+    for name, a, b, c in zip(("fx", "fy", "tau"), got, (p["fx"], p["fy"], p["tau"]), sc):
+        # a pair contributes up to two contact forces: bound by twice the scale
+        assert np.all(np.abs(a - b) <= TOL * 2 * c + 1e-15), name
+    o = ob.scalars()
+    assert o["E"] > 0
+    for k in ("E", "Pxx", "Pyy", "Pxy", "cPxx", "cPyy", "cPxy", "cPyx"):
+        assert bx[k] == pytest.approx(o[k], rel=TOL, abs=TOL * abs(o["E"])), k
+
+
+def test_two_point_heun_trajectory():
+    pk = _packing("aspect4")
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_contact_model("two_point")
+        s.set_gamma(0.21, 0.0)
+        s.set_dynamics(strain_rate=1e-3, step=0.01)
+        s.strain_steps(30)
+        x, y, phi = s.positions()
+    finally:
+        s.close()
+    b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 1.0)
+    b.set_contact_model(1)
+    b.set_params(0.21, 0.0, 1e-3, 0.0, 0.01)
+    b.calc_forces()
+    b.run_steps(30, 0)
+    p = b.particles()
+    d = lambda u, v, L: np.max(np.abs((u - v) - L * np.round((u - v) / L)))
+    assert d(x, p["x"], pk.L) < 1e-10
+    assert d(y, p["y"], pk.L) < 1e-10
+    assert d(phi, p["phi"], 2 * 3.14159265358979) < 1e-10
+
+
+def _fire_two_point(pk, precision, max_iter):
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_contact_model("two_point")
+        s.calc_se()
+        start = s.box()
+        if precision == "mixed":
+            s.set_precision("mixed")
+        it, fr, st = s.fire(native.fire_params(**FIRE, max_iter=max_iter), switch_frms=1e-6)
+        x, y, phi = s.positions()
+        bx = s.box()
+    finally:
+        s.close()
+    print("two-point FIRE", pk.name, precision, "iterations", int(it[0]), "F_rms", float(fr[0]),
+          flush=True)
+    assert st == 0 and fr[0] < 1e-10
+    ob = oracle.OracleBox(pk.L, x, y, phi, pk.A, pk.R, 0.1)
+    ob.set_contact_model(1)
+    assert ob.frms() < 1e-10  # the oracle judges the GPU's final state
+    ob.calc_se()
+    return start, bx, ob.scalars()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_point_fire_unjammed(precision):
+    """FIRE with the two-point model on a packing (aspect 2, phi 0.83) where the
+    closest-point model stalls.  It relaxes to an overlap-free state: energy and
+    stress end at rounding level (~1e-20, ~1e-11), where each contact force is
+    a difference of nearly equal numbers, so they are compared against the
+    oracle at the GPU's final state in absolute terms, 1e-8 of their values at
+    the start of the minimisation."""
+    R, A = configs.bidisperse(1024)
+    pk = configs.make_packing(R, A, 0.83, 67, "unjammed aspect 2 phi 0.83")
+    start, bx, o = _fire_two_point(pk, precision, 20000)
+    for k in ("E", "cPxx", "cPyy", "cPxy"):
+        assert abs(bx[k] - o[k]) <= 1e-8 * abs(start[k]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_point_fire_jammed_stress(precision):
+    """A jammed packing (aspect 2, phi 0.90): FIRE reaches F_rms < 1e-10 at a
+    finite pressure, and the stress and energy the GPU reports agree with the
+    FP64 oracle's at that state to 1e-8 relative (mixed mode included: its
+    last iterations, the residual and the reported state are FP64)."""
+    R, A = configs.bidisperse(256)
+    pk = configs.make_packing(R, A, 0.90, 1, "jammed aspect 2 phi 0.90")
+    _, bx, o = _fire_two_point(pk, precision, 60000)
+    assert o["E"] > 1e-8
+    for k in ("E", "cPxx", "cPyy", "Pxx", "Pyy"):
+        assert bx[k] == pytest.approx(o[k], rel=1e-8), k
+    assert bx["cPxy"] == pytest.approx(o["cPxy"], rel=1e-8, abs=1e-8 * abs(o["cPxx"]))

@@ -348,19 +348,39 @@ FIRE = np.array([0.05, 0.5, 5, 1.1, 0.5, 0.1, 0.99, 0.02, 1e-10, 200000], dtype=
 
 
 def test_fire_converges_small():
-    # below jamming the minimum is overlap-free, so F_rms reaches the tolerance
+    # an overlap-free minimum without aligned overlapping pairs on the way:
+    # F_rms reaches the tolerance with every particle still finite
     R, A = configs.bidisperse(256)
-    pk = configs.make_packing(R, A, 0.75, 41)
+    pk = configs.make_packing(R, A, 0.70, 41)
     b = oracle.OracleBox.from_packing(pk, 0.1)
     it, fr = b.fire(FIRE)
     assert fr < 1e-10 and it < 5000
     assert b.frms() < 1e-10
+    p = b.particles()
+    assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["phi"]))
+
+
+def test_kat_crossed_rods_sit_on_an_energy_plateau():
+    """Rods crossing in an X: their segments intersect, d = 0 for every small
+    displacement, so the energy is flat (its maximum, (1 - 0)^2 / 2 per
+    contact) and the force and torque vanish; the reference would normalise
+    the rounding noise of D instead."""
+    p, s = two_body(20.0, (10.0, 10.0, 0.3, 1.0, 0.5), (10.1, 10.05, 1.9, 1.0, 0.5))
+    np.testing.assert_array_equal(p["fx"], [0.0, 0.0])
+    np.testing.assert_array_equal(p["fy"], [0.0, 0.0])
+    np.testing.assert_array_equal(p["tau"], [0.0, 0.0])
+    assert s["E"] == pytest.approx(0.5 / 2, rel=1e-15)
+    # just uncrossed (the end of q clear of p by 1e-3): the full contact force
+    p2, _ = two_body(20.0, (10.0, 10.0, 0.0, 1.0, 0.5), (10.0, 11.001, math.pi / 2, 1.0, 0.5))
+    assert p2["fy"][0] == pytest.approx(-(1 - 0.001) / 1.0, rel=1e-12)
 
 
 def test_fire_stalls_at_parallel_kink():
-    """Above jamming the single-contact energy has |theta| kinks at aligned
-    overlapping pairs (distance h - A|theta|), so the torque residual cannot
-    vanish; the minimiser reports non-convergence instead of a false pass."""
+    """The closest-point energy has |theta| kinks at aligned overlapping pairs
+    (distance h - A|theta|): the torque flips sign across alignment, FIRE keeps
+    resetting, and the residual does not vanish even where the two-point model
+    relaxes the same packing to zero energy (tests/test_two_point_cpu.py); the
+    minimiser reports non-convergence instead of a false pass."""
     R, A = configs.bidisperse(128)
     pk = configs.make_packing(R, A, 0.86, 47)
     b = oracle.OracleBox.from_packing(pk, 0.1)
