@@ -91,16 +91,18 @@ def minimization(n_packings, reps, cpu_budget_s):
     """packings/s minimised to F_rms < 1e-10 over a ragged ensemble: N log-uniform
     in [64, 4096], phi U[0.80, 0.86], aspect {1.2, 2, 4}; each packing first
     gets 2000 sheared Heun steps (strain rate 1e-4, dt 0.01), then FIRE with the
-    two-point contact model.  Three GPU paths from the same states: all packings
-    batched in mixed precision, batched in FP64, and one packing at a time in
-    FP64."""
+    two-point contact model.  GPU paths from the same states: all packings
+    batched in FP64 (the headline), batched in mixed precision, batched FP64
+    from the learned warm start (when trained weights exist), and one packing
+    at a time in FP64.  Each rate counts only packings that reached the
+    tolerance."""
     from spherocyls_amd import ensemble
 
     pks = configs.ensemble(n_packings)
     t0 = time.perf_counter()
     work = ensemble.prepare(pks, heun_steps=2000, strain_rate=1e-4, dt=0.01)
     prep_s = time.perf_counter() - t0
-    params = native.fire_params(max_iter=100000)
+    params = native.fire_params(max_iter=20000)
     out = {"metric": "packings/s minimized to F_rms<1e-10", "unit": "packings/s",
            "packings": n_packings, "particles": work.particles, "contact_model": "two_point",
            "prepare_s": prep_s}
@@ -114,17 +116,30 @@ def minimization(n_packings, reps, cpu_budget_s):
         sec, it, fr, st = best
         conv = int(np.sum(fr < 1e-10))
         pit = float(np.sum(np.array(work.sizes) * it))
-        runs[label] = {"value": n_packings / sec, "seconds": sec, "converged": conv,
+        # only packings that reached F_rms < 1e-10 count
+        runs[label] = {"value": conv / sec, "seconds": sec, "converged": conv,
                        "max_iterations": int(it.max()), "mean_iterations": float(it.mean()),
                        "particle_iterations_per_s": pit / sec,
                        "hbm_frac_fire_bytes": pit * BYTES_FIRE / sec / (HBM_PEAK_GBS * 1e9)}
+    from spherocyls_amd import warmstart
+    if os.path.exists(warmstart.DEFAULT_WEIGHTS):
+        net = warmstart.load()
+        try:
+            sec, it, fr, fb = warmstart.minimize_with_warm_start(work, params, net, precision="fp64")
+        finally:
+            net.close()
+        conv = int(np.sum(fr < 1e-10))
+        runs["batched_fp64_warm_start"] = {"value": conv / sec, "seconds": sec, "converged": conv,
+                                           "mean_iterations": float(it.mean()),
+                                           "fallbacks": len(fb)}
     sec, it, fr = ensemble.minimize_one_by_one(work, params, precision="fp64")
-    runs["one_by_one_fp64"] = {"value": n_packings / sec, "seconds": sec,
-                               "converged": int(np.sum(fr < 1e-10)),
+    conv = int(np.sum(fr < 1e-10))
+    runs["one_by_one_fp64"] = {"value": conv / sec, "seconds": sec, "converged": conv,
                                "mean_iterations": float(it.mean())}
     out.update(runs)
-    out["value"] = runs["batched_mixed"]["value"]
-    out["speedup_vs_one_by_one_fp64"] = runs["batched_mixed"]["value"] / runs["one_by_one_fp64"]["value"]
+    # headline: every packing in one batched FP64 system
+    out["value"] = runs["batched_fp64"]["value"]
+    out["speedup_vs_one_by_one_fp64"] = runs["batched_fp64"]["value"] / runs["one_by_one_fp64"]["value"]
     if cpu_budget_s > 0:
         out["cpu_baseline"] = cpu_minimization(work, params, cpu_budget_s)
     return out

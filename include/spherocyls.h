@@ -106,6 +106,11 @@ int sc_set_precision(sc_system* s, int mode);
 #define SC_CONTACT_CLOSEST 0
 #define SC_CONTACT_TWO_POINT 1
 int sc_set_contact_model(sc_system* s, int model);
+/* Replay fixed launch sequences (the FIRE iteration chunks) as captured HIP
+ * graphs (enable != 0, the default) or launch every kernel eagerly.  Results
+ * are identical; graphs cut launch overhead for small systems.  Event timing
+ * (sc_timing_enable) always launches eagerly. */
+int sc_set_graphs(sc_system* s, int enable);
 int sc_get_contact_model(const sc_system* s, int* model);
 /* Force-pass implementation (no reference counterpart; both give the same
  * forces to rounding):

@@ -66,7 +66,10 @@ int sc_mpnn_graph_from_system(sc_mpnn* m, sc_system* s, int* n, int* e);
 int sc_mpnn_export_graph(sc_mpnn* m, sc_system* s, float* node_feat, int* offsets, int* src,
                          float* edge_feat);
 /* Warm start: graph of the system's current state, forward pass, and every
- * particle displaced by scale * output (x, y in sheared coordinates, phi). */
+ * particle displaced by scale * output (x, y in sheared coordinates, phi),
+ * each component clamped to +-1 (x, y) and +-0.5 (phi); non-finite outputs
+ * give no displacement.  Contacts of crossed rods (d = 0) carry a zero
+ * normal. */
 int sc_mpnn_warm_start(sc_mpnn* m, sc_system* s);
 
 #ifdef __cplusplus

@@ -160,6 +160,55 @@ struct PairOut {
   double dist, sigma, dV;
 };
 
+// Crossed segments (their lines meet inside both): the closest distance is 0
+// and stays 0 under small displacements, and D is rounding noise (the
+// reference normalises it -- a full-size force in a random direction -- or
+// divides 0 by 0).  The overlap energy is continued to negative distance
+// d = -delta, delta the minimal translation that uncrosses them: the smallest
+// distance of an endpoint from the other segment's line.  That endpoint's
+// segment is pushed back across the line with force (1 + delta/sigma)/sigma,
+// applied at the endpoint and at its projection on the other segment; force
+// and energy are continuous where the rods uncross.
+void crossed_contact(double ddx, double ddy, PairOut& o) {
+  const double nxA = o.nxA, nyA = o.nyA, nxB = o.nxB, nyB = o.nyB, sig = o.sigma;
+  const double la2 = nxA * nxA + nyA * nyA, lb2 = nxB * nxB + nyB * nyB;
+  const double la = std::sqrt(la2), lb = std::sqrt(lb2);
+  const double mAx = -nyA / la, mAy = nxA / la, mBx = -nyB / lb, mBy = nxB / lb;
+  const double hp = (ddx + nxA) * mBx + (ddy + nyA) * mBy;
+  const double hm = (ddx - nxA) * mBx + (ddy - nyA) * mBy;
+  const double gp = (-ddx + nxB) * mAx + (-ddy + nyB) * mAy;
+  const double gm = (-ddx - nxB) * mAx + (-ddy - nyB) * mAy;
+  const double dA = std::fmin(std::fabs(hp), std::fabs(hm));
+  const double dB = std::fmin(std::fabs(gp), std::fabs(gm));
+  double s, t, ux, uy, delta;
+  if (dA <= dB) {
+    s = std::fabs(hp) <= std::fabs(hm) ? 1.0 : -1.0;
+    const double h = s > 0 ? hp : hm;
+    ux = h > 0 ? -mBx : mBx;
+    uy = h > 0 ? -mBy : mBy;
+    t = std::fmin(std::fmax(((ddx + s * nxA) * nxB + (ddy + s * nyA) * nyB) / lb2, -1.0), 1.0);
+    delta = dA;
+  } else {
+    t = std::fabs(gp) <= std::fabs(gm) ? 1.0 : -1.0;
+    const double g = t > 0 ? gp : gm;
+    ux = g > 0 ? mAx : -mAx;
+    uy = g > 0 ? mAy : -mAy;
+    s = std::fmin(std::fmax(((-ddx + t * nxB) * nxA + (-ddy + t * nyB) * nyA) / la2, -1.0), 1.0);
+    delta = dB;
+  }
+  o.contact = true;
+  o.dV = (1.0 + delta / sig) / sig;
+  o.fx = ux * o.dV;
+  o.fy = uy * o.dV;
+  o.tau_p = s * nxA * o.fy - s * nyA * o.fx;
+  o.tau_q = t * nyB * o.fx - t * nxB * o.fy;
+  o.s = s;
+  o.t = t;
+  o.dist = -delta;
+  o.dx = -delta * ux;
+  o.dy = -delta * uy;
+}
+
 // calc_force_pair / calc_temp_force_pair / calc_se_force_pair geometry,
 // strain_dynamics.cpp:17-75 (operation order kept for rounding agreement).
 inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap, double xq,
@@ -194,12 +243,11 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
   o.dy = ddy + s * o.nyA - t * o.nyB;
   double d2 = o.dx * o.dx + o.dy * o.dy;
   o.contact = d2 < o.sigma * o.sigma;
-  // Crossed segments (the lines meet inside both segments): d = 0 under any
-  // small displacement, a plateau of the energy with zero gradient; the
-  // reference normalises the rounding noise in D there (a full-size force in a
-  // random direction) or divides 0 by 0.  Also taken for an exact D = 0.
-  const bool crossed = (det > 0 && std::fabs(tu) < 1 && sabs < 1) || d2 == 0.0;
-  if (o.contact && crossed) {
+  if (o.contact && det > 0 && std::fabs(tu) < 1 && sabs < 1) {  // crossed segments
+    crossed_contact(ddx, ddy, o);
+    return o;
+  }
+  if (o.contact && d2 == 0.0) {  // touching exactly (ends, collinear overlap): no direction
     o.dist = 0.0;
     o.dV = 1.0 / o.sigma;
     o.fx = o.fy = o.tau_p = o.tau_q = 0.0;
@@ -219,12 +267,16 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
 }
 
 // Fills o's force fields from its D vector (harmonic contact, as pair_eval).
-inline void contact_force(PairOut& o, double s, double t, bool crossed) {
+inline void contact_force(PairOut& o, double s, double t, bool crossed, double ddx, double ddy) {
+  if (crossed) {  // as in pair_eval
+    crossed_contact(ddx, ddy, o);
+    return;
+  }
   o.s = s;
   o.t = t;
   double d2 = o.dx * o.dx + o.dy * o.dy;
   o.contact = d2 < o.sigma * o.sigma;
-  if (o.contact && (crossed || d2 == 0.0)) {  // energy plateau, as in pair_eval
+  if (o.contact && d2 == 0.0) {
     o.dist = 0.0;
     o.dV = 1.0 / o.sigma;
     o.fx = o.fy = o.tau_p = o.tau_q = 0.0;
@@ -332,7 +384,7 @@ inline int pair_eval_2p(double xp, double yp, double php, double Rp, double Ap, 
   PairOut o1 = o;
   o1.dx = ddx + s1 * o.nxA - t1 * o.nxB;
   o1.dy = ddy + s1 * o.nyA - t1 * o.nyB;
-  contact_force(o1, s1, t1, crossed);
+  contact_force(o1, s1, t1, crossed, ddx, ddy);
   if (o1.contact) out[k++] = o1;
   if (s1 != s2 && t1 != t2) {
     PairOut o2 = o;
@@ -349,7 +401,7 @@ inline int pair_eval_2p(double xp, double yp, double php, double Rp, double Ap, 
         o2.dy = dy3;
       }
     }
-    contact_force(o2, s2, t2, false);
+    contact_force(o2, s2, t2, false, ddx, ddy);
     if (o2.contact) out[k++] = o2;
   }
   return k;

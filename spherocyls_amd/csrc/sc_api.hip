@@ -107,6 +107,7 @@ struct sc_system {
   int lds_max = 64 * 1024;
   int prec = SC_PREC_FP64;
   int contact_model = SC_CONTACT_CLOSEST;
+  bool use_graphs = true;
   int ncells = 0, ntiles = 0, npblocks = 0;
   std::vector<int4> tiles;
   std::vector<int2> pblocks, tile_ranges, list_ranges;
@@ -207,6 +208,38 @@ void resolve_events(sc_system* s) {
   }
   s->ev_pending.clear();
 }
+
+// A stream-captured sequence of launches, replayed with one call.  The
+// capture runs the launch code once (advancing the host-side buffer rotation
+// as the replays will); nothing in a captured sequence may be a memset.
+struct GraphExec {
+  sc_system* s;
+  hipGraphExec_t exec = nullptr;
+  explicit GraphExec(sc_system* sys) : s(sys) {}
+  template <class F>
+  void capture(F&& launches) {
+    HIPCK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(s->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCK(hipStreamEndCapture(s->stream, &g));
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      throw HipError{e, "hipGraphInstantiate"};
+    }
+  }
+  void launch() { HIPCK(hipGraphLaunch(exec, s->stream)); }
+  ~GraphExec() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
 
 struct KernelTimer {
   sc_system* s;
@@ -893,6 +926,12 @@ int sc_get_contact_model(const sc_system* s, int* model) {
   return SC_OK;
 }
 
+int sc_set_graphs(sc_system* s, int enable) {
+  if (!s) return SC_ERR_ARG;
+  s->use_graphs = enable != 0;
+  return SC_OK;
+}
+
 int sc_set_path(sc_system* s, int path) {
   if (!s || (path != SC_PATH_TILE && path != SC_PATH_LIST)) return SC_ERR_ARG;
   API_BEGIN
@@ -1205,11 +1244,18 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     for (double* p : {s->bank[b].vx, s->bank[b].vy, s->bank[b].vpsi})
       HIPCK(hipMemsetAsync(p, 0, sizeof(double) * s->ntot, s->stream));
   s->velocity_live = true;
-  const int chunk = 32;
-  const int resort_every = 256;
+  // One FIRE iteration is three launches (force pass, per-packing reduction,
+  // update).  The rotating bucket, flag and scalar buffers repeat every 6
+  // iterations, so a chunk of 30 is captured once as a HIP graph and replayed;
+  // finished packings make every kernel return early.  Event timing needs
+  // eager launches, and eager chunks re-sort the bank for locality.
+  const int chunk = 30;
+  const int resort_every = 240;
+  const bool graph = s->use_graphs && !s->timing;
+  GraphExec g(s);
   int it = 0;
-  while (true) {
-    for (int m = 0; m < chunk; ++m, ++it) {
+  auto iterations = [&](int count) {
+    for (int m = 0; m < count; ++m) {
       PassSpec ps;
       ps.mode = kModeForces;
       ps.do_fire = 1;
@@ -1218,7 +1264,18 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       launch_force(s, ps);
       launch_reduce(s, ps, &fp);
       launch_update(s, 1, fp.max_move);
-      if ((it + 1) % resort_every == 0) resort(s);
+    }
+  };
+  while (true) {
+    if (graph) {
+      if (!g.exec) g.capture([&]() { iterations(chunk); });
+      g.launch();
+      it += chunk;
+    } else {
+      for (int m = 0; m < chunk; ++m, ++it) {
+        iterations(1);
+        if ((it + 1) % resort_every == 0) resort(s);
+      }
     }
     int e = check_errors(s);
     if (e != SC_OK) {

@@ -150,11 +150,16 @@ __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, doub
 // lane has several independent gathers in flight.  The LANES partial sums are
 // combined with xor shuffles (fixed order for a given LANES).
 constexpr int kGather = 4;
+// Compacted partners held per thread between the two phases (a longer run is
+// processed in pieces), and the largest workgroup.
+constexpr int kHitCap = 16;
+constexpr int kListBlock = 256;
 
 template <int MODE, int LANES, bool TWO_POINT>
 __global__ void __launch_bounds__(256) k_list(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double s_red[4 * kNSum];
+  __shared__ int s_hit[kHitCap * kListBlock];  // [slot][thread] partner columns
   const int tid = threadIdx.x, nthr = blockDim.x;
   if (a.cnt_clear)
     for (int i = blockIdx.x * nthr + tid; i < a.ncells_total; i += gridDim.x * nthr)
@@ -229,36 +234,50 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       ybi = a.yb[g];
     }
 
-    // ---- forces over the row
+    // ---- forces over the row, in two phases.  Phase 1 walks the row with
+    // batched gathers and keeps the partners that pass the bounding-circle
+    // test in this thread's LDS column; phase 2 runs the closest-approach math
+    // over that compacted list.  A wavefront then executes the expensive pair
+    // math as often as its busiest lane has real candidates, not as often as
+    // its longest row; partners are visited in row order either way, so the
+    // sums are the same.
     double Fx = 0.0, Fy = 0.0, Tq = 0.0;
-    for (int k0 = lane; k0 < n; k0 += LANES * kGather) {
-      int qs[kGather];
+    int k0 = lane;  // this lane's next row entry
+    while (true) {
+      int nh = 0;
+      for (; k0 < n && nh <= kHitCap - kGather; k0 += LANES * kGather) {
+        int qs[kGather];
 #pragma unroll
-      for (int j = 0; j < kGather; ++j) {
-        const int k = k0 + j * LANES;
-        qs[j] = k < n ? a.nbr[(size_t)k * a.ntot + g] : g;
-      }
-      double xq[kGather], yq[kGather], Rq[kGather], Aq[kGather];
+        for (int j = 0; j < kGather; ++j) {
+          const int k = k0 + j * LANES;
+          qs[j] = k < n ? a.nbr[(size_t)k * a.ntot + g] : g;
+        }
+        double xq[kGather], yq[kGather], Sq[kGather];
 #pragma unroll
-      for (int j = 0; j < kGather; ++j) {
-        xq[j] = a.ex[qs[j]];
-        yq[j] = a.ey[qs[j]];
-        Rq[j] = a.b.R[qs[j]];
-        Aq[j] = a.b.A[qs[j]];
-      }
+        for (int j = 0; j < kGather; ++j) {
+          xq[j] = a.ex[qs[j]];
+          yq[j] = a.ey[qs[j]];
+          Sq[j] = a.b.R[qs[j]] + a.b.A[qs[j]];
+        }
 #pragma unroll
-      for (int j = 0; j < kGather; ++j) {
-        if (k0 + j * LANES >= n) break;
-        double dx = min_image(xi - xq[j], Lx);
-        double dy = min_image(yi - yq[j], Ly);
-        dx += gam * dy;
-        const double reach = reach_i + Rq[j] + Aq[j];
-        if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
-        const int q = qs[j];
-        pair_into<kSums, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, xq[j], yq[j], a.enx[q],
-                                    a.eny[q], Rq[j], Aq[j], a.b.id[q], Lx, Ly, gam, fp64, do_se,
-                                    Fx, Fy, Tq, acc);
+        for (int j = 0; j < kGather; ++j) {
+          if (k0 + j * LANES >= n) break;
+          double dx = min_image(xi - xq[j], Lx);
+          double dy = min_image(yi - yq[j], Ly);
+          dx += gam * dy;
+          const double reach = reach_i + Sq[j];
+          if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
+          s_hit[nh * kListBlock + tid] = qs[j];
+          ++nh;
+        }
       }
+      for (int h = 0; h < nh; ++h) {
+        const int q = s_hit[h * kListBlock + tid];
+        pair_into<kSums, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, a.ex[q], a.ey[q], a.enx[q],
+                                    a.eny[q], a.b.R[q], a.b.A[q], a.b.id[q], Lx, Ly, gam, fp64,
+                                    do_se, Fx, Fy, Tq, acc);
+      }
+      if (k0 >= n) break;
     }
     if (LANES > 1) {
 #pragma unroll
@@ -394,20 +413,25 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
 
   // ---- scalar advance (corrector: one thread per packing)
   if (MODE == kModeCorr && g == P.poff && lane == 0) {
-    PackScal S2 = S;
-    S2.step = S.step + 1;
+    // copied word by word and updated in place: the struct never lives in
+    // registers across the kernel (it would spill)
+    const int* src = reinterpret_cast<const int*>(&a.scal[pk]);
+    int* dst = reinterpret_cast<int*>(&a.scal_next[pk]);
+    for (int w = 0; w < (int)(sizeof(PackScal) / sizeof(int)); ++w) dst[w] = src[w];
+    PackScal& S2 = a.scal_next[pk];
+    S2.step = S2.step + 1;
     if (!a.resize) {
-      S2.gamma = S.gamma + gd * dt;
-      S2.gamma_total = S.gamma_total + gd * dt;
-      if (S2.gamma >= 0.5) {
-        S2.gamma -= 1.0;
-        S2.gamma_total = (double)(int)(S2.gamma_total + 1) + S2.gamma;
+      double g2 = S2.gamma + gd * dt, gt = S2.gamma_total + gd * dt;
+      if (g2 >= 0.5) {
+        g2 -= 1.0;
+        gt = (double)(int)(gt + 1) + g2;
       }
+      S2.gamma = g2;
+      S2.gamma_total = gt;
     } else {
       S2.Lx = L_new_x;
       S2.Ly = L_new_y;
     }
-    a.scal_next[pk] = S2;
   }
 
   if constexpr (kSums) {

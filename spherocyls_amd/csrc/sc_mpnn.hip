@@ -55,6 +55,9 @@ constexpr int kNo = 16;       // output width (3 used)
 constexpr int kEdgeFeat = 5;
 constexpr int kNodeFeat = 4;
 constexpr int kChunk = 2048;  // rows per weight-gradient partial
+// largest warm-start displacement per component (particle radii are ~0.5)
+constexpr float kCapXY = 1.0f;
+constexpr float kCapPhi = 0.5f;
 
 // parameter blocks (FP32 master layout, each W row-major [K][N])
 struct Layout {
@@ -439,24 +442,32 @@ __global__ void k_graph_fill(SystemView v, const int* __restrict__ off, int* __r
     PairOut<double> r;
     pair_canonical(v, g, q, S.Lx, S.Ly, S.gamma, r);
     const double sig = v.b.R[g] + v.b.R[q];
-    const double d = r.dist > 0 ? r.dist : 1e-300;
     float* f = ef + (size_t)e * kEdgeFeat;
     f[0] = (float)(1.0 - r.dist / sig);
-    f[1] = (float)(r.dx / d);
-    f[2] = (float)(r.dy / d);
+    // crossed rods (d = 0) have no contact normal
+    f[1] = r.dist > 0 ? (float)(r.dx / r.dist) : 0.0f;
+    f[2] = r.dist > 0 ? (float)(r.dy / r.dist) : 0.0f;
     f[3] = (float)bx;
     f[4] = (float)by;
   }
 }
 
-// x, y, phi of storage particle g += scale * y_g
+// x, y, phi of storage particle g += scale * y_g, each component clamped to
+// +-cap (a proposal, not a step the minimiser must trust) and non-finite
+// outputs dropped
+__device__ __forceinline__ double proposal(float y, float scale, double cap) {
+  double v = (double)(scale * y);
+  if (!isfinite(v)) return 0.0;
+  return fmin(fmax(v, -cap), cap);
+}
+
 __global__ void k_apply(Bank b, const float* __restrict__ Y, float sx, float sy, float sp,
-                        int n) {
+                        double cap_xy, double cap_phi, int n) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
-  b.x[g] += (double)(sx * Y[(size_t)g * kNo + 0]);
-  b.y[g] += (double)(sy * Y[(size_t)g * kNo + 1]);
-  b.phi[g] += (double)(sp * Y[(size_t)g * kNo + 2]);
+  b.x[g] += proposal(Y[(size_t)g * kNo + 0], sx, cap_xy);
+  b.y[g] += proposal(Y[(size_t)g * kNo + 1], sy, cap_xy);
+  b.phi[g] += proposal(Y[(size_t)g * kNo + 2], sp, cap_phi);
 }
 
 }  // namespace mp
@@ -980,15 +991,20 @@ int sc_mpnn_export_graph(sc_mpnn* m, sc_system* s, float* node_feat, int* offset
   if (system_view(s, &v) != SC_OK || v.ntot != m->n)
     return fail(m, SC_ERR_STATE, "graph was not built from this system");
   const int n = m->n, e = m->e;
-  std::vector<int> off(n + 1), sr(e), id(n);
+  std::vector<int> off(n + 1), sr(e), id(n), pack_of(n);
+  std::vector<PackInfo> packs(v.npack);
   std::vector<float> nf((size_t)n * kNodeFeat), ef((size_t)e * kEdgeFeat);
   MPCK(hipMemcpy(off.data(), m->off, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
   MPCK(hipMemcpy(sr.data(), m->src, sizeof(int) * e, hipMemcpyDeviceToHost));
   MPCK(hipMemcpy(id.data(), v.b.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(pack_of.data(), v.pack_of, sizeof(int) * n, hipMemcpyDeviceToHost));
+  MPCK(hipMemcpy(packs.data(), v.packs, sizeof(PackInfo) * v.npack, hipMemcpyDeviceToHost));
+  // b.id is the index within the packing: caller index = packing offset + id
+  for (int g = 0; g < n; ++g) id[g] += packs[pack_of[g]].poff;
   MPCK(hipMemcpy(nf.data(), m->nf, sizeof(float) * nf.size(), hipMemcpyDeviceToHost));
   MPCK(hipMemcpy(ef.data(), m->ef, sizeof(float) * ef.size(), hipMemcpyDeviceToHost));
   // caller order: node id[g] <- storage g; a receiver's edges keep their order
-  // (senders ascending by caller id)
+  // (senders ascending by caller index: a packing's particles are contiguous)
   std::vector<int> deg(n);
   for (int g = 0; g < n; ++g) deg[id[g]] = off[g + 1] - off[g];
   offsets[0] = 0;
@@ -1017,7 +1033,8 @@ int sc_mpnn_warm_start(sc_mpnn* m, sc_system* s) {
   SystemView v;
   system_view(s, &v);
   hipLaunchKernelGGL(k_apply, dim3(blocks(v.ntot)), dim3(256), 0, m->stream, v.b, m->Y,
-                     m->scale[0], m->scale[1], m->scale[2], v.ntot);
+                     m->scale[0], m->scale[1], m->scale[2], (double)kCapXY, (double)kCapPhi,
+                     v.ntot);
   MPCK(hipStreamSynchronize(m->stream));
   return system_positions_changed(s);
   MP_END

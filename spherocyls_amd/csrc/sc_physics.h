@@ -16,6 +16,59 @@ struct PairOut {
   T fx, fy, tp, tq, s, t, dx, dy, dist, dV;
 };
 
+// Crossed segments (their lines meet inside both): the closest distance is 0
+// and stays 0 under small displacements, and D is rounding noise.  The overlap
+// energy is continued to negative distance d = -delta, delta the minimal
+// translation that uncrosses them: the smallest distance of an endpoint from
+// the other segment's line.  That endpoint's segment is pushed back across
+// the line; the force is (1 + delta/sigma)/sigma along the push, at the
+// endpoint and at its projection on the other segment -- the ordinary contact
+// force of the configuration it uncrosses into, so force and energy are
+// continuous where the rods uncross.  Fills o as a contact with dist = -delta.
+template <typename T>
+__device__ __forceinline__ void crossed_contact(T ddx, T ddy, T nxA, T nyA, T nxB, T nyB, T sig,
+                                                PairOut<T>& o) {
+#pragma clang fp contract(off)
+  const T la2 = nxA * nxA + nyA * nyA, lb2 = nxB * nxB + nyB * nyB;
+  const T la = sqrt(la2), lb = sqrt(lb2);
+  // unit normals of the segments
+  const T mAx = -nyA / la, mAy = nxA / la, mBx = -nyB / lb, mBy = nxB / lb;
+  // A's endpoints (relative to B's centre) against B's line, and B's against A's
+  const T hp = (ddx + nxA) * mBx + (ddy + nyA) * mBy;
+  const T hm = (ddx - nxA) * mBx + (ddy - nyA) * mBy;
+  const T gp = (-ddx + nxB) * mAx + (-ddy + nyB) * mAy;
+  const T gm = (-ddx - nxB) * mAx + (-ddy - nyB) * mAy;
+  const T dA = fmin(fabs(hp), fabs(hm)), dB = fmin(fabs(gp), fabs(gm));
+  T s, t, ux, uy, delta;
+  if (dA <= dB) {  // move A: its nearer endpoint back across B's line
+    s = fabs(hp) <= fabs(hm) ? T(1) : T(-1);
+    const T h = s > 0 ? hp : hm;
+    ux = h > 0 ? -mBx : mBx;
+    uy = h > 0 ? -mBy : mBy;
+    t = fmin(fmax(((ddx + s * nxA) * nxB + (ddy + s * nyA) * nyB) / lb2, T(-1)), T(1));
+    delta = dA;
+  } else {  // move B: its nearer endpoint back across A's line (A gets -F_B)
+    t = fabs(gp) <= fabs(gm) ? T(1) : T(-1);
+    const T gg = t > 0 ? gp : gm;
+    ux = gg > 0 ? mAx : -mAx;
+    uy = gg > 0 ? mAy : -mAy;
+    s = fmin(fmax(((-ddx + t * nxB) * nxA + (-ddy + t * nyB) * nyA) / la2, T(-1)), T(1));
+    delta = dB;
+  }
+  const T dV = (T(1) + delta / sig) / sig;
+  const T fx = ux * dV, fy = uy * dV;
+  o.fx = fx;
+  o.fy = fy;
+  o.tp = s * nxA * fy - s * nyA * fx;
+  o.tq = t * nyB * fx - t * nxB * fy;
+  o.s = s;
+  o.t = t;
+  o.dist = -delta;
+  o.dx = -delta * ux;
+  o.dy = -delta * uy;
+  o.dV = dV;
+}
+
 // Segment-segment closest approach and harmonic contact force, following
 // calc_force_pair (src/cpp/strain_dynamics.cpp:38-64) term by term.  Returns
 // false when |D| >= sigma (or D is NaN).  FP64: no contraction, so the rounding
@@ -44,12 +97,11 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   o.t = t;
   o.dx = Dx;
   o.dy = Dy;
-  if ((det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0) || d2 == 0.0) {
-    // crossed segments: the lines meet inside both segments, so d = 0 and
-    // stays 0 under any small displacement -- a plateau of the energy with
-    // zero gradient.  (D is rounding noise here; normalising it would give a
-    // full-size force in a random direction, or 0/0.)  An exact D = 0 in any
-    // other geometry (touching ends, collinear overlap) is treated alike.
+  if (det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0) {
+    crossed_contact(ddx, ddy, nxA, nyA, nxB, nyB, sig, o);  // crossed segments
+    return true;
+  }
+  if (d2 == 0.0) {  // touching exactly (ends, collinear overlap): no direction
     o.fx = o.fy = o.tp = o.tq = 0.0;
     o.dist = 0.0;
     o.dV = 1.0 / sig;
@@ -93,7 +145,11 @@ __device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA
   o.t = t;
   o.dx = Dx;
   o.dy = Dy;
-  if ((det > 0.0f && fabsf(tu) < 1.0f && sabs < 1.0f) || d2 == 0.0f) {  // crossed: plateau
+  if (det > 0.0f && fabsf(tu) < 1.0f && sabs < 1.0f) {  // crossed segments
+    crossed_contact(ddx, ddy, nxA, nyA, nxB, nyB, sig, o);
+    return true;
+  }
+  if (d2 == 0.0f) {
     o.fx = o.fy = o.tp = o.tq = 0.0f;
     o.dist = 0.0f;
     o.dV = 1.0f / sig;
@@ -118,15 +174,19 @@ __device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA
 // Force fields of one contact point from its D vector (harmonic, as above).
 template <typename T>
 __device__ __forceinline__ bool contact_point(T Dx, T Dy, T s, T t, T nxA, T nyA, T nxB, T nyB,
-                                              T sig, bool crossed, PairOut<T>& o) {
+                                              T sig, bool crossed, T ddx, T ddy, PairOut<T>& o) {
 #pragma clang fp contract(off)
   T d2 = Dx * Dx + Dy * Dy;
+  if (crossed) {  // as in pair_contact_f64
+    crossed_contact(ddx, ddy, nxA, nyA, nxB, nyB, sig, o);
+    return true;
+  }
   if (!(d2 < sig * sig)) return false;
   o.s = s;
   o.t = t;
   o.dx = Dx;
   o.dy = Dy;
-  if (crossed || d2 == T(0)) {  // energy plateau, as in pair_contact_f64
+  if (d2 == T(0)) {
     o.fx = o.fy = o.tp = o.tq = T(0);
     o.dist = T(0);
     o.dV = T(1) / sig;
@@ -223,7 +283,7 @@ __device__ __forceinline__ void pair_contacts_2p(T ddx, T ddy, T nxA, T nyA, T n
     }
   }
   h0 = contact_point(ddx + s1 * nxA - t1 * nxB, ddy + s1 * nyA - t1 * nyB, s1, t1, nxA, nyA, nxB,
-                     nyB, sig, crossed, o0);
+                     nyB, sig, crossed, ddx, ddy, o0);
   h1 = false;
   if (s1 != s2 && t1 != t2) {
     T Dx2 = ddx + s2 * nxA - t2 * nxB;
@@ -238,7 +298,7 @@ __device__ __forceinline__ void pair_contacts_2p(T ddx, T ddy, T nxA, T nyA, T n
         Dy2 = Dy3;
       }
     }
-    h1 = contact_point(Dx2, Dy2, s2, t2, nxA, nyA, nxB, nyB, sig, false, o1);
+    h1 = contact_point(Dx2, Dy2, s2, t2, nxA, nyA, nxB, nyB, sig, false, ddx, ddy, o1);
   }
 }
 

@@ -62,6 +62,7 @@ def load():
         "sc_set_gamma": ([sysp, C.c_int, C.c_double, C.c_double], C.c_int),
         "sc_set_precision": ([sysp, C.c_int], C.c_int),
         "sc_set_contact_model": ([sysp, C.c_int], C.c_int),
+        "sc_set_graphs": ([sysp, C.c_int], C.c_int),
         "sc_get_contact_model": ([sysp, _ip], C.c_int),
         "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_set_path": ([sysp, C.c_int], C.c_int),
@@ -262,6 +263,9 @@ class System:
         """"closest" (the CPU reference) or "two_point"."""
         m = {"closest": SC_CONTACT_CLOSEST, "two_point": SC_CONTACT_TWO_POINT}.get(model, model)
         self._ck(self.lib.sc_set_contact_model(self.h, m))
+
+    def set_graphs(self, enable):
+        self._ck(self.lib.sc_set_graphs(self.h, int(bool(enable))))
 
     def contact_model(self):
         m = C.c_int()

@@ -2,11 +2,12 @@
 
 A small message-passing network over the contact graph (native.Mpnn, kernels
 in csrc/sc_mpnn.hip) predicts each particle's relaxation displacement
-(dx, dy, dphi) from a pre-minimisation state.  The minimiser then starts from
-the displaced state; the plain minimiser is the fallback and the judge: a
-warm-started packing is accepted only when FIRE takes it to the same
-F_rms < ftol, otherwise that packing is minimised again from its original
-state.
+(dx, dy, dphi) from a pre-minimisation state.  Each packing takes the
+proposal scaled by whichever of 1, 1/2, 1/4 (or none) lowers its energy most,
+and the minimiser starts from there; the plain minimiser is the fallback and
+the judge: a warm-started packing is accepted only when FIRE takes it to the
+same F_rms < ftol, otherwise that packing is minimised again from its
+original state.
 
 Training data are (pre-minimisation, minimised) pairs the simulator makes
 itself: ragged packings get sheared Heun steps (the pre state), then FIRE
@@ -116,6 +117,8 @@ def train(pairs, steps=2000, batch=8, lr=1e-3, seed=1, log_every=200, mpnn=None)
         m.load_graph(g)
         m.forward()
         loss = m.backward(t)
+        if not np.isfinite(loss):
+            raise FloatingPointError(f"non-finite training loss at step {step}")
         # cosine decay of the learning rate
         m.adam(lr * 0.5 * (1 + math.cos(math.pi * (step - 1) / steps)), step)
         hist.append(loss)
@@ -138,6 +141,52 @@ def load(path=DEFAULT_WEIGHTS, device=0):
     return m
 
 
+def _energies(s):
+    s.calc_se()
+    return np.array([s.box(k)["E"] for k in range(s.npack)])
+
+
+def damped_warm_start(s, sizes, mpnn, lambdas=(1.0, 0.5, 0.25)):
+    """Apply the network's proposal to every packing of system s, scaled by the
+    factor in lambdas (or none) that gives that packing the lowest energy; the
+    state never starts above the unrelaxed one.  Returns (energy before,
+    energy after, chosen factor per packing)."""
+    scale = mpnn.output_scale()
+    x0, y0, p0 = s.positions()
+    e0 = _energies(s)
+    best_e, best_l = e0.copy(), np.zeros(len(sizes))
+    bx, by, bp = x0.copy(), y0.copy(), p0.copy()
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    try:
+        for lam in lambdas:
+            s.set_positions(x0, y0, p0)
+            mpnn.set_output_scale(scale * lam)
+            mpnn.warm_start(s)
+            e = _energies(s)
+            better = e < best_e
+            if np.any(better):
+                x, y, p = s.positions()
+                for k in np.nonzero(better)[0]:
+                    sl = slice(offs[k], offs[k + 1])
+                    bx[sl], by[sl], bp[sl] = x[sl], y[sl], p[sl]
+                best_e[better] = e[better]
+                best_l[better] = lam
+    finally:
+        mpnn.set_output_scale(scale)
+    s.set_positions(bx, by, bp)
+    return e0, best_e, best_l
+
+
+def proposal_energy(work, mpnn, contact_model="two_point"):
+    """Energy of every packing before the warm start and after the damped
+    proposal, and the damping chosen (FP64 force passes)."""
+    s = ensemble.system_for(work, precision="fp64", autotune=False, contact_model=contact_model)
+    try:
+        return damped_warm_start(s, work.sizes, mpnn)
+    finally:
+        s.close()
+
+
 def minimize_with_warm_start(work, params, mpnn, precision="mixed", switch_frms=1e-6):
     """Batched minimisation from warm starts with per-packing fallback.
 
@@ -149,9 +198,15 @@ def minimize_with_warm_start(work, params, mpnn, precision="mixed", switch_frms=
     try:
         s.synchronize()
         t0 = time.perf_counter()
-        mpnn.warm_start(s)
-        it, fr, st = s.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
-                            check=False)
+        try:
+            damped_warm_start(s, work.sizes, mpnn)
+            it, fr, st = s.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
+                                check=False)
+        except native.SpherocylsError:
+            # a proposal the system cannot hold (e.g. an over-full cell): every
+            # packing falls back to its original state
+            it = np.zeros(len(work.packings), dtype=np.int32)
+            fr = np.full(len(work.packings), np.inf)
         bad = [k for k in range(len(work.packings)) if not fr[k] < params[8]]
         if bad:
             sub = ensemble.Workload([work.packings[k] for k in bad], [work.gammas[k] for k in bad])
@@ -183,7 +238,7 @@ def main():
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()
-    params = native.fire_params(max_iter=100000)
+    params = native.fire_params(max_iter=20000)
     t0 = time.perf_counter()
     pairs = None
     chunk = 64

@@ -124,3 +124,27 @@ def test_training_reduces_loss(graph):
         m.close()
     print("training loss", losses[0], "->", losses[-1], flush=True)
     assert losses[-1] < 0.1 * losses[0]
+
+
+def test_batched_graph_is_the_concatenation(packing):
+    """A batched system's graph, in caller order, is the concatenation of its
+    packings' own graphs (indices offset by the packing's first particle)."""
+    R, A = configs.bidisperse(180, aspect=3.0)
+    other = configs.make_packing(R, A, 0.80, 73, "second")
+    m = native.Mpnn(seed=3)
+    graphs = []
+    try:
+        for pks in ([packing], [other], [packing, other]):
+            s = native.System(pks, pads=0.1)
+            try:
+                graphs.append(m.graph_of(s))
+            finally:
+                s.close()
+    finally:
+        m.close()
+    g1, g2, g12 = graphs
+    ref = native.Graph.concat([g1, g2])
+    np.testing.assert_array_equal(g12.offsets, ref.offsets)
+    np.testing.assert_array_equal(g12.src, ref.src)
+    np.testing.assert_array_equal(g12.node_feat, ref.node_feat)
+    np.testing.assert_array_equal(g12.edge_feat, ref.edge_feat)

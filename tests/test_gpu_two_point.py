@@ -125,10 +125,18 @@ def test_two_point_fire_jammed_stress(precision):
     """A jammed packing (aspect 2, phi 0.90): FIRE reaches F_rms < 1e-10 at a
     finite pressure, and the stress and energy the GPU reports agree with the
     FP64 oracle's at that state to 1e-8 relative (mixed mode included: its
-    last iterations, the residual and the reported state are FP64)."""
+    last iterations, the residual and the reported state are FP64).
+
+    The two-point model takes the closer of two candidate second contacts, so
+    its energy is the larger of two smooth branches where they tie: a convex
+    kink, on which some minima sit, and a FIRE trajectory that ends in one
+    stalls near F_rms ~ 1e-8.  Whether it does depends on the trajectory (on
+    seed 1 at this density, 2 of 8 oracle runs from starts perturbed by 1e-12
+    stall), so rounding differences alone can decide it.  This packing (seed 3)
+    converges from all 8 perturbed starts in the oracle (4567-4902 iterations)."""
     R, A = configs.bidisperse(256)
-    pk = configs.make_packing(R, A, 0.90, 1, "jammed aspect 2 phi 0.90")
-    _, bx, o = _fire_two_point(pk, precision, 60000)
+    pk = configs.make_packing(R, A, 0.90, 3, "jammed aspect 2 phi 0.90")
+    _, bx, o = _fire_two_point(pk, precision, 20000)
     assert o["E"] > 1e-8
     for k in ("E", "cPxx", "cPyy", "Pxx", "Pyy"):
         assert bx[k] == pytest.approx(o[k], rel=1e-8), k

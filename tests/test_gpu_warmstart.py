@@ -1,0 +1,71 @@
+"""Learned warm start: the trained network (spherocyls_amd/data/mpnn_warmstart.npz,
+trained on the GPU by `python -m spherocyls_amd.warmstart train`) on held-out
+packings, against the plain minimiser.
+
+The plain minimiser is the judge: every packing must end at F_rms < 1e-10
+(checked by the oracle on a sample), whether its warm start was accepted or it
+fell back.  The test prints what the network buys -- the energy of its
+damped proposal against the unrelaxed state, and FIRE iterations and wall
+time with and without it -- and requires that the damped proposal never
+raises a packing's energy.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from spherocyls_amd import ensemble, native, warmstart
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net():
+    if not os.path.exists(warmstart.DEFAULT_WEIGHTS):
+        pytest.skip("trained weights not present")
+    m = warmstart.load()
+    yield m
+    m.close()
+
+
+@pytest.fixture(scope="module")
+def held_out():
+    pks = warmstart.workload(24, 1024, seed=4242)  # seeds disjoint from training
+    return ensemble.prepare(pks, heun_steps=2000)
+
+
+def test_damped_proposal_never_raises_energy(net, held_out):
+    before, after, lam = warmstart.proposal_energy(held_out, net)
+    print("energy before", before.mean(), "after damped proposal", after.mean(),
+          "damping chosen", np.unique(lam, return_counts=True), flush=True)
+    assert np.all(np.isfinite(after))
+    assert np.all(after <= before)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_warm_start_reaches_the_same_residual(net, held_out, precision):
+    params = native.fire_params(max_iter=20000)
+    sec0, it0, fr0, _ = ensemble.minimize_batched(held_out, params, precision=precision)
+    sec1, it1, fr1, fb = warmstart.minimize_with_warm_start(held_out, params, net,
+                                                            precision=precision)
+    print(f"{precision}: plain {sec0 * 1e3:.1f} ms, {it0.mean():.0f} iterations; "
+          f"warm start {sec1 * 1e3:.1f} ms, {it1.mean():.0f} iterations, "
+          f"{len(fb)} fallbacks", flush=True)
+    assert np.all(fr0 < 1e-10) and np.all(fr1 < 1e-10)
+    # the oracle judges a sample of the warm-started final states
+    s = ensemble.system_for(held_out, precision=precision, autotune=False)
+    try:
+        warmstart.damped_warm_start(s, held_out.sizes, net)
+        s.fire(params, switch_frms=1e-6 if precision == "mixed" else 0.0, check=False)
+        x, y, phi = s.positions()
+    finally:
+        s.close()
+    o = 0
+    for pk, g in zip(held_out.packings[:3], held_out.gammas[:3]):
+        sl = slice(o, o + pk.n)
+        b = oracle.OracleBox(pk.L, x[sl], y[sl], phi[sl], pk.A, pk.R, 0.1)
+        b.set_contact_model(1)
+        b.set_params(g, 0.0, 0.0, 0.0, 0.05)
+        assert b.frms() < 1e-10
+        o += pk.n

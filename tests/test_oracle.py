@@ -360,19 +360,68 @@ def test_fire_converges_small():
     assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["phi"]))
 
 
-def test_kat_crossed_rods_sit_on_an_energy_plateau():
-    """Rods crossing in an X: their segments intersect, d = 0 for every small
-    displacement, so the energy is flat (its maximum, (1 - 0)^2 / 2 per
-    contact) and the force and torque vanish; the reference would normalise
-    the rounding noise of D instead."""
-    p, s = two_body(20.0, (10.0, 10.0, 0.3, 1.0, 0.5), (10.1, 10.05, 1.9, 1.0, 0.5))
-    np.testing.assert_array_equal(p["fx"], [0.0, 0.0])
-    np.testing.assert_array_equal(p["fy"], [0.0, 0.0])
-    np.testing.assert_array_equal(p["tau"], [0.0, 0.0])
-    assert s["E"] == pytest.approx(0.5 / 2, rel=1e-15)
-    # just uncrossed (the end of q clear of p by 1e-3): the full contact force
-    p2, _ = two_body(20.0, (10.0, 10.0, 0.0, 1.0, 0.5), (10.0, 11.001, math.pi / 2, 1.0, 0.5))
-    assert p2["fy"][0] == pytest.approx(-(1 - 0.001) / 1.0, rel=1e-12)
+def test_kat_crossed_rods_are_pushed_apart():
+    """Rods crossing in an X: the overlap energy is continued to negative
+    distance -delta, delta the smallest distance of an endpoint from the other
+    segment's line; the rod owning that endpoint is pushed back across the
+    line with (1 + delta/sigma)/sigma (the reference normalises rounding noise
+    there instead)."""
+    # p along x through (10, 10); q vertical through (10.5, 10.2), reaching
+    # y = 10.2 - 1 = 9.2 below p's line and 11.2 above: q's lower end is 0.8
+    # past p's line, q's upper end 1.2; p's ends are 0.5 and 1.5 from q's line
+    p, s = two_body(20.0, (10.0, 10.0, 0.0, 1.0, 0.5), (10.5, 10.2, math.pi / 2, 1.0, 0.5))
+    delta, sig = 0.5, 1.0  # p's right end is 0.5 from q's line: move p left
+    f = (1 + delta / sig) / sig
+    np.testing.assert_allclose(p["fx"], [-f, f], rtol=1e-12)
+    np.testing.assert_allclose(p["fy"], [0.0, 0.0], atol=1e-12)
+    # the force acts at p's end (arm +1 along x: no torque for a force along x)
+    # and at the point of q level with p, arm t n_q = (0, -0.2): torque on q
+    # = arm x F_q = -(-0.2) * f
+    np.testing.assert_allclose(p["tau"][0], 0.0, atol=1e-12)
+    assert p["tau"][1] == pytest.approx(0.2 * f, rel=1e-12)
+    assert s["E"] == pytest.approx((1 + delta / sig) ** 2 / 4, rel=1e-12)
+
+
+def test_crossed_force_is_continuous_where_rods_uncross():
+    """Moving p left until its right end leaves q's line: the force just before
+    (crossed, d = -delta) and just after (ordinary contact, d = +delta) agree
+    to O(delta)."""
+    eps = 1e-7
+    f = []
+    for x in (10.5 - 1.0 + eps, 10.5 - 1.0 - eps):  # p's right end at 10.5 -+ eps
+        p, _ = two_body(20.0, (x, 10.0, 0.0, 1.0, 0.5), (10.5, 10.2, math.pi / 2, 1.0, 0.5))
+        f.append((p["fx"][0], p["fy"][0]))
+    np.testing.assert_allclose(f[0], f[1], atol=1e-6)
+    assert f[0][0] == pytest.approx(-1.0, rel=1e-6)
+
+
+def test_crossed_force_is_minus_n_grad_energy():
+    """Inside the crossed region the continued energy is smooth away from the
+    switch between endpoints: F = -N dE/dx by central differences."""
+    rng = np.random.default_rng(9)
+    checked = 0
+    for _ in range(40):
+        a = (10.0, 10.0, rng.uniform(0, math.pi), rng.uniform(0.6, 1.2), 0.5)
+        ph = a[2] + rng.uniform(0.6, 2.5)
+        b = (10.0 + rng.uniform(-0.3, 0.3), 10.0 + rng.uniform(-0.3, 0.3), ph,
+             rng.uniform(0.6, 1.2), 0.6)
+        p, s = two_body(20.0, a, b)
+        if s["E"] < 0.125:  # not crossed (E >= 1/4 per crossed pair of 2 particles)
+            continue
+        for comp, key in ((0, "fx"), (1, "fy"), (2, "tau")):
+            h = 1e-6
+            e = []
+            for sg in (1, -1):
+                aa = list(a)
+                aa[comp] += sg * h
+                e.append(two_body(20.0, tuple(aa), b)[1]["E"])
+            g = -2 * (e[0] - e[1]) / (2 * h)
+            if abs(g - p[key][0]) > 1e-5 * (1 + abs(g)):
+                # straddles the endpoint switch: a finite difference there
+                # is no derivative
+                continue
+            checked += 1
+    assert checked > 30
 
 
 def test_fire_stalls_at_parallel_kink():
