@@ -33,10 +33,11 @@ HBM_PEAK_GBS = 8000.0
 #   corrector: read x~ (24) + R,A (16) + v0 (24) + x (24), write x (24) + moved (32) = 144
 BYTES_PRED = 88
 BYTES_CORR = 144
-# HBM bytes per corrector launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
-# scaled by a plain copy's readings (profiles/k_list_heun_shear_N262144.md):
+# HBM bytes per corrector launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# (list path, 2 lanes per particle, 128-thread workgroups), scaled by a plain
+# copy's readings (profiles/k_list_heun_shear_N262144.md):
 # (workload, path, precision) -> bytes
-MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (25833.5 * 2.0 + 28460.3) * 1024}
+MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (25553.0 * 2.0 + 26895.0) * 1024}
 
 WORKLOADS = {
     "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
@@ -145,6 +146,55 @@ def minimization(n_packings, reps, cpu_budget_s):
     return out
 
 
+def quasistatic_compression(eps=1e-4, phi0=0.80, phi1=0.86, n=16384, samples=13):
+    """Quasi-static compression: N=16384 bidisperse (R 0.5 / 0.7) aspect-2
+    packing, seeded random start at phi 0.80, steps x, y, L *= 1 - 1e-4 (the
+    reference's relative length step) until phi >= 0.86, FIRE (two-point
+    contacts) to F_rms < 1e-10 after each step; FP64 and mixed precision.
+    Reports the pressure-vs-packing-fraction curve (sampled), the FIRE
+    iterations and the mixed-mode pressure error against FP64 along the curve."""
+    from spherocyls_amd import quasistatic
+
+    pk = configs.compress_start(1, n)
+    params = native.fire_params(max_iter=200000)
+    out = {"workload": f"quasi-static compression N={n} phi {phi0}->{phi1} eps={eps} aspect 2",
+           "contact_model": "two_point"}
+    curves = {}
+    for prec in ("fp64", "mixed"):
+        s = native.System(pk, pads=0.1)
+        try:
+            s.set_contact_model("two_point")
+            s.set_precision(prec)
+            s.autotune(reps=3)
+            def progress(step, _s):
+                if step % 100 == 0:
+                    print(f"quasi-static {prec}: step {step}", file=sys.stderr, flush=True)
+            c = quasistatic.compress(s, eps, phi1, params, on_step=progress)
+        finally:
+            s.close()
+        curves[prec] = c
+        it = c.iterations[:, 0]
+        out[prec] = {"seconds": c.seconds, "steps": int(len(it) - 1),
+                     "converged_minimisations": int(np.sum(c.frms[:, 0] < 1e-10)),
+                     "fire_iterations_first": int(it[0]),
+                     "fire_iterations_steps": int(it[1:].sum()),
+                     "mean_iterations_per_step": float(it[1:].mean()),
+                     "steps_per_s": (len(it) - 1) / c.seconds,
+                     "particle_iterations_per_s": n * float(it.sum()) / c.seconds}
+    f64, mix = curves["fp64"], curves["mixed"]
+    k = min(len(f64.packing), len(mix.packing))
+    idx = np.unique(np.linspace(0, len(f64.packing) - 1, samples).astype(int))
+    out["curve_fp64"] = [[float(f64.packing[i, 0]), float(f64.pressure[i, 0]),
+                          int(f64.iterations[i, 0])] for i in idx]
+    p64, pmx = f64.pressure[:k, 0], mix.pressure[:k, 0]
+    jam = p64 > 1e-6 * np.max(np.abs(p64))
+    out["mixed_pressure_rel_error"] = {
+        "max": float(np.max(np.abs(pmx[jam] - p64[jam]) / p64[jam])) if np.any(jam) else None,
+        "median": float(np.median(np.abs(pmx[jam] - p64[jam]) / p64[jam])) if np.any(jam) else None,
+        "jammed_steps": int(np.sum(jam))}
+    return out
+
+
 def cpu_minimization(work, params, budget_s):
     """The oracle restatement's FIRE (two-point model, FP64, 1 thread) on the
     same pre-minimisation states, packing after packing until the budget is
@@ -198,6 +248,8 @@ def main():
     ap.add_argument("--no-minimization", action="store_true",
                     help="skip the packings/s minimisation measurement")
     ap.add_argument("--packings", type=int, default=256, help="minimisation ensemble size")
+    ap.add_argument("--no-quasistatic", action="store_true",
+                    help="skip the quasi-static compression measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -277,6 +329,8 @@ def main():
     if not args.no_minimization:
         out["minimization"] = minimization(args.packings, 1,
                                            0.0 if args.no_cpu_baseline else 15.0)
+    if not args.no_quasistatic:
+        out["quasistatic"] = quasistatic_compression()
     print(json.dumps(out), flush=True)
 
 

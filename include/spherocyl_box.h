@@ -23,6 +23,7 @@
 #define SPHEROCYLS_SPHEROCYL_BOX_H
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -106,6 +107,39 @@ class SpherocylBox {
   // bThrowIfNotConverged.
   int fire_minimize(const double* pdParams = nullptr, bool bThrowIfNotConverged = false);
   void set_precision(bool bMixed);
+  // SC_CONTACT_CLOSEST (the CPU reference's model, the default) or
+  // SC_CONTACT_TWO_POINT (src/cuda/2point_dynamics.cu:391-546).
+  void set_contact_model(int nModel);
+  // FIRE parameters (SC ABI order) of the quasi-static protocols below;
+  // nullptr restores the defaults.
+  void set_minimizer_params(const double* pdParams);
+
+  // Quasi-static protocols of the reference's GPU code
+  // (src/cuda/2point_dynamics.cu:4300-4606).  A step multiplies x, y and L by
+  // 1 - eps (compress) or 1 + eps (expand), as resize_step does
+  // (src/cuda/strain_dynamics.cu:1111-1128), then minimises to F_rms < ftol
+  // with FIRE (the reference runs its conjugate-gradient minimiser).  After
+  // each step: a line "t packing E Pxx Pyy Pxy Pyx" in the SE file (the
+  // reference's printf formats; the stress is the corrected virial with the
+  // CPU code's sign) and a position file sd%010lu.dat in the GPU code's text
+  // format, every step by default (set_qs_position_interval).  Each protocol
+  // runs with the box's contact model and returns the last step number.
+  //   compress_qs2p: minimise, then compress while packing < dMaxPacking.
+  //   expand_qs2p:   minimise, then expand while packing > dMinPacking.
+  //   find_jam_2p:   bracket the packing where the minimised energy crosses
+  //                  dJamE: compress (E < dJamE) or expand (E > dJamE) with
+  //                  steps growing by 1.1 until E crosses, then continue with
+  //                  step * 1.1^(steps - 1) / 2, until the step is below 1e-15;
+  //                  the final state also goes to sd%010lu.bin.
+  long unsigned int compress_qs2p(double dMaxPacking, double dResizeStep);
+  long unsigned int expand_qs2p(double dMinPacking, double dResizeStep);
+  long unsigned int find_jam_2p(double dJamE, double dResizeStep);
+  void set_qs_position_interval(int nSteps) { m_nQsPosEvery = nSteps; }
+  // iterations of every minimisation of the last quasi-static protocol
+  const std::vector<int>& qs_iterations() const { return m_vQsIters; }
+  // save_positions_bin / the GPU code's save_positions (sd%010lu.bin / .dat)
+  void save_positions_bin(unsigned long nTime);
+  void save_positions_sd(unsigned long nTime);
   void set_seed(uint64_t nSeed) { m_nSeed = nSeed; }
   void set_device(int nDevice) { m_nDevice = nDevice; }
   void set_resize_image(bool bCurrentLength) { m_bResizeCorrected = bCurrentLength; }
@@ -136,6 +170,12 @@ class SpherocylBox {
   void read_scalars();
   void write_se_line(std::ostream& out, unsigned long t, bool bPacking);
   double packing_from_arrays() const;
+  int qs_minimize();
+  void qs_step(double dEpsilon);
+  void qs_line(FILE* f, const char* fmt, unsigned long t);
+  long unsigned int qs_run(double dLimit, double dResizeStep, bool bCompress);
+  long unsigned int resize_to_energy_2p(FILE* f, long unsigned int nTime, double dEnergy,
+                                        double dStep);
 
   int m_nParticles = 0;
   double m_dLx = 0, m_dLy = 0;
@@ -156,6 +196,10 @@ class SpherocylBox {
   int m_nPrecision = SC_PREC_FP64;
   uint64_t m_nSeed = 1;
   bool m_bResizeCorrected = false;
+  int m_nContactModel = SC_CONTACT_CLOSEST;
+  double m_dFire[10] = {0.05, 0.5, 5, 1.1, 0.5, 0.1, 0.99, 0.02, 1e-10, 200000};
+  int m_nQsPosEvery = 1;
+  std::vector<int> m_vQsIters;
   bool m_bGeomStale = true;   // radii, lengths, box or padding changed: rebuild the system
   bool m_bDevStale = true;    // host positions newer than the device
   bool m_bHostStale = false;  // device positions newer than the host mirror

@@ -62,7 +62,10 @@ extern "C" {
 #define SC_BOX_COLS 16
 #define SC_BOX_PAD 17
 #define SC_BOX_ITERS 18 /* minimiser iterations of the last sc_fire_minimize */
-#define SC_BOX_NFIELDS 19
+#define SC_BOX_FIRE_DT 19    /* FIRE time step and mixing factor at the end of the */
+#define SC_BOX_FIRE_ALPHA 20 /*   last sc_fire_minimize                            */
+#define SC_BOX_PAIR_FP32 21  /* 1 while this packing's pair math runs in FP32      */
+#define SC_BOX_NFIELDS 22
 
 /* sc_strain_steps / sc_resize_steps record layout: per recorded sub-step and
    packing, SC_SE_NFIELDS doubles: E Pxx Pyy Pxy (reference formulas),

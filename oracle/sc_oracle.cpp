@@ -209,6 +209,18 @@ void crossed_contact(double ddx, double ddy, PairOut& o) {
   o.dy = -delta * uy;
 }
 
+// Whether the segments intersect: each one's endpoints lie strictly on opposite
+// sides of the other's line.  The closest-approach test of crossing (|tu| < 1,
+// |s| < 1) also fires on rounding noise for nearly parallel segments (det =
+// ac - b^2 cancels); a crossing is taken only when this confirms it.
+bool segments_cross(double ddx, double ddy, double nxA, double nyA, double nxB, double nyB) {
+  const double hp = nxB * (ddy + nyA) - nyB * (ddx + nxA);
+  const double hm = nxB * (ddy - nyA) - nyB * (ddx - nxA);
+  const double gp = nxA * (nyB - ddy) - nyA * (nxB - ddx);
+  const double gm = nxA * (-nyB - ddy) - nyA * (-nxB - ddx);
+  return ((hp > 0 && hm < 0) || (hp < 0 && hm > 0)) && ((gp > 0 && gm < 0) || (gp < 0 && gm > 0));
+}
+
 // calc_force_pair / calc_temp_force_pair / calc_se_force_pair geometry,
 // strain_dynamics.cpp:17-75 (operation order kept for rounding agreement).
 inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap, double xq,
@@ -243,7 +255,8 @@ inline PairOut pair_eval(double xp, double yp, double php, double Rp, double Ap,
   o.dy = ddy + s * o.nyA - t * o.nyB;
   double d2 = o.dx * o.dx + o.dy * o.dy;
   o.contact = d2 < o.sigma * o.sigma;
-  if (o.contact && det > 0 && std::fabs(tu) < 1 && sabs < 1) {  // crossed segments
+  if (o.contact && det > 0 && std::fabs(tu) < 1 && sabs < 1 &&
+      segments_cross(ddx, ddy, o.nxA, o.nyA, o.nxB, o.nyB)) {  // crossed segments
     crossed_contact(ddx, ddy, o);
     return o;
   }
@@ -344,7 +357,7 @@ inline int pair_eval_2p(double xp, double yp, double php, double Rp, double Ap, 
     s1 = -(b * t1 + d) / a;
     double sarg = std::fabs(s1);
     s1 = clamp1(s1);
-    crossed = std::fabs(tu) < 1 && sarg < 1;
+    crossed = std::fabs(tu) < 1 && sarg < 1 && segments_cross(ddx, ddy, o.nxA, o.nyA, o.nxB, o.nyB);
     if (sarg >= 1) {
       t1 = clamp1(-(b * s1 + e) / c);
       s2 = -s1;

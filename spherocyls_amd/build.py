@@ -37,9 +37,11 @@ HIP_SOURCES = ["sc_device.hip", "sc_list.hip", "sc_api.hip", "sc_mpnn.hip"]
 CXX_SOURCES = [os.path.join(CSRC, "sc_configgen.cpp")]
 HOST_LIB_SOURCES = ["spherocyl_box.cpp", "dat_file.cpp"]
 DRIVERS = {
-    "spherocyl_strain": "main_strain.cpp",
-    "spherocyl_resize": "main_resize.cpp",
-    "spherocyl_minimize": "main_relax.cpp",
+    "spherocyl_strain": ("main_strain.cpp", []),
+    "spherocyl_resize": ("main_resize.cpp", []),
+    "spherocyl_minimize": ("main_relax.cpp", []),
+    "spherocyl_qscompress": ("main_qs2p.cpp", []),
+    "spherocyl_findjam": ("main_qs2p.cpp", ["-DSPHEROCYL_FIND_JAM"]),
 }
 
 
@@ -67,11 +69,11 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     os.makedirs(BIN, exist_ok=True)
-    for exe, src in DRIVERS.items():
+    for exe, (src, defs) in DRIVERS.items():
         path = os.path.join(HOST, src)
         if not os.path.exists(path):
             continue
-        _run(["g++", *CXX_FLAGS, path, "-o", os.path.join(BIN, exe), f"-L{PKG}", "-lspherocyls",
+        _run(["g++", *CXX_FLAGS, *defs, path, "-o", os.path.join(BIN, exe), f"-L{PKG}", "-lspherocyls",
               f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."], verbose)
     return LIB
 

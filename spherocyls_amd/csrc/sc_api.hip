@@ -18,7 +18,7 @@ namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
 template <int MODE, bool TWO_POINT>
 __global__ void k_tile(TileArgs a);
-template <int MODE, int LANES, bool TWO_POINT>
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED>
 __global__ void k_list(ListArgs a);
 __global__ void k_reduce(ReduceArgs a);
 __global__ void k_segvec(Bank b, int ntot);
@@ -535,18 +535,23 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.corrected_image = ps.corrected_image;
     a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     using Fn = void (*)(ListArgs);
-#define SC_LIST_ROW(M, T) \
-  {&k_list<M, 1, T>, &k_list<M, 2, T>, &k_list<M, 4, T>, &k_list<M, 8, T>}
-    static const Fn table[2][3][4] = {
-        {SC_LIST_ROW(kModeForces, false), SC_LIST_ROW(kModePred, false),
-         SC_LIST_ROW(kModeCorr, false)},
-        {SC_LIST_ROW(kModeForces, true), SC_LIST_ROW(kModePred, true),
-         SC_LIST_ROW(kModeCorr, true)}};
+#define SC_LIST_ROW(M, T, X) \
+  {&k_list<M, 1, T, X>, &k_list<M, 2, T, X>, &k_list<M, 4, T, X>, &k_list<M, 8, T, X>}
+#define SC_LIST_MODES(T, X)                                            \
+  {SC_LIST_ROW(kModeForces, T, X), SC_LIST_ROW(kModePred, T, X), \
+   SC_LIST_ROW(kModeCorr, T, X)}
+    // [pair precision: FP64 only / mixed][contact model][mode][lanes]
+    static const Fn table[2][2][3][4] = {
+        {SC_LIST_MODES(false, false), SC_LIST_MODES(true, false)},
+        {SC_LIST_MODES(false, true), SC_LIST_MODES(true, true)}};
+#undef SC_LIST_MODES
 #undef SC_LIST_ROW
     const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
-    Fn fn = table[a.two_point ? 1 : 0][ps.mode][li];
+    const int xi = s->prec == SC_PREC_MIXED ? 1 : 0;
+    Fn fn = table[xi][a.two_point ? 1 : 0][ps.mode][li];
     KernelTimer t(s, ps.kernel_id);
-    hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block), 0, s->stream, a);
+    hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block),
+                       sizeof(int) * kHitCap * s->list_block, s->stream, a);
     s->kseq++;
   }
 }
@@ -1065,6 +1070,9 @@ int sc_get_box(sc_system* s, int k, double* out) {
   out[SC_BOX_COLS] = P.cols;
   out[SC_BOX_PAD] = P.pad;
   out[SC_BOX_ITERS] = S.iters;
+  out[SC_BOX_FIRE_DT] = S.fire_dt;
+  out[SC_BOX_FIRE_ALPHA] = S.fire_alpha;
+  out[SC_BOX_PAIR_FP32] = S.prec == kFP32 ? 1.0 : 0.0;
   return SC_OK;
   API_END
 }

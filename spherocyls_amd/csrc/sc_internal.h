@@ -116,6 +116,10 @@ struct TileArgs {
 // chosen with a skin; a force kernel rebuilds the lists of a packing (each
 // thread its own row) when the previous kernel raised the packing's flag.
 constexpr int kNbrCap = 64;
+// Partners a list-kernel thread holds between its prefilter and pair-math
+// phases (a longer run is processed in pieces); the kernel's dynamic LDS is
+// kHitCap ints per thread.
+constexpr int kHitCap = 16;
 
 struct ListArgs {
   const PackInfo* packs;

@@ -150,16 +150,16 @@ __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, doub
 // lane has several independent gathers in flight.  The LANES partial sums are
 // combined with xor shuffles (fixed order for a given LANES).
 constexpr int kGather = 4;
-// Compacted partners held per thread between the two phases (a longer run is
-// processed in pieces), and the largest workgroup.
-constexpr int kHitCap = 16;
-constexpr int kListBlock = 256;
 
-template <int MODE, int LANES, bool TWO_POINT>
+// MIXED: the kernel carries the FP32-force pair path besides the FP64 one and
+// picks per packing (a mixed-mode FIRE moves packings to FP64 one by one).
+// FP64 systems launch the MIXED = false kernel, whose single pair path needs
+// fewer registers: 4 wavefronts per SIMD instead of 3 for the Heun passes.
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED>
 __global__ void __launch_bounds__(256) k_list(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double s_red[4 * kNSum];
-  __shared__ int s_hit[kHitCap * kListBlock];  // [slot][thread] partner columns
+  extern __shared__ int s_hit[];  // [slot][thread] partner columns, kHitCap x blockDim
   const int tid = threadIdx.x, nthr = blockDim.x;
   if (a.cnt_clear)
     for (int i = blockIdx.x * nthr + tid; i < a.ncells_total; i += gridDim.x * nthr)
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       Ly = L_new_y;
     }
   }
-  const bool fp64 = S.prec == kFP64;
+  const bool fp64 = !MIXED || S.prec == kFP64;
 
   double acc[kSums ? kNSum : 1];
 #pragma unroll
@@ -267,12 +267,12 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
           dx += gam * dy;
           const double reach = reach_i + Sq[j];
           if (!(dx * dx + dy * dy < reach * reach * (1.0 + 1e-10))) continue;
-          s_hit[nh * kListBlock + tid] = qs[j];
+          s_hit[nh * nthr + tid] = qs[j];
           ++nh;
         }
       }
       for (int h = 0; h < nh; ++h) {
-        const int q = s_hit[h * kListBlock + tid];
+        const int q = s_hit[h * nthr + tid];
         pair_into<kSums, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, a.ex[q], a.ey[q], a.enx[q],
                                     a.eny[q], a.b.R[q], a.b.A[q], a.b.id[q], Lx, Ly, gam, fp64,
                                     do_se, Fx, Fy, Tq, acc);
@@ -441,18 +441,20 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   }
 }
 
-#define SC_LIST_INST(L, T)                                       \
-  template __global__ void k_list<kModeForces, L, T>(ListArgs); \
-  template __global__ void k_list<kModePred, L, T>(ListArgs);   \
-  template __global__ void k_list<kModeCorr, L, T>(ListArgs);
-SC_LIST_INST(1, false)
-SC_LIST_INST(2, false)
-SC_LIST_INST(4, false)
-SC_LIST_INST(8, false)
-SC_LIST_INST(1, true)
-SC_LIST_INST(2, true)
-SC_LIST_INST(4, true)
-SC_LIST_INST(8, true)
+#define SC_LIST_INST(L, T, X)                                       \
+  template __global__ void k_list<kModeForces, L, T, X>(ListArgs); \
+  template __global__ void k_list<kModePred, L, T, X>(ListArgs);   \
+  template __global__ void k_list<kModeCorr, L, T, X>(ListArgs);
+#define SC_LIST_INST_LANES(T, X) \
+  SC_LIST_INST(1, T, X)          \
+  SC_LIST_INST(2, T, X)          \
+  SC_LIST_INST(4, T, X)          \
+  SC_LIST_INST(8, T, X)
+SC_LIST_INST_LANES(false, false)
+SC_LIST_INST_LANES(true, false)
+SC_LIST_INST_LANES(false, true)
+SC_LIST_INST_LANES(true, true)
+#undef SC_LIST_INST_LANES
 #undef SC_LIST_INST
 
 // (nx, ny) = A (cos phi, sin phi) for every particle of the bank.

@@ -69,6 +69,24 @@ __device__ __forceinline__ void crossed_contact(T ddx, T ddy, T nxA, T nyA, T nx
   o.dV = dV;
 }
 
+// Whether two segments intersect: each one's endpoints lie strictly on
+// opposite sides of the other's line (cross products, no normalisation).  The
+// closest-approach test of crossing (|tu| < 1 and |s| < 1) also fires on noise
+// for nearly parallel segments, whose det = a c - b^2 loses its digits to
+// cancellation (in FP32 from angles of ~1e-3 down); a crossing is taken only
+// when this test confirms it.  For such pairs the endpoints sit a contact
+// distance from the other line, far above the rounding of these products.
+template <typename T>
+__device__ __forceinline__ bool segments_cross(T ddx, T ddy, T nxA, T nyA, T nxB, T nyB) {
+#pragma clang fp contract(off)
+  const T hp = nxB * (ddy + nyA) - nyB * (ddx + nxA);
+  const T hm = nxB * (ddy - nyA) - nyB * (ddx - nxA);
+  const T gp = nxA * (nyB - ddy) - nyA * (nxB - ddx);
+  const T gm = nxA * (-nyB - ddy) - nyA * (-nxB - ddx);
+  return ((hp > T(0) && hm < T(0)) || (hp < T(0) && hm > T(0))) &&
+         ((gp > T(0) && gm < T(0)) || (gp < T(0) && gm > T(0)));
+}
+
 // Segment-segment closest approach and harmonic contact force, following
 // calc_force_pair (src/cpp/strain_dynamics.cpp:38-64) term by term.  Returns
 // false when |D| >= sigma (or D is NaN).  FP64: no contraction, so the rounding
@@ -97,7 +115,8 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   o.t = t;
   o.dx = Dx;
   o.dy = Dy;
-  if (det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0) {
+  if (det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0 &&
+      segments_cross(ddx, ddy, nxA, nyA, nxB, nyB)) {
     crossed_contact(ddx, ddy, nxA, nyA, nxB, nyB, sig, o);  // crossed segments
     return true;
   }
@@ -111,54 +130,6 @@ __device__ __forceinline__ bool pair_contact_f64(double ddx, double ddy, double 
   double dV = (1.0 - dist / sig) / sig;
   double fx = Dx * dV / dist;
   double fy = Dy * dV / dist;
-  o.fx = fx;
-  o.fy = fy;
-  o.tp = s * nxA * fy - s * nyA * fx;
-  o.tq = t * nyB * fx - t * nxB * fy;
-  o.dist = dist;
-  o.dV = dV;
-  return true;
-}
-
-// The same closest-approach force in FP32 (mixed mode), contraction allowed.
-__device__ __forceinline__ bool pair_contact_f32(float ddx, float ddy, float nxA, float nyA,
-                                                 float nxB, float nyB, float Ap, float Aq,
-                                                 float sig, PairOut<float>& o) {
-#pragma clang fp contract(fast)
-  float a = Ap * Ap;
-  float b = -(nxA * nxB + nyA * nyB);
-  float c = Aq * Aq;
-  float d = nxA * ddx + nyA * ddy;
-  float e = -nxB * ddx - nyB * ddy;
-  float det = a * c - b * b;
-  float tu = (b * d - a * e) / det;
-  float t = fminf(fmaxf(tu, -1.0f), 1.0f);
-  float s = -(b * t + d) / a;
-  float sabs = fabsf(s);
-  s = fminf(fmaxf(s, -1.0f), 1.0f);
-  if (sabs > 1.0f) t = fminf(fmaxf(-(b * s + e) / c, -1.0f), 1.0f);
-  float Dx = ddx + s * nxA - t * nxB;
-  float Dy = ddy + s * nyA - t * nyB;
-  float d2 = Dx * Dx + Dy * Dy;
-  if (!(d2 < sig * sig)) return false;
-  o.s = s;
-  o.t = t;
-  o.dx = Dx;
-  o.dy = Dy;
-  if (det > 0.0f && fabsf(tu) < 1.0f && sabs < 1.0f) {  // crossed segments
-    crossed_contact(ddx, ddy, nxA, nyA, nxB, nyB, sig, o);
-    return true;
-  }
-  if (d2 == 0.0f) {
-    o.fx = o.fy = o.tp = o.tq = 0.0f;
-    o.dist = 0.0f;
-    o.dV = 1.0f / sig;
-    return true;
-  }
-  float dist = sqrtf(d2);
-  float dV = (1.0f - dist / sig) / sig;
-  float fx = Dx * dV / dist;
-  float fy = Dy * dV / dist;
   o.fx = fx;
   o.fy = fy;
   o.tp = s * nxA * fy - s * nyA * fx;
@@ -205,6 +176,36 @@ __device__ __forceinline__ bool contact_point(T Dx, T Dy, T s, T t, T nxA, T nyA
   return true;
 }
 
+// Mixed precision (FP32 pair forces).  The closest-approach parameters s, t
+// and the crossing decision stay FP64, in pair_contact_f64's order: they come
+// from differences of nearly equal products (det = a c - b^2, b d - a e), which
+// in FP32 are noise for nearly parallel rods -- a noise tu in (-1, 1) would
+// put the contact anywhere along the overlap, or call the pair crossed.  The
+// contact vector D is formed in FP64 as well; its length, the overlap force
+// and the torques are FP32.
+__device__ __forceinline__ bool pair_contact_mixed(double ddx, double ddy, double nxA, double nyA,
+                                                   double nxB, double nyB, double Ap, double Aq,
+                                                   double sig, PairOut<float>& o) {
+#pragma clang fp contract(off)
+  double a = Ap * Ap;
+  double b = -(nxA * nxB + nyA * nyB);
+  double c = Aq * Aq;
+  double d = nxA * ddx + nyA * ddy;
+  double e = -nxB * ddx - nyB * ddy;
+  double det = a * c - b * b;
+  double tu = (b * d - a * e) / det;
+  double t = fmin(fmax(tu, -1.0), 1.0);
+  double s = -(b * t + d) / a;
+  double sabs = fabs(s);
+  s = fmin(fmax(s, -1.0), 1.0);
+  if (sabs > 1.0) t = fmin(fmax(-(b * s + e) / c, -1.0), 1.0);
+  const bool crossed = det > 0.0 && fabs(tu) < 1.0 && sabs < 1.0 &&
+                       segments_cross(ddx, ddy, nxA, nyA, nxB, nyB);
+  return contact_point<float>((float)(ddx + s * nxA - t * nxB), (float)(ddy + s * nyA - t * nyB),
+                              (float)s, (float)t, (float)nxA, (float)nyA, (float)nxB, (float)nyB,
+                              (float)sig, crossed, (float)ddx, (float)ddy, o);
+}
+
 template <typename T>
 __device__ __forceinline__ T clamp1(T v) {
   return fmin(fmax(v, T(-1)), T(1));
@@ -218,79 +219,83 @@ __device__ __forceinline__ T clamp1(T v) {
 // segment fixed the first (s2 = -s1 or t2 = -t1) projected onto the other
 // segment; a third candidate from the other segment's matching end replaces it
 // when it is closer.  o0 / h0: the closest-point contact; o1 / h1: the second.
+// The candidate parameters, every choice between them and the contact vectors
+// are FP64; T is the type of each contact's force evaluation (FP32 in mixed
+// mode, see pair_contact_mixed).
 template <typename T>
-__device__ __forceinline__ void pair_contacts_2p(T ddx, T ddy, T nxA, T nyA, T nxB, T nyB, T Ap,
-                                                 T Aq, T sig, PairOut<T>& o0, bool& h0,
+__device__ __forceinline__ void pair_contacts_2p(double ddx, double ddy, double nxA, double nyA,
+                                                 double nxB, double nyB, double Ap, double Aq,
+                                                 double sig, PairOut<T>& o0, bool& h0,
                                                  PairOut<T>& o1, bool& h1) {
 #pragma clang fp contract(off)
-  const T a = Ap * Ap;
-  const T b = -(nxA * nxB + nyA * nyB);
-  const T c = Aq * Aq;
-  const T d = nxA * ddx + nyA * ddy;
-  const T e = -nxB * ddx - nyB * ddy;
-  const T det = a * c - b * b;
-  T s1, t1, s2, t2, s3, t3;
+  const double a = Ap * Ap;
+  const double b = -(nxA * nxB + nyA * nyB);
+  const double c = Aq * Aq;
+  const double d = nxA * ddx + nyA * ddy;
+  const double e = -nxB * ddx - nyB * ddy;
+  const double det = a * c - b * b;
+  double s1, t1, s2, t2, s3, t3;
   bool crossed = false;
-  if (det <= T(0)) {  // parallel: the two ends of the overlap
+  if (det <= 0.0) {  // parallel: the two ends of the overlap
     s1 = clamp1(-(b + d) / a);
     s2 = clamp1(-(d - b) / a);
-    if (fabs(s1) == T(1)) t1 = clamp1(-(b * s1 + e) / c);
-    else t1 = b < T(0) ? clamp1(-(b + e) / c) : clamp1(-(e - b) / c);
-    if (fabs(s2) == T(1)) t2 = clamp1(-(b * s2 + e) / c);
-    else t2 = b < T(0) ? clamp1(-(e - b) / c) : clamp1(-(b + e) / c);
+    if (fabs(s1) == 1.0) t1 = clamp1(-(b * s1 + e) / c);
+    else t1 = b < 0.0 ? clamp1(-(b + e) / c) : clamp1(-(e - b) / c);
+    if (fabs(s2) == 1.0) t2 = clamp1(-(b * s2 + e) / c);
+    else t2 = b < 0.0 ? clamp1(-(e - b) / c) : clamp1(-(b + e) / c);
     s3 = s2;
     t3 = t2;
   } else {
-    const T tu = (b * d - a * e) / det;
+    const double tu = (b * d - a * e) / det;
     t1 = clamp1(tu);
     s1 = -(b * t1 + d) / a;
-    T sarg = fabs(s1);
+    double sarg = fabs(s1);
     s1 = clamp1(s1);
-    crossed = fabs(tu) < T(1) && sarg < T(1);
-    if (sarg >= T(1)) {
+    crossed = fabs(tu) < 1.0 && sarg < 1.0 && segments_cross(ddx, ddy, nxA, nyA, nxB, nyB);
+    if (sarg >= 1.0) {
       t1 = clamp1(-(b * s1 + e) / c);
       s2 = -s1;
       t2 = -(b * s2 + e) / c;
-      T targ = fabs(t2);
+      double targ = fabs(t2);
       t2 = clamp1(t2);
-      if (targ > T(1)) {
+      if (targ > 1.0) {
         s2 = clamp1(-(b * t2 + d) / a);
         s3 = s2;
         t3 = t2;
       } else {
-        t3 = b < T(0) ? -s1 : s1;
+        t3 = b < 0.0 ? -s1 : s1;
         s3 = -(b * t3 + d) / a;
         sarg = fabs(s3);
         s3 = clamp1(s3);
-        if (sarg >= T(1)) t3 = clamp1(-(b * s3 + e) / c);
+        if (sarg >= 1.0) t3 = clamp1(-(b * s3 + e) / c);
       }
     } else {
       t2 = -t1;
       s2 = -(b * t2 + d) / a;
       sarg = fabs(s2);
       s2 = clamp1(s2);
-      if (sarg > T(1)) {
+      if (sarg > 1.0) {
         t2 = clamp1(-(b * s2 + e) / c);
         s3 = s2;
         t3 = t2;
       } else {
-        s3 = b < T(0) ? -t1 : t1;
+        s3 = b < 0.0 ? -t1 : t1;
         t3 = -(b * s3 + e) / c;
-        T targ = fabs(t3);
+        double targ = fabs(t3);
         t3 = clamp1(t3);
-        if (targ >= T(1)) s3 = clamp1(-(b * t3 + d) / a);
+        if (targ >= 1.0) s3 = clamp1(-(b * t3 + d) / a);
       }
     }
   }
-  h0 = contact_point(ddx + s1 * nxA - t1 * nxB, ddy + s1 * nyA - t1 * nyB, s1, t1, nxA, nyA, nxB,
-                     nyB, sig, crossed, ddx, ddy, o0);
+  h0 = contact_point<T>(T(ddx + s1 * nxA - t1 * nxB), T(ddy + s1 * nyA - t1 * nyB), T(s1), T(t1),
+                        T(nxA), T(nyA), T(nxB), T(nyB), T(sig), crossed, T(ddx), T(ddy), o0);
   h1 = false;
   if (s1 != s2 && t1 != t2) {
-    T Dx2 = ddx + s2 * nxA - t2 * nxB;
-    T Dy2 = ddy + s2 * nyA - t2 * nyB;
+    double Dx2 = ddx + s2 * nxA - t2 * nxB;
+    double Dy2 = ddy + s2 * nyA - t2 * nyB;
     if (s3 != s1 && s3 != s2) {
-      T Dx3 = ddx + s3 * nxA - t3 * nxB;
-      T Dy3 = ddy + s3 * nyA - t3 * nyB;
+      const double Dx3 = ddx + s3 * nxA - t3 * nxB;
+      const double Dy3 = ddy + s3 * nyA - t3 * nyB;
       if (Dx3 * Dx3 + Dy3 * Dy3 < Dx2 * Dx2 + Dy2 * Dy2) {
         s2 = s3;
         t2 = t3;
@@ -298,7 +303,8 @@ __device__ __forceinline__ void pair_contacts_2p(T ddx, T ddy, T nxA, T nyA, T n
         Dy2 = Dy3;
       }
     }
-    h1 = contact_point(Dx2, Dy2, s2, t2, nxA, nyA, nxB, nyB, sig, false, ddx, ddy, o1);
+    h1 = contact_point<T>(T(Dx2), T(Dy2), T(s2), T(t2), T(nxA), T(nyA), T(nxB), T(nyB), T(sig),
+                          false, T(ddx), T(ddy), o1);
   }
 }
 
@@ -310,8 +316,8 @@ __device__ __forceinline__ void widen(const PairOut<T>& a, PairOut<double>& o) {
 
 // Contacts of one pair in the orientation given (A = lower id): the
 // closest-point model of the CPU reference, or (TWO_POINT) the two-point model;
-// FP64 or FP32 pair math, FP64 results.  o0 / h0: closest-point contact;
-// o1 / h1: the second contact of the two-point model.
+// FP64 or mixed (FP32 force evaluation) pair math, FP64 results.  o0 / h0:
+// closest-point contact; o1 / h1: the second contact of the two-point model.
 template <bool TWO_POINT>
 __device__ __forceinline__ void pair_contacts(double ddx, double ddy, double nxA, double nyA,
                                               double nxB, double nyB, double Ap, double Aq,
@@ -323,8 +329,7 @@ __device__ __forceinline__ void pair_contacts(double ddx, double ddy, double nxA
       h0 = pair_contact_f64(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, o0);
     } else {
       PairOut<float> f0;
-      h0 = pair_contact_f32((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
-                            (float)nyB, (float)Ap, (float)Aq, (float)sig, f0);
+      h0 = pair_contact_mixed(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, f0);
       if (h0) widen(f0, o0);
     }
     return;
@@ -333,8 +338,7 @@ __device__ __forceinline__ void pair_contacts(double ddx, double ddy, double nxA
     pair_contacts_2p<double>(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, o0, h0, o1, h1);
   } else {
     PairOut<float> f0, f1;
-    pair_contacts_2p<float>((float)ddx, (float)ddy, (float)nxA, (float)nyA, (float)nxB,
-                            (float)nyB, (float)Ap, (float)Aq, (float)sig, f0, h0, f1, h1);
+    pair_contacts_2p<float>(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aq, sig, f0, h0, f1, h1);
     if (h0) widen(f0, o0);
     if (h1) widen(f1, o1);
   }

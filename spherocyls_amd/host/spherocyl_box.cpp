@@ -178,6 +178,18 @@ void SpherocylBox::set_precision(bool bMixed) {
   if (m_sys) check(sc_set_precision(m_sys, m_nPrecision), "set_precision");
 }
 
+void SpherocylBox::set_contact_model(int nModel) {
+  if (nModel != SC_CONTACT_CLOSEST && nModel != SC_CONTACT_TWO_POINT)
+    throw SpherocylError(SC_ERR_ARG, "set_contact_model: unknown model");
+  m_nContactModel = nModel;
+  if (m_sys) check(sc_set_contact_model(m_sys, m_nContactModel), "set_contact_model");
+}
+
+void SpherocylBox::set_minimizer_params(const double* pdParams) {
+  static const double kDefaults[10] = {0.05, 0.5, 5, 1.1, 0.5, 0.1, 0.99, 0.02, 1e-10, 200000};
+  std::copy(pdParams ? pdParams : kDefaults, (pdParams ? pdParams : kDefaults) + 10, m_dFire);
+}
+
 // ------------------------------------------------------------ device state
 
 void SpherocylBox::ensure_device() {
@@ -203,6 +215,7 @@ void SpherocylBox::ensure_device() {
     }
     m_sys = s;
     check(sc_set_precision(m_sys, m_nPrecision), "set_precision");
+    check(sc_set_contact_model(m_sys, m_nContactModel), "set_contact_model");
     m_bGeomStale = false;
     m_bDevStale = true;
   }
@@ -765,6 +778,142 @@ int SpherocylBox::fire_minimize(const double* pdParams, bool bThrowIfNotConverge
   return iters;
 }
 
+// ------------------------------------------------------------ quasi-static protocols
+
+void SpherocylBox::save_positions_sd(unsigned long nTime) {
+  sync_host();
+  std::string path = m_strOutputDir + "/" + time_name("sd", nTime);
+  if (sc_write_positions_sd(path.c_str(), m_nParticles, m_dLx, m_dLy, m_dPacking, m_dGamma,
+                            m_dGammaTotal, m_dStrainRate, m_dStep, m_vX.data(), m_vY.data(),
+                            m_vPhi.data(), m_vR.data(), m_vA.data()) != SC_OK)
+    throw SpherocylError(SC_ERR_ARG, "save_positions_sd: cannot write " + path);
+}
+
+void SpherocylBox::save_positions_bin(unsigned long nTime) {
+  sync_host();
+  char name[32];
+  std::snprintf(name, sizeof(name), "sd%010lu.bin", nTime);
+  std::string path = m_strOutputDir + "/" + name;
+  if (sc_write_positions_bin(path.c_str(), m_nParticles, m_dLx, m_dLy, m_dPacking, m_dGamma,
+                             m_dGammaTotal, m_dStrainRate, m_dStep, m_vX.data(), m_vY.data(),
+                             m_vPhi.data(), m_vR.data(), m_vA.data()) != SC_OK)
+    throw SpherocylError(SC_ERR_ARG, "save_positions_bin: cannot write " + path);
+}
+
+// FIRE to the box's tolerance; energy, stress, box length and packing
+// fraction of the minimised state.  A minimisation that stops at max_iter
+// is kept (the reference's minimiser also stops at its step limit) and shows
+// in qs_iterations.
+int SpherocylBox::qs_minimize() {
+  int it = fire_minimize(m_dFire, false);
+  m_dPacking = packing_from_arrays();
+  m_vQsIters.push_back(it);
+  return it;
+}
+
+// x, y, L *= 1 + eps (src/cuda/strain_dynamics.cu:1111-1128), then minimise.
+void SpherocylBox::qs_step(double dEpsilon) {
+  ensure_device();
+  const double scale = 1.0 + dEpsilon;
+  check(sc_affine_resize(m_sys, &scale), "affine_resize");
+  m_bHostStale = true;
+  read_scalars();
+  qs_minimize();
+}
+
+void SpherocylBox::qs_line(FILE* f, const char* fmt, unsigned long t) {
+  std::fprintf(f, fmt, (int)t, m_dPacking, m_dEnergy, m_dCorr[0], m_dCorr[1], m_dCorr[2],
+               m_dCorr[3]);
+  std::fflush(f);
+}
+
+// compress_qs2p / expand_qs2p (src/cuda/2point_dynamics.cu:4300-4390)
+long unsigned int SpherocylBox::qs_run(double dLimit, double dResizeStep, bool bCompress) {
+  if (dResizeStep == 0 || !std::isfinite(dResizeStep))
+    throw SpherocylError(SC_ERR_ARG, "quasi-static resize step must be nonzero");
+  if (m_nQsPosEvery < 1) throw SpherocylError(SC_ERR_ARG, "position interval < 1");
+  dResizeStep = std::fabs(dResizeStep);
+  ensure_device();
+  m_dPacking = packing_from_arrays();
+  if (bCompress ? !(dLimit >= m_dPacking) : !(dLimit <= m_dPacking))
+    throw SpherocylError(SC_ERR_ARG, "quasi-static target packing is on the wrong side");
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  FILE* f = std::fopen(sePath.c_str(), "w");
+  if (!f) throw SpherocylError(SC_ERR_ARG, "quasi-static protocol: cannot open " + sePath);
+  m_vQsIters.clear();
+  unsigned long t = 0;
+  try {
+    qs_minimize();
+    save_positions_sd(t);
+    qs_line(f, "%d %f %g %g %g %g %g\n", t);
+    while (bCompress ? m_dPacking < dLimit : m_dPacking > dLimit) {
+      ++t;
+      qs_step(bCompress ? -dResizeStep : dResizeStep);
+      if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
+      qs_line(f, "%d %.9f %g %g %g %g %g\n", t);
+    }
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  return t;
+}
+
+long unsigned int SpherocylBox::compress_qs2p(double dMaxPacking, double dResizeStep) {
+  return qs_run(dMaxPacking, dResizeStep, true);
+}
+
+long unsigned int SpherocylBox::expand_qs2p(double dMinPacking, double dResizeStep) {
+  return qs_run(dMinPacking, dResizeStep, false);
+}
+
+// resize_to_energy_2p (src/cuda/2point_dynamics.cu:4393-4450): the energy
+// is that of the current state (the reference's calc_fe pass).
+long unsigned int SpherocylBox::resize_to_energy_2p(FILE* f, long unsigned int nTime,
+                                                     double dEnergy, double dStep) {
+  dStep = std::fabs(dStep);
+  calc_se();
+  const bool compress = m_dEnergy < dEnergy;
+  while (compress ? m_dEnergy < dEnergy : m_dEnergy > dEnergy) {
+    ++nTime;
+    qs_step(compress ? -dStep : dStep);
+    if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
+    qs_line(f, "%d %.10f %.9g %.9g %.9g %.9g %.9g\n", nTime);
+    dStep *= 1.1;
+  }
+  return nTime;
+}
+
+// find_jam_2p (src/cuda/2point_dynamics.cu:4582-4606)
+long unsigned int SpherocylBox::find_jam_2p(double dJamE, double dResizeStep) {
+  if (!(dResizeStep > 0) || !std::isfinite(dResizeStep) || !(dJamE >= 0))
+    throw SpherocylError(SC_ERR_ARG, "find_jam_2p: need dJamE >= 0 and a positive step");
+  if (m_nQsPosEvery < 1) throw SpherocylError(SC_ERR_ARG, "position interval < 1");
+  ensure_device();
+  m_dPacking = packing_from_arrays();
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  FILE* f = std::fopen(sePath.c_str(), "w");
+  if (!f) throw SpherocylError(SC_ERR_ARG, "find_jam_2p: cannot open " + sePath);
+  m_vQsIters.clear();
+  unsigned long t = 0, told = 0;
+  try {
+    save_positions_sd(0);
+    const double dMinResizeStep = 1e-15;
+    while (dResizeStep > dMinResizeStep) {
+      t = resize_to_energy_2p(f, t, dJamE, dResizeStep);
+      dResizeStep = dResizeStep * std::pow(1.1, (double)((long)t - (long)told - 1)) / 2;
+      told = t;
+    }
+    save_positions_bin(t);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  return t;
+}
+
 // ------------------------------------------------------------ position files
 
 PositionFile read_position_file(const std::string& path) {
@@ -831,8 +980,14 @@ extern "C" int sc_read_positions(const char* path, int cap, int* n, double* head
   std::istringstream hs(head);
   int nh = 0;
   if (!(hs >> nh)) return SC_ERR_ARG;
-  for (int i = 0; i < 6; ++i)
-    if (!(hs >> header[i])) return SC_ERR_ARG;
+  std::vector<double> hv;
+  for (double v; hs >> v;) hv.push_back(v);
+  if (hv.size() == 7) {  // the GPU code's "N Lx Ly packing ..." (square boxes only)
+    if (hv[0] != hv[1]) return SC_ERR_ARG;
+    hv.erase(hv.begin() + 1);
+  }
+  if (hv.size() != 6) return SC_ERR_ARG;
+  for (int i = 0; i < 6; ++i) header[i] = hv[i];
   int rows = 0;
   std::string line;
   while (std::getline(in, line)) {
@@ -851,6 +1006,72 @@ extern "C" int sc_read_positions(const char* path, int cap, int* n, double* head
     ++rows;
   }
   *n = rows;
+  return SC_OK;
+}
+
+extern "C" int sc_write_positions_sd(const char* path, int n, double Lx, double Ly,
+                                     double packing, double gamma, double gamma_total,
+                                     double strain_rate, double step, const double* x,
+                                     const double* y, const double* phi, const double* R,
+                                     const double* A) {
+  if (!path || n < 0 || (n > 0 && (!x || !y || !phi || !R || !A))) return SC_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return SC_ERR_ARG;
+  bool ok = std::fprintf(f, "%d %.17g %.17g %f %.17g %.14g %g %g\n", n, Lx, Ly, packing, gamma,
+                         gamma_total, strain_rate, step) > 0;
+  for (int p = 0; ok && p < n; ++p)
+    ok = std::fprintf(f, "%.17g %.17g %.17g %.2g %.17g\n", x[p], y[p], phi[p], R[p], A[p]) > 0;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SC_OK : SC_ERR_ARG;
+}
+
+extern "C" int sc_write_positions_bin(const char* path, int n, double Lx, double Ly,
+                                      double packing, double gamma, double gamma_total,
+                                      double strain_rate, double step, const double* x,
+                                      const double* y, const double* phi, const double* R,
+                                      const double* A) {
+  if (!path || n < 0 || (n > 0 && (!x || !y || !phi || !R || !A))) return SC_ERR_ARG;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return SC_ERR_ARG;
+  const double head[7] = {Lx, Ly, packing, gamma, gamma_total, strain_rate, step};
+  bool ok = std::fwrite(&n, sizeof(int), 1, f) == 1 && std::fwrite(head, sizeof(double), 7, f) == 7;
+  for (int p = 0; ok && p < n; ++p) {
+    const double rec[5] = {x[p], y[p], phi[p], R[p], A[p]};
+    ok = std::fwrite(rec, sizeof(double), 5, f) == 5;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SC_OK : SC_ERR_ARG;
+}
+
+extern "C" int sc_read_positions_bin(const char* path, int cap, int* n, double* header, double* x,
+                                     double* y, double* phi, double* R, double* A) {
+  if (!path || !n || !header || cap < 0 || (cap > 0 && (!x || !y || !phi || !R || !A)))
+    return SC_ERR_ARG;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return SC_ERR_ARG;
+  int count = 0;
+  bool ok = std::fread(&count, sizeof(int), 1, f) == 1 && count >= 0 &&
+            std::fread(header, sizeof(double), 7, f) == 7;
+  if (ok) {  // the file must hold exactly count records
+    const long body = std::ftell(f);
+    ok = std::fseek(f, 0, SEEK_END) == 0 &&
+         std::ftell(f) == body + (long)sizeof(double) * 5 * (long)count &&
+         std::fseek(f, body, SEEK_SET) == 0;
+  }
+  for (int p = 0; ok && p < count && p < cap; ++p) {
+    double rec[5];
+    ok = std::fread(rec, sizeof(double), 5, f) == 5;
+    if (ok) {
+      x[p] = rec[0];
+      y[p] = rec[1];
+      phi[p] = rec[2];
+      R[p] = rec[3];
+      A[p] = rec[4];
+    }
+  }
+  std::fclose(f);
+  if (!ok) return SC_ERR_ARG;
+  *n = count;
   return SC_OK;
 }
 

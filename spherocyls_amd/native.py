@@ -95,6 +95,10 @@ def load():
                                 C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp],
                                C.c_int),
         "sc_read_positions": ([C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "sc_write_positions_sd": ([C.c_char_p, C.c_int] + [C.c_double] * 7 + [_dp] * 5, C.c_int),
+        "sc_write_positions_bin": ([C.c_char_p, C.c_int] + [C.c_double] * 7 + [_dp] * 5, C.c_int),
+        "sc_read_positions_bin": ([C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp],
+                                  C.c_int),
         "sc_format_se_line": ([C.c_char_p, C.c_int, C.c_ulong, C.c_int, _dp, C.c_int], C.c_int),
         "sc_mpnn_create": ([C.POINTER(sysp), C.c_int, C.c_ulonglong], C.c_int),
         "sc_mpnn_destroy": ([sysp], C.c_int),
@@ -185,6 +189,51 @@ def read_positions(path):
     return head, dict(zip(("x", "y", "phi", "R", "A"), (a[:n.value] for a in arrs)))
 
 
+SD_HEADER = ("Lx", "Ly", "packing", "gamma", "gamma_total", "strain_rate", "step")
+
+
+def write_positions_sd(path, Lx, Ly, packing, gamma, gamma_total, strain_rate, step, x, y, phi,
+                       R, A):
+    """The GPU code's text position file (sd%010lu.dat)."""
+    lib = load()
+    x, y, phi, R, A = (_f64(v) for v in (x, y, phi, R, A))
+    st = lib.sc_write_positions_sd(os.fsencode(path), x.shape[0], Lx, Ly, packing, gamma,
+                                   gamma_total, strain_rate, step, _p(x), _p(y), _p(phi), _p(R),
+                                   _p(A))
+    if st != 0:
+        raise SpherocylsError(st, f"cannot write {path}")
+
+
+def write_positions_bin(path, Lx, Ly, packing, gamma, gamma_total, strain_rate, step, x, y, phi,
+                        R, A):
+    """save_positions_bin file (sd%010lu.bin)."""
+    lib = load()
+    x, y, phi, R, A = (_f64(v) for v in (x, y, phi, R, A))
+    st = lib.sc_write_positions_bin(os.fsencode(path), x.shape[0], Lx, Ly, packing, gamma,
+                                    gamma_total, strain_rate, step, _p(x), _p(y), _p(phi), _p(R),
+                                    _p(A))
+    if st != 0:
+        raise SpherocylsError(st, f"cannot write {path}")
+
+
+def read_positions_bin(path):
+    """(header dict, dict of arrays x y phi R A) of a save_positions_bin file."""
+    lib = load()
+    n = C.c_int()
+    h = np.zeros(7)
+    st = lib.sc_read_positions_bin(os.fsencode(path), 0, C.byref(n), _p(h), None, None, None,
+                                   None, None)
+    if st != 0:
+        raise SpherocylsError(st, f"cannot read {path}")
+    arrs = [np.empty(max(n.value, 1)) for _ in range(5)]
+    st = lib.sc_read_positions_bin(os.fsencode(path), n.value, C.byref(n), _p(h),
+                                   *[_p(a) for a in arrs])
+    if st != 0:
+        raise SpherocylsError(st, f"cannot read {path}")
+    return dict(zip(SD_HEADER, h.tolist())), dict(zip(("x", "y", "phi", "R", "A"),
+                                                      (a[:n.value] for a in arrs)))
+
+
 def format_se_line(t, values, precision=6):
     """One stress/energy line as the reference's writers format it."""
     lib = load()
@@ -208,7 +257,8 @@ def fire_params(**kw):
 
 
 BOX_FIELDS = ("Lx", "Ly", "gamma", "gamma_total", "E", "Pxx", "Pyy", "Pxy", "cPxx", "cPyy",
-              "cPxy", "cPyx", "ncontacts", "frms", "packing", "rows", "cols", "pad", "iters")
+              "cPxy", "cPyx", "ncontacts", "frms", "packing", "rows", "cols", "pad", "iters",
+              "fire_dt", "fire_alpha", "pair_fp32")
 SE_FIELDS = ("E", "Pxx", "Pyy", "Pxy", "cPxx", "cPyy", "cPxy", "cPyx", "ncontacts")
 
 
@@ -235,6 +285,7 @@ class System:
         if st != 0:
             raise SpherocylsError(st, "sc_system_create")
         self.h = h
+        self.precision = "fp64"
 
     def close(self):
         if getattr(self, "h", None):
@@ -258,6 +309,7 @@ class System:
 
     def set_precision(self, mode):
         self._ck(self.lib.sc_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)))
+        self.precision = {0: "fp64", 1: "mixed"}.get(mode, mode)
 
     def set_contact_model(self, model):
         """"closest" (the CPU reference) or "two_point"."""

@@ -1,0 +1,76 @@
+"""Batched quasi-static compression: every packing of one system takes the
+compress_qs2p step of the reference's GPU code (src/cuda/2point_dynamics.cu:
+4300-4340) -- x, y and L multiplied by 1 - eps -- and is then minimised to
+F_rms < ftol, all packings in one batched FIRE.  A packing that has reached
+its final packing fraction takes scale 1 and, already minimised, costs its
+FIRE nothing.  The per-packing loop of SpherocylBox::compress_qs2p is the same
+protocol for one box, with its files.
+
+Optional learned warm start: after each affine step the network's proposal
+(damped so that no packing's energy rises, see warmstart.damped_warm_start)
+moves the particles before FIRE; FIRE judges the result as usual.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from . import native
+
+
+@dataclasses.dataclass
+class Curve:
+    """One row per step (row 0: the minimised start) and packing."""
+    packing: np.ndarray     # [steps+1, npack]
+    pressure: np.ndarray    # (cPxx + cPyy) / 2, corrected virial
+    energy: np.ndarray
+    iterations: np.ndarray  # FIRE iterations of each minimisation
+    frms: np.ndarray
+    seconds: float
+
+
+def _record(s):
+    b = [s.box(k) for k in range(s.npack)]
+    return (np.array([x["packing"] for x in b]), np.array([0.5 * (x["cPxx"] + x["cPyy"]) for x in b]),
+            np.array([x["E"] for x in b]))
+
+
+def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm=None,
+             on_step=None, after_step=None):
+    """Compress every packing of system s in steps of x, y, L *= 1 - eps until
+    its packing fraction reaches phi_final (a scalar or one per packing),
+    minimising after each step (mixed precision: FP32 pair math down to
+    switch_frms).  warm: a native.Mpnn whose damped proposal precedes FIRE.
+    on_step(step, s) runs after each affine step, before the warm start and
+    FIRE, and after_step(step, s) after FIRE (used to record pre- and
+    post-minimisation states).  Returns a Curve."""
+    from . import warmstart
+
+    fin = np.broadcast_to(np.asarray(phi_final, dtype=np.float64), (s.npack,))
+    sw = switch_frms if s.precision == "mixed" else 0.0
+    rows = []
+    s.synchronize()
+    t0 = time.perf_counter()
+    it, fr, _ = s.fire(params, switch_frms=sw, check=False)
+    rows.append(_record(s) + (it, fr))
+    for step in range(1, max_steps + 1):
+        packing = rows[-1][0]
+        active = packing < fin
+        if not np.any(active):
+            break
+        s.affine_resize(np.where(active, 1.0 - eps, 1.0))
+        if on_step is not None:
+            on_step(step, s)
+        if warm is not None:
+            warmstart.damped_warm_start(s, s.sizes, warm)
+        it, fr, _ = s.fire(params, switch_frms=sw, check=False)
+        if after_step is not None:
+            after_step(step, s)
+        it = np.where(active, it, 0)
+        rows.append(_record(s) + (it, fr))
+    s.synchronize()
+    sec = time.perf_counter() - t0
+    cols = list(zip(*rows))
+    return Curve(*(np.array(c) for c in cols), seconds=sec)

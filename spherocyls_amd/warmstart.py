@@ -10,11 +10,14 @@ same F_rms < ftol, otherwise that packing is minimised again from its
 original state.
 
 Training data are (pre-minimisation, minimised) pairs the simulator makes
-itself: ragged packings get sheared Heun steps (the pre state), then FIRE
-(the target).  Training runs on the GPU through the network's own forward,
-backward and Adam kernels.
+itself.  The committed weights come from quasi-static compression: the state
+right after an affine step x, y, L *= 1 - 1e-4 (the pre state) and the state
+FIRE relaxes it to (the target), N = 4096 bidisperse aspect-2 packings
+compressed from phi 0.80 to 0.86.  The `train` variant uses ragged packings
+after sheared Heun steps instead.  Training runs on the GPU through the
+network's own forward, backward and Adam kernels.
 
-    python -m spherocyls_amd.warmstart train --out spherocyls_amd/data/mpnn_warmstart.npz
+    python -m spherocyls_amd.warmstart train-qs --out spherocyls_amd/data/mpnn_warmstart.npz
 """
 from __future__ import annotations
 
@@ -98,6 +101,73 @@ def make_pairs(packings, params, heun_steps=2000, mpnn=None):
     ok = fr < params[8]
     keep = [i for i in range(len(graphs)) if ok[i]]
     return Pairs([graphs[i] for i in keep], [targets[i] for i in keep], it[ok]), work
+
+
+def qs_packings(count, n, seed, phi0=0.80, aspect=2.0):
+    """Bidisperse packings of n particles at phi0 from seeds seed*100003 + k
+    (the quasi-static compression start of the N=16384 workload, smaller)."""
+    R, A = configs.bidisperse(n, aspect=aspect)
+    return [configs.make_packing(R, A, phi0, seed * 100003 + k, f"qs[{seed}:{k}] N={n}")
+            for k in range(count)]
+
+
+def qs_pairs(packings, params, eps=1e-4, phi_final=0.86, every=10, mpnn=None, log=None):
+    """(pre-step, minimised) pairs from batched quasi-static compression: every
+    `every`-th step, the state right after the affine step x, y, L *= 1 - eps
+    (its contact graph) and the displacement FIRE then makes."""
+    from . import quasistatic
+
+    s = native.System(packings, pads=0.1)
+    own = mpnn is None
+    m = native.Mpnn() if own else mpnn
+    graphs, targets, iters = [], [], []
+    pre = {}
+    sizes = s.sizes
+
+    def on_step(step, sy):
+        pre.clear()
+        if step % every == 0:
+            pre["g"] = m.graph_of(sy)
+            pre["x"] = sy.positions()
+            pre["L"] = [sy.box(k)["Lx"] for k in range(sy.npack)]
+
+    def after_step(step, sy):
+        if not pre:
+            return
+        post = sy.positions()
+        it = [int(sy.box(k)["iters"]) for k in range(sy.npack)]
+        o = 0
+        for k, (g, nk) in enumerate(zip(split_graph(pre["g"], sizes), sizes)):
+            sl = slice(o, o + nk)
+            o += nk
+            if sy.box(k)["frms"] >= params[8]:
+                continue  # not minimised: no target
+            graphs.append(g)
+            targets.append(_displacement(pre["x"], post, sl, pre["L"][k]).astype(np.float32))
+            iters.append(it[k])
+        if log and step % (10 * every) == 0:
+            log(f"step {step}: {len(graphs)} pairs")
+
+    try:
+        s.set_contact_model("two_point")
+        s.set_precision("fp64")
+        curve = quasistatic.compress(s, eps, phi_final, params, on_step=on_step,
+                                     after_step=after_step)
+    finally:
+        s.close()
+        if own:
+            m.close()
+    return Pairs(graphs, targets, np.array(iters)), curve
+
+
+def _displacement(pre, post, sl, L):
+    dx = post[0][sl] - pre[0][sl]
+    dy = post[1][sl] - pre[1][sl]
+    dx -= L * np.round(dx / L)
+    dy -= L * np.round(dy / L)
+    dp = post[2][sl] - pre[2][sl]
+    dp -= 2 * REF_PI * np.round(dp / (2 * REF_PI))
+    return np.column_stack([dx, dy, dp])
 
 
 def train(pairs, steps=2000, batch=8, lr=1e-3, seed=1, log_every=200, mpnn=None):
@@ -229,30 +299,39 @@ def minimize_with_warm_start(work, params, mpnn, precision="mixed", switch_frms=
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
-    t = sub.add_parser("train")
-    t.add_argument("--out", default=DEFAULT_WEIGHTS)
-    t.add_argument("--packings", type=int, default=512)
-    t.add_argument("--n", type=int, default=4096)
-    t.add_argument("--steps", type=int, default=3000)
-    t.add_argument("--batch", type=int, default=8)
-    t.add_argument("--lr", type=float, default=1e-3)
-    t.add_argument("--seed", type=int, default=1)
+    for name in ("train-qs", "train"):
+        t = sub.add_parser(name)
+        t.add_argument("--out", default=DEFAULT_WEIGHTS)
+        t.add_argument("--packings", type=int, default=16 if name == "train-qs" else 512)
+        t.add_argument("--n", type=int, default=4096)
+        t.add_argument("--steps", type=int, default=3000)
+        t.add_argument("--batch", type=int, default=8)
+        t.add_argument("--lr", type=float, default=1e-3)
+        t.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()
-    params = native.fire_params(max_iter=20000)
     t0 = time.perf_counter()
-    pairs = None
-    chunk = 64
-    for c0 in range(0, args.packings, chunk):
-        pk = workload(min(chunk, args.packings - c0), args.n, seed=1000 + args.seed + c0)
-        p, _ = make_pairs(pk, params)
-        if pairs is None:
-            pairs = p
-        else:
-            pairs.graphs += p.graphs
-            pairs.targets += p.targets
-            pairs.iterations = np.concatenate([pairs.iterations, p.iterations])
-        print(f"pairs {len(pairs.graphs)} ({time.perf_counter() - t0:.1f} s), "
-              f"FIRE iterations mean {pairs.iterations.mean():.0f}", flush=True)
+    if args.cmd == "train-qs":
+        # (pre-step, minimised) pairs of quasi-static compression, phi 0.80 -> 0.86
+        # in steps of 1e-4, every 10th step (the committed weights)
+        params = native.fire_params(max_iter=200000)
+        pairs, _ = qs_pairs(qs_packings(args.packings, args.n, seed=10 + args.seed), params,
+                            log=lambda m: print(m, flush=True))
+    else:
+        # (sheared, minimised) pairs of the ragged-ensemble physics
+        params = native.fire_params(max_iter=20000)
+        pairs = None
+        chunk = 64
+        for c0 in range(0, args.packings, chunk):
+            pk = workload(min(chunk, args.packings - c0), args.n, seed=1000 + args.seed + c0)
+            p, _ = make_pairs(pk, params)
+            if pairs is None:
+                pairs = p
+            else:
+                pairs.graphs += p.graphs
+                pairs.targets += p.targets
+                pairs.iterations = np.concatenate([pairs.iterations, p.iterations])
+    print(f"pairs {len(pairs.graphs)} ({time.perf_counter() - t0:.1f} s), "
+          f"FIRE iterations mean {pairs.iterations.mean():.0f}", flush=True)
     m, scale, hist = train(pairs, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed)
     save(m, args.out)
     print(f"saved {args.out}: scale {scale}, final loss {hist[-100:].mean():.4f}", flush=True)

@@ -166,3 +166,58 @@ def test_random_packing_is_random_mode():
     b = native.random_configuration(30.0, A, R, seed=3, config=0)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+def _state(n=37, seed=4):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0, 20, n), rng.uniform(0, 20, n), rng.uniform(-3, 3, n),
+            np.where(np.arange(n) % 2 == 0, 0.5, 0.7), rng.uniform(0, 1.5, n))
+
+
+def test_gpu_code_text_positions_layout(tmp_path):
+    """sd%010lu.dat of the GPU code's quasi-static protocols
+    (src/cuda/strain_dynamics.cu:831-838): the printf formats of that writer,
+    R at two significant digits."""
+    x, y, phi, R, A = _state()
+    head = (20.25, 20.25, 0.8123456789, 0.1, 1.25, 1e-5, 0.01)
+    path = tmp_path / "sd0000000007.dat"
+    native.write_positions_sd(str(path), *head, x, y, phi, R, A)
+    want = "%d %.17g %.17g %f %.17g %.14g %g %g\n" % ((x.shape[0],) + head)
+    want += "".join("%.17g %.17g %.17g %.2g %.17g\n" % v for v in zip(x, y, phi, R, A))
+    assert path.read_text() == want
+    # .17g round-trips doubles: the columns read back exactly (R is 0.5 / 0.7)
+    rows = np.loadtxt(path, skiprows=1)
+    for k, a in enumerate((x, y, phi, R, A)):
+        np.testing.assert_array_equal(rows[:, k], a)
+
+
+def test_binary_positions_layout_and_round_trip(tmp_path):
+    """sd%010lu.bin (src/cuda/strain_dynamics.cu:845-875): a native int, seven
+    doubles, then x y phi R A per particle."""
+    x, y, phi, R, A = _state()
+    head = (20.25, 20.5, 0.81, 0.1, 1.25, 1e-5, 0.01)
+    path = str(tmp_path / "sd0000000007.bin")
+    native.write_positions_bin(path, *head, x, y, phi, R, A)
+    raw = open(path, "rb").read()
+    assert len(raw) == 4 + 8 * (7 + 5 * x.shape[0])
+    assert np.frombuffer(raw[:4], dtype=np.int32)[0] == x.shape[0]
+    np.testing.assert_array_equal(np.frombuffer(raw[4:60], dtype=np.float64), head)
+    recs = np.frombuffer(raw[60:], dtype=np.float64).reshape(-1, 5)
+    np.testing.assert_array_equal(recs, np.column_stack([x, y, phi, R, A]))
+    h, a = native.read_positions_bin(path)
+    assert tuple(h[k] for k in native.SD_HEADER) == head
+    for k, v in zip(("x", "y", "phi", "R", "A"), (x, y, phi, R, A)):
+        np.testing.assert_array_equal(a[k], v)
+
+
+def test_binary_positions_reader_rejects_bad_files(tmp_path):
+    x, y, phi, R, A = _state(5)
+    path = tmp_path / "a.bin"
+    native.write_positions_bin(str(path), 10, 10, 0.5, 0, 0, 0, 0.01, x, y, phi, R, A)
+    raw = path.read_bytes()
+    for bad in (raw[:-8], raw + b"\0" * 8, raw[:30], b""):
+        path.write_bytes(bad)
+        with pytest.raises(native.SpherocylsError):
+            native.read_positions_bin(str(path))
+    with pytest.raises(native.SpherocylsError):
+        native.read_positions_bin(str(tmp_path / "missing.bin"))

@@ -1,0 +1,121 @@
+"""Quasi-static protocols (compress_qs2p, find_jam_2p of the reference's GPU
+code, src/cuda/2point_dynamics.cu:4300-4606) through the command-line drivers.
+
+The reference's GPU code cannot run (it reads an unallocated buffer), so the
+files are checked against what pins them: the affine step rule fixes every
+packing fraction; the oracle, evaluated at the state in each step's position
+file (written at 17 digits), must find a minimum (F_rms < 1e-10) with the
+energy and stress printed on that step's SE line; and find_jam_2p must end
+on the target energy.
+"""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import oracle
+from spherocyls_amd import native
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "spherocyls_amd", "bin")
+REF_PI = 3.14159265358979
+
+
+def _run(exe, args):
+    print("driver:", exe, " ".join(args), flush=True)
+    r = subprocess.run([os.path.join(BIN, exe)] + args, capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, SPHEROCYL_SEED="3"))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def _sd(path):
+    with open(path) as f:
+        head = f.readline().split()
+    rows = np.loadtxt(path, skiprows=1, ndmin=2)
+    return int(head[0]), float(head[1]), float(head[2]), float(head[3]), rows
+
+
+def _oracle_at(path):
+    n, Lx, Ly, packing, rows = _sd(path)
+    assert rows.shape == (n, 5) and Lx == Ly
+    b = oracle.OracleBox(Lx, rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], rows[:, 3], 0.1)
+    b.set_contact_model(1)
+    b.find_neighbors()
+    b.calc_se()
+    return b, packing, rows
+
+
+def _check_line(line, b, packing, digits):
+    v = [float(t) for t in line.split()]
+    s = b.scalars()
+    assert v[1] == pytest.approx(packing, abs=10.0 ** -digits)
+    if s["E"] < 1e-14:
+        # unjammed: energy and stress are rounding-level remainders of forces
+        # that cancel; both sides must be at that level
+        assert abs(v[2]) < 1e-14 and max(abs(x) for x in v[3:]) < 1e-7
+        return
+    for got, key in zip(v[2:], ("E", "cPxx", "cPyy", "cPxy", "cPyx")):
+        scale = max(abs(s["cPxx"]), abs(s["cPyy"])) if key != "E" else abs(s["E"])
+        # %g prints 6 significant digits
+        assert got == pytest.approx(s[key], rel=1e-5, abs=1e-5 * scale), key
+
+
+def test_compress_qs2p_files(tmp_path):
+    eps, phi0, phi1, n = 1e-3, 0.84, 0.86, 128
+    _run("spherocyl_qscompress", ["r", str(tmp_path), str(n), str(eps), "0.01", str(phi1), "0.01",
+                                  "0.01", "0.1", str(phi0), "1", "1.4"])
+    lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
+    files = sorted(glob.glob(str(tmp_path / "sd*.dat")))
+    files = [f for f in files if not f.endswith("sd_stress_energy.dat")]
+    steps = len(lines) - 1
+    # packing grows by (1 - eps)^-2 per step until it reaches phi1
+    assert steps == int(np.ceil(np.log(phi1 / phi0) / (-2 * np.log1p(-eps)) - 1e-9))
+    assert len(files) == steps + 1
+    assert lines[0].split()[0] == "0" and lines[-1].split()[0] == str(steps)
+    pressures = []
+    for t, (line, path) in enumerate(zip(lines, files)):
+        assert os.path.basename(path) == "sd%010d.dat" % t
+        b, packing, rows = _oracle_at(path)
+        want = phi0 / (1 - eps) ** (2 * t)
+        assert packing == pytest.approx(want, abs=1e-6)  # printed with %f
+        assert b.frms() < 1e-10, t  # every state is a minimum, judged by the oracle
+        _check_line(line, b, want, 6 if t == 0 else 9)
+        s = b.scalars()
+        pressures.append(0.5 * (s["cPxx"] + s["cPyy"]))
+    print("compress_qs2p: steps", steps, "pressure", pressures[0], "->", pressures[-1], flush=True)
+    # the first line prints the packing with %f, later ones with %.9f
+    assert len(lines[0].split()[1].split(".")[1]) == 6
+    assert len(lines[1].split()[1].split(".")[1]) == 9
+
+
+def test_find_jam_2p_ends_on_the_target_energy(tmp_path):
+    target, n = 1e-6, 64
+    _run("spherocyl_findjam", ["r", str(tmp_path), str(n), "1e-3", "0.01", str(target), "0.01",
+                               "0.01", "0.1", "0.80", "1", "1.4"])
+    lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
+    bins = glob.glob(str(tmp_path / "sd*.bin"))
+    assert len(bins) == 1
+    h, a = native.read_positions_bin(bins[0])
+    t_end = int(os.path.basename(bins[0])[2:12])
+    assert int(lines[-1].split()[0]) == t_end and a["x"].shape[0] == n
+    energies = np.array([float(l.split()[2]) for l in lines])
+    packings = np.array([float(l.split()[1]) for l in lines])
+    # the search brackets the target ever more tightly: the last steps straddle it
+    crossings = np.nonzero(np.diff(np.sign(energies - target)))[0]
+    assert len(crossings) >= 5
+    assert abs(packings[-1] - packings[-2]) < 1e-9
+    # the final state, in the binary file, is a minimum whose energy the oracle
+    # puts next to the target (within the last step's change)
+    b = oracle.OracleBox(h["Lx"], a["x"], a["y"], a["phi"], a["A"], a["R"], 0.1)
+    b.set_contact_model(1)
+    b.find_neighbors()
+    b.calc_se()
+    assert b.frms() < 1e-10
+    e = b.scalars()["E"]
+    print("find_jam_2p: steps", t_end, "phi_J", packings[-1], "E", e, flush=True)
+    assert e == pytest.approx(energies[-1], rel=1e-5)
+    assert abs(e - target) < 0.1 * target

@@ -1,6 +1,8 @@
 """Learned warm start: the trained network (spherocyls_amd/data/mpnn_warmstart.npz,
-trained on the GPU by `python -m spherocyls_amd.warmstart train`) on held-out
-packings, against the plain minimiser.
+trained on the GPU by `python -m spherocyls_amd.warmstart train-qs` on
+quasi-static compression steps) on held-out packings, against the plain
+minimiser: in the quasi-static compression it was trained for, and on the
+sheared ragged ensemble.
 
 The plain minimiser is the judge: every packing must end at F_rms < 1e-10
 (checked by the oracle on a sample), whether its warm start was accepted or it
@@ -15,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from spherocyls_amd import ensemble, native, warmstart
+from spherocyls_amd import ensemble, native, quasistatic, warmstart
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +71,29 @@ def test_warm_start_reaches_the_same_residual(net, held_out, precision):
         b.set_params(g, 0.0, 0.0, 0.0, 0.05)
         assert b.frms() < 1e-10
         o += pk.n
+
+
+def test_quasistatic_warm_start_gain(net):
+    """Held-out seeds, 60 compression steps of x, y, L *= 1 - 1e-4 from phi 0.80:
+    every minimisation converges with and without the warm start, and the test
+    prints the FIRE iterations and wall time of both (the gain)."""
+    pks = warmstart.qs_packings(4, 2048, seed=777)
+    params = native.fire_params(max_iter=200000)
+    phi_final = 0.80 / (1 - 1e-4) ** 120
+    res = {}
+    for label, warm in (("plain", None), ("warm start", net)):
+        s = native.System(pks, pads=0.1)
+        try:
+            s.set_contact_model("two_point")
+            c = quasistatic.compress(s, 1e-4, phi_final, params, warm=warm)
+        finally:
+            s.close()
+        assert np.all(c.frms < 1e-10), label
+        res[label] = c
+        print(f"quasi-static {label}: {c.iterations[1:].mean():.1f} FIRE iterations per step, "
+              f"{c.seconds:.2f} s for {c.iterations.shape[0] - 1} steps", flush=True)
+    gain = res["plain"].iterations[1:].sum() / res["warm start"].iterations[1:].sum()
+    print(f"iteration ratio plain / warm start: {gain:.3f}", flush=True)
+    assert res["plain"].iterations.shape == res["warm start"].iterations.shape
+    # the damped warm start never makes FIRE do materially more work
+    assert gain > 0.9
