@@ -131,6 +131,20 @@ class SpherocylBox {
   //                  steps growing by 1.1 until E crosses, then continue with
   //                  step * 1.1^(steps - 1) / 2, until the step is below 1e-15;
   //                  the final state also goes to sd%010lu.bin.
+  // run_pure_strain_2p (src/cuda/2point_dynamics.cu:2286-2397): pure shear
+  // (x expands, y contracts at half the strain rate; sc_pure_strain_steps) at
+  // |strain rate| until Lx / Ly reaches dStopAspect.  Every dSvStressGamma of
+  // strain a line "t E Pxx Pyy P Pxy Pyx" (printf %.7g; corrected stress with
+  // the CPU code's sign, P = (Pxx + Pyy) / 2) of the state the step starts
+  // from; every dSvPosGamma a position file sd%010lu.dat.  Restarts at nStart
+  // seek the SE file as run_strain does.  The reference ends with one more
+  // step of its Lees-Edwards integrator before the final save; here the final
+  // state is saved as it is.  Returns the last time stamp.
+  long unsigned int run_pure_strain(long unsigned int nStart, double dStopAspect,
+                                    double dSvStressGamma, double dSvPosGamma);
+  // flip_shear_direction (src/cuda/find_neighbors.cu:565-579): x -> L - x,
+  // phi -> -phi, gamma -> -gamma
+  void flip_shear_direction();
   long unsigned int compress_qs2p(double dMaxPacking, double dResizeStep);
   long unsigned int expand_qs2p(double dMinPacking, double dResizeStep);
   long unsigned int find_jam_2p(double dJamE, double dResizeStep);

@@ -165,6 +165,20 @@ int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out);
  * the predictor pass (the reference uses the old one, :101). */
 int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, int se_every,
                     double* se_out);
+/* Pure shear, the GPU code's pure_strain_step_2p (src/cuda/2point_dynamics.cu:
+ * 1663-2285) in this library's conventions (mobility 1, torque / I): each
+ * step is a Heun step under the affine field that expands x and contracts y
+ * at half the packing's strain rate.  Predictor: x~ = x + dt (Fx + rate x / 2),
+ * y~ = y + dt (Fy - rate y / 2), phi~ = phi + dt (tau / I - rate RC sin(2 phi) / 2);
+ * the box becomes Lx (1 + dt rate / 2) by Ly (1 - dt rate / 2); corrector:
+ * x = x~ + dt (Fx~ - Fx) / 2 and likewise y and phi, the forces of the
+ * predicted state evaluated in the new box.  gamma_total advances by
+ * dt rate; the Lees-Edwards strain gamma stays.  List path only
+ * (SC_ERR_STATE otherwise).  se_every / se_out as in sc_strain_steps. */
+int sc_pure_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out);
+/* flip_shear_direction (src/cuda/find_neighbors.cu:552-579): x -> Lx - x,
+ * phi -> -phi, gamma -> -gamma for packing k (k = -1: all). */
+int sc_flip_shear(sc_system* s, int k);
 /* gradient_descent_minimize loop (src/cpp/relaxation_dynamics.cpp:33-85,
  * gradient_relax_step :11-31): at most nmax steps, stopping when E <= emin. */
 int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done);

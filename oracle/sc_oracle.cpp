@@ -463,9 +463,12 @@ void forces_main(orc_box* b) {
 
 // calc_temp_forces (strain_dynamics.cpp:156-172), with the predicted state and
 // gamma + dt*gamma_dot; Lx/Ly are passed so the resize predictor can choose.
+void forces_temp_at(orc_box* b, double Lx, double Ly, double gam);
 void forces_temp(orc_box* b, double Lx, double Ly) {
+  forces_temp_at(b, Lx, Ly, b->gamma + b->dt * b->strain_rate);
+}
+void forces_temp_at(orc_box* b, double Lx, double Ly, double gam) {
   for (int i = 0; i < b->n; ++i) b->tfx[i] = b->tfy[i] = b->ttau[i] = 0.0;
-  double gam = b->gamma + b->dt * b->strain_rate;
   for (int p = 0; p < b->n; ++p)
     for (int q : b->nbr[p]) {
       if (q <= p) continue;
@@ -581,6 +584,63 @@ void heun_step(orc_box* b) {
   b->gamma += b->strain_rate * b->dt;
   b->gamma_total += b->strain_rate * b->dt;
   if (b->gamma >= 0.5) lees_edwards_remap(b);
+}
+
+// Pure shear: pure_strain_step_2p of the reference's GPU code
+// (src/cuda/2point_dynamics.cu:1663-2285; pure_euler_est_2p :1885-1890,
+// pure_heun_corr_2p :2141-2166, the box :2273-2274) with the CPU code's
+// conventions (mobility 1, torque / I).  x expands and y contracts at half the
+// strain rate er:
+//   predictor  x~ = x + dt (Fx + er x / 2),  y~ = y + dt (Fy - er y / 2),
+//              phi~ = phi + dt (tau / I - er RC sin(2 phi) / 2)
+//   box        Lx' = Lx + dt er Lx / 2,  Ly' = Ly - dt er Ly / 2
+//   corrector  forces of the predicted state in the new box at the same
+//              strain gamma; x = x~ + dt (Fx~ - Fx) / 2, likewise y, and phi
+//              with torque / I
+// gamma_total advances by dt er.  The checker rebuilds cells and lists every
+// step.
+void pure_step(orc_box* b) {
+  const int n = b->n;
+  const double er = b->strain_rate, dt = b->dt;
+  configure_cells(b);
+  build_neighbors(b);
+  forces_main(b);
+  std::vector<double> f0x(n), f0y(n), f0t(n);
+  for (int i = 0; i < n; ++i) {
+    f0x[i] = b->fx[i];
+    f0y[i] = b->fy[i];
+    f0t[i] = b->tau[i] / b->I[i];
+    b->tx[i] = b->x[i] + dt * (b->fx[i] + 0.5 * er * b->x[i]);
+    b->ty[i] = b->y[i] + dt * (b->fy[i] - 0.5 * er * b->y[i]);
+    b->tphi[i] = b->phi[i] + dt * (f0t[i] + er * (-0.5 * b->RC[i] * std::sin(2.0 * b->phi[i])));
+  }
+  const double Lxn = b->Lx + 0.5 * dt * er * b->Lx;
+  const double Lyn = b->Ly - 0.5 * dt * er * b->Ly;
+  forces_temp_at(b, Lxn, Lyn, b->gamma);
+  for (int i = 0; i < n; ++i) {
+    b->x[i] = b->tx[i] + 0.5 * dt * (b->tfx[i] - f0x[i]);
+    b->y[i] = b->ty[i] + 0.5 * dt * (b->tfy[i] - f0y[i]);
+    b->phi[i] = b->tphi[i] + 0.5 * dt * (b->ttau[i] / b->I[i] - f0t[i]);
+  }
+  b->Lx = Lxn;
+  b->Ly = Lyn;
+  b->gamma_total += dt * er;
+  for (int i = 0; i < n; ++i) {
+    wrap_pos(b->x[i], b->Lx);
+    wrap_pos(b->y[i], b->Ly);
+  }
+  configure_cells(b);
+  build_neighbors(b);
+}
+
+// flip_shear_direction (src/cuda/find_neighbors.cu:552-579)
+void flip_shear(orc_box* b) {
+  for (int i = 0; i < b->n; ++i) {
+    b->x[i] = b->Lx - b->x[i];
+    b->phi[i] = -b->phi[i];
+  }
+  b->gamma = -b->gamma;
+  build_neighbors(b);
 }
 
 // reconfigure_cells, resize_dynamics.cpp:106-121 (the reference compares rows
@@ -846,6 +906,8 @@ void orc_resize_step(orc_box* b, int shrink, int corrected_image) {
   resize_step(b, shrink != 0, corrected_image != 0);
 }
 void orc_gd_step(orc_box* b) { gd_step(b); }
+void orc_pure_step(orc_box* b) { pure_step(b); }
+void orc_flip_shear(orc_box* b) { flip_shear(b); }
 void orc_set_back_gamma(orc_box* b) { lees_edwards_remap(b); }
 
 int orc_run_steps(orc_box* b, int nsteps, int se_every, double* se_out) {

@@ -61,12 +61,12 @@ struct StatusError {
 };
 
 // Cell grid of configure_cells (src/cpp/spherocyl_box.cpp:228-241).
-void cell_grid(double L, double Rmax, double Amax, double& pad, int& rows, int& cols) {
+void cell_grid(double Lx, double Ly, double Rmax, double Amax, double& pad, int& rows, int& cols) {
   double wmin = 2.24 * (Rmax + Amax) + pad;
   double hmin = 2 * (Rmax + Amax) + pad;
-  rows = std::max(static_cast<int>(L / hmin), 1);
-  cols = std::max(static_cast<int>(L / wmin), 1);
-  if (pad == 0) pad = std::max(L / cols - wmin, L / rows - hmin);
+  rows = std::max(static_cast<int>(Ly / hmin), 1);
+  cols = std::max(static_cast<int>(Lx / wmin), 1);
+  if (pad == 0) pad = std::max(Lx / cols - wmin, Ly / rows - hmin);
 }
 
 struct EventPair {
@@ -428,7 +428,7 @@ void ensure_se_ring(sc_system* s, size_t nrec) {
 struct PassSpec {
   int mode = kModeForces;
   int se_every = 0, se_always = 0, do_fire = 0, do_gd = 0, do_resid = 0;
-  int resize = 0, shrink = 0, corrected_image = 0;
+  int resize = 0, shrink = 0, corrected_image = 0, pure = 0;
   double resize_scale = 1.0;
   int read_buf = 0, fill_buf = -1, clear_buf = -1;
   int kernel_id = SC_KERNEL_FORCE;
@@ -533,6 +533,7 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.resize = ps.resize;
     a.shrink = ps.shrink;
     a.corrected_image = ps.corrected_image;
+    a.pure = ps.pure;
     a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     using Fn = void (*)(ListArgs);
 #define SC_LIST_ROW(M, T, X) \
@@ -603,7 +604,7 @@ void set_prec_all(sc_system* s, int prec) {
 // rotate: c0 holds the cells of x, the predictor bins x~ into c1 and zeroes
 // c2, the corrector reads c1, bins x' into c2 and zeroes c0.
 void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every, int st,
-                  double scale) {
+                  double scale, bool pure = false) {
   const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
   if (se_every > 0 && st > 0 && st % se_every == 0) {
     // stress / energy record of the state the step starts from (the reference
@@ -620,6 +621,7 @@ void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se
   p.resize = resize;
   p.shrink = shrink;
   p.corrected_image = corrected;
+  p.pure = pure;
   p.resize_scale = scale;
   p.read_buf = c0;
   p.fill_buf = c1;
@@ -670,7 +672,7 @@ void launch_update(sc_system* s, int mode, double max_move) {
   s->cur_scal = 1 - s->cur_scal;
 }
 
-int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint);
+int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, double>>* Lhint);
 
 }  // namespace
 
@@ -776,7 +778,7 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
       }
       s->filled[k] = filled;
       P.pad = pad[k];
-      cell_grid(L[k], s->Rmax[k], s->Amax[k], P.pad, P.rows, P.cols);
+      cell_grid(L[k], L[k], s->Rmax[k], s->Amax[k], P.pad, P.rows, P.cols);
       PackScal& S = s->scal[k];
       S.Lx = S.Ly = L[k];
       S.prec = kFP64;
@@ -1136,13 +1138,16 @@ int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, i
   clear_other_buffers(s);
   // the host mirrors L (same arithmetic as the device) to know when the cell
   // grid has to change without a round trip per step
-  std::vector<double> Lm(s->npack);
-  for (int k = 0; k < s->npack; ++k) Lm[k] = s->scal[k].Lx;
+  std::vector<std::pair<double, double>> Lm(s->npack);
+  for (int k = 0; k < s->npack; ++k) Lm[k] = {s->scal[k].Lx, s->scal[k].Ly};
   for (int st = 0; st < nsteps; ++st) {
     // all packings share one resize rate and step in this call
     double scale = std::pow(1.0 - s->packs[0].resize_rate, s->packs[0].dt);
     heun_substep(s, true, shrink != 0, corrected_image != 0, se_every, st, scale);
-    for (auto& v : Lm) v = shrink ? v * scale : v / scale;
+    for (auto& v : Lm) {
+      v.first = shrink ? v.first * scale : v.first / scale;
+      v.second = shrink ? v.second * scale : v.second / scale;
+    }
     int st2 = reconfigure_if_needed(s, &Lm);
     if (st2 != SC_OK) return st2;
   }
@@ -1152,6 +1157,64 @@ int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, i
   download_scal(s);
   return e;
   API_END
+}
+
+int sc_pure_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out) {
+  if (!s || nsteps < 0 || se_every < 0) return SC_ERR_ARG;
+  if (s->path != SC_PATH_LIST)
+    return fail(s, SC_ERR_STATE, "pure shear runs on the list path (sc_set_path)");
+  API_BEGIN
+  int nrec = se_every > 0 ? nsteps / se_every : 0;
+  if (nrec > 0) ensure_se_ring(s, nrec);
+  reset_step_counters(s);
+  clear_other_buffers(s);
+  // host mirror of the box (the device's arithmetic) for the cell-grid checks
+  std::vector<std::pair<double, double>> Lm(s->npack);
+  for (int k = 0; k < s->npack; ++k) Lm[k] = {s->scal[k].Lx, s->scal[k].Ly};
+  for (int st = 0; st < nsteps; ++st) {
+    heun_substep(s, false, false, false, se_every, st, 1.0, true);
+    for (int k = 0; k < s->npack; ++k) {
+      const double h = 0.5 * s->packs[k].dt * s->packs[k].strain_rate;
+      Lm[k].first = Lm[k].first + h * Lm[k].first;
+      Lm[k].second = Lm[k].second - h * Lm[k].second;
+    }
+    int st2 = reconfigure_if_needed(s, &Lm);
+    if (st2 != SC_OK) return st2;
+  }
+  force_pass(s, false, se_every);
+  copy_se_out(s, nrec, se_out);
+  int e = check_errors(s);
+  download_scal(s);
+  return e;
+  API_END
+}
+
+int sc_flip_shear(sc_system* s, int k) {
+  if (!s || k < -1 || k >= s->npack) return SC_ERR_ARG;
+  std::vector<double> x(s->ntot), y(s->ntot), phi(s->ntot);
+  int st = sc_get_positions(s, x.data(), y.data(), phi.data());
+  if (st != SC_OK) return st;
+  try {
+    download_scal(s);
+  } catch (const HipError& e) {
+    return fail(s, SC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+  }
+  std::vector<double> gam(s->npack);
+  for (int i = 0; i < s->npack; ++i) {
+    gam[i] = s->scal[i].gamma;
+    if (k >= 0 && i != k) continue;
+    const PackInfo& P = s->packs[i];
+    const double Lx = s->scal[i].Lx;
+    for (int g = P.poff; g < P.poff + P.n; ++g) {
+      x[g] = Lx - x[g];
+      phi[g] = -phi[g];
+    }
+    gam[i] = -gam[i];
+  }
+  st = sc_set_positions(s, x.data(), y.data(), phi.data());
+  for (int i = 0; st == SC_OK && i < s->npack; ++i)
+    if (k < 0 || i == k) st = sc_set_gamma(s, i, gam[i], s->scal[i].gamma_total);
+  return st;
 }
 
 int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
@@ -1526,13 +1589,13 @@ namespace {
 // reconfigure_cells (src/cpp/resize_dynamics.cpp:106-121): when the box length
 // changes the number of cell rows or columns, rebuild the grid, tiles and
 // buckets and re-sort.
-int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint) {
+int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, double>>* Lhint) {
   if (Lhint) {
     bool any = false;
     for (int k = 0; k < s->npack; ++k) {
       double pad = s->packs[k].pad;
       int rows, cols;
-      cell_grid((*Lhint)[k], s->Rmax[k], s->Amax[k], pad, rows, cols);
+      cell_grid((*Lhint)[k].first, (*Lhint)[k].second, s->Rmax[k], s->Amax[k], pad, rows, cols);
       any |= rows != s->packs[k].rows || cols != s->packs[k].cols;
     }
     if (!any) return SC_OK;
@@ -1543,7 +1606,7 @@ int reconfigure_if_needed(sc_system* s, const std::vector<double>* Lhint) {
     PackInfo& P = s->packs[k];
     double pad = P.pad;
     int rows, cols;
-    cell_grid(s->scal[k].Lx, s->Rmax[k], s->Amax[k], pad, rows, cols);
+    cell_grid(s->scal[k].Lx, s->scal[k].Ly, s->Rmax[k], s->Amax[k], pad, rows, cols);
     if (rows != P.rows || cols != P.cols) {
       P.rows = rows;
       P.cols = cols;

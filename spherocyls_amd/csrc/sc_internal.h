@@ -150,6 +150,7 @@ struct ListArgs {
   int ntot, npack;
   int se_every, se_always, do_fire, do_gd, do_resid;
   int resize, shrink, corrected_image;
+  int pure;                // pure shear: x expands, y contracts at the packing's rate / 2
   int two_point;           // two-point contact model instead of the closest point
 };
 

@@ -191,11 +191,21 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
     L_new_x = a.shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
     L_new_y = a.shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
   }
+  if (MODE != kModeForces && a.pure) {
+    // the box of pure_strain_step_2p (src/cuda/2point_dynamics.cu:2273-2274)
+    L_new_x = Lx + 0.5 * dt * gd * Lx;
+    L_new_y = Ly - 0.5 * dt * gd * Ly;
+  }
   if (MODE == kModeCorr) {
-    gam = S.gamma + dt * gd;
-    if (a.resize && a.corrected_image) {
+    if (a.pure) {  // forces of the predicted state in the new box, strain unchanged
       Lx = L_new_x;
       Ly = L_new_y;
+    } else {
+      gam = S.gamma + dt * gd;
+      if (a.resize && a.corrected_image) {
+        Lx = L_new_x;
+        Ly = L_new_y;
+      }
     }
   }
   const bool fp64 = !MIXED || S.prec == kFP64;
@@ -307,7 +317,18 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
     } else if (MODE == kModePred) {
       const double p0 = a.b.phi[g];
       double v0x, v0y, v0t, qx, qy, qp;
-      if (!a.resize) {
+      if (a.pure) {
+        // pure_euler_est_2p (src/cuda/2point_dynamics.cu:1885-1890), mobility 1:
+        // the affine field (+x/2, -y/2) times the rate, and the rod's rotation
+        // by the strain, -RC sin(2 phi) / 2 times the rate; v0 keeps the force
+        // part for the corrector
+        v0x = Fx;
+        v0y = Fy;
+        v0t = Tq / a.b.I[g];
+        qx = xi + dt * (Fx + 0.5 * gd * xi);
+        qy = yi + dt * (Fy - 0.5 * gd * yi);
+        qp = p0 + dt * (v0t + gd * (-0.5 * a.b.RC[g] * sin(2.0 * p0)));
+      } else if (!a.resize) {
         // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
         v0x = Fx - S.gamma * Fy;
         v0y = Fy;
@@ -341,7 +362,8 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       a.v0x[g] = v0x;
       a.v0y[g] = v0y;
       a.v0t[g] = v0t;
-      const double Lbx = a.resize ? L_new_x : Lx, Lby = a.resize ? L_new_y : Ly;
+      const double Lbx = (a.resize || a.pure) ? L_new_x : Lx;
+      const double Lby = (a.resize || a.pure) ? L_new_y : Ly;
       double wx = qx, wy = qy;
       bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
       if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
@@ -351,12 +373,20 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       const double gnext = S.gamma + dt * gd;
       raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
               fabs(min_image(qy - ybi, Lby)) >= half_skin ||
-              fabs(gnext - center) >= a.gamma_window || a.resize;
+              fabs(gnext - center) >= a.gamma_window || a.resize || a.pure;
     } else {  // kModeCorr
       double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
       double Lbx = Lx, Lby = Ly;
       bool remap = false;
-      if (!a.resize) {
+      if (a.pure) {
+        // pure_heun_corr_2p (src/cuda/2point_dynamics.cu:2141-2166): the
+        // predicted state plus half the change of the force part
+        x1 = a.px[g] + 0.5 * dt * (Fx - a.v0x[g]);
+        y1 = a.py[g] + 0.5 * dt * (Fy - a.v0y[g]);
+        p1 = a.pphi[g] + 0.5 * dt * (Tq / a.b.I[g] - a.v0t[g]);
+        Lbx = L_new_x;
+        Lby = L_new_y;
+      } else if (!a.resize) {
         // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
         double v1x = Fx - gam * Fy;
         double v1y = Fy;
@@ -403,9 +433,9 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
       a.b.nx[g] = Ai * cs;
       a.b.ny[g] = Ai * sn;
       bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
-      double gnext = S.gamma + gd * dt;
+      double gnext = a.pure ? S.gamma : S.gamma + gd * dt;
       if (remap) gnext -= 1.0;
-      raise = remap || a.resize || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
+      raise = remap || a.resize || a.pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
               fabs(min_image(y1 - ybi, Lby)) >= half_skin || fabs(gnext - center) >= a.gamma_window;
     }
     if (raise && a.flag_out) a.flag_out[pk] = 1;
@@ -420,7 +450,11 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
     for (int w = 0; w < (int)(sizeof(PackScal) / sizeof(int)); ++w) dst[w] = src[w];
     PackScal& S2 = a.scal_next[pk];
     S2.step = S2.step + 1;
-    if (!a.resize) {
+    if (a.pure) {
+      S2.Lx = L_new_x;
+      S2.Ly = L_new_y;
+      S2.gamma_total = S2.gamma_total + dt * gd;
+    } else if (!a.resize) {
       double g2 = S2.gamma + gd * dt, gt = S2.gamma_total + gd * dt;
       if (g2 >= 0.5) {
         g2 -= 1.0;

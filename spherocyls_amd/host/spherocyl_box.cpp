@@ -196,6 +196,9 @@ void SpherocylBox::ensure_device() {
   if (m_nParticles <= 0) throw SpherocylError(SC_ERR_STATE, "box has no particles");
   if (!m_sys || m_bGeomStale) {
     sync_host();
+    if (m_dLx != m_dLy)
+      throw SpherocylError(SC_ERR_STATE,
+                           "SpherocylBox: a sheared-out (non-square) box cannot be rebuilt");
     if (m_sys) {
       sc_system_destroy(m_sys);
       m_sys = nullptr;
@@ -825,6 +828,79 @@ void SpherocylBox::qs_line(FILE* f, const char* fmt, unsigned long t) {
   std::fprintf(f, fmt, (int)t, m_dPacking, m_dEnergy, m_dCorr[0], m_dCorr[1], m_dCorr[2],
                m_dCorr[3]);
   std::fflush(f);
+}
+
+// ------------------------------------------------------------ pure shear
+
+void SpherocylBox::flip_shear_direction() {
+  ensure_device();
+  check(sc_flip_shear(m_sys, 0), "flip_shear");
+  m_bHostStale = true;
+  read_scalars();
+}
+
+long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double dStopAspect,
+                                                double dSvStressGamma, double dSvPosGamma) {
+  if (m_dStrainRate == 0.0) throw SpherocylError(SC_ERR_ARG, "run_pure_strain: zero strain rate");
+  m_dStrainRate = std::fabs(m_dStrainRate);  // as the reference does (2point_dynamics.cu:2292-2295)
+  const double rate = m_dStrainRate;
+  ensure_device();
+  read_scalars();
+  if (m_dLx / m_dLy > dStopAspect)
+    throw SpherocylError(SC_ERR_ARG, "run_pure_strain: box aspect already beyond the stop");
+  if (dSvStressGamma < rate * m_dStep) dSvStressGamma = rate * m_dStep;
+  if (dSvPosGamma < rate) dSvPosGamma = rate;
+  const unsigned long nIntStep = (unsigned long)(1.0 / m_dStep + 0.5);
+  const unsigned long nSvStress = (unsigned long)(dSvStressGamma / (rate * m_dStep) + 0.5);
+  const unsigned long nSvPos = (unsigned long)(dSvPosGamma / rate + 0.5);
+  if (nIntStep < 1 || nSvStress < 1 || nSvPos < 1)
+    throw SpherocylError(SC_ERR_ARG, "run_pure_strain: step or save cadence rounds to zero");
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  std::fstream outfSE;
+  if (nStart == 0) {
+    outfSE.open(sePath.c_str(), std::ios::out);
+  } else {
+    outfSE.open(sePath.c_str(), std::ios::in | std::ios::out);
+    if (outfSE && !seek_after_time(outfSE, nStart))
+      throw SpherocylError(SC_ERR_ARG, "run_pure_strain: start time not found in " + sePath);
+  }
+  if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_pure_strain: cannot open " + sePath);
+  check(sc_set_path(m_sys, SC_PATH_LIST), "set_path");
+  check(sc_set_dynamics(m_sys, 0, rate, 0.0, m_dStep), "set_dynamics");
+  auto line = [&](unsigned long t) {
+    calc_se();
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%lu %.7g %.7g %.7g %.7g %.7g %.7g\n", t, m_dEnergy,
+                  m_dCorr[0], m_dCorr[1], 0.5 * (m_dCorr[0] + m_dCorr[1]), m_dCorr[2], m_dCorr[3]);
+    outfSE << buf;
+  };
+  unsigned long nTime = nStart, nTotal = nTime * nIntStep;
+  while (m_dLx / m_dLy < dStopAspect) {
+    unsigned long i = 0;
+    while (i < nIntStep) {
+      const bool svPos = i == 0 && nTime % nSvPos == 0;
+      if (svPos || nTotal % nSvStress == 0) {
+        line(nTime);
+        if (svPos) {
+          outfSE.flush();
+          save_positions_sd(nTime);
+        }
+      }
+      // steps up to the next stress record or the end of this time unit
+      unsigned long k = std::min(nIntStep - i, nSvStress - nTotal % nSvStress);
+      check(sc_pure_strain_steps(m_sys, (int)k, 0, nullptr), "pure_strain_steps");
+      m_bHostStale = true;
+      i += k;
+      nTotal += k;
+    }
+    ++nTime;
+    read_scalars();
+  }
+  line(nTime);
+  outfSE.flush();
+  save_positions_sd(nTime);
+  outfSE.close();
+  return nTime;
 }
 
 // compress_qs2p / expand_qs2p (src/cuda/2point_dynamics.cu:4300-4390)

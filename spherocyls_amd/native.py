@@ -80,6 +80,8 @@ def load():
         "sc_resize_steps": ([sysp, C.c_int, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
         "sc_gd_steps": ([sysp, C.c_int, C.c_double, _ip], C.c_int),
         "sc_affine_resize": ([sysp, _dp], C.c_int),
+        "sc_pure_strain_steps": ([sysp, C.c_int, C.c_int, _dp], C.c_int),
+        "sc_flip_shear": ([sysp, C.c_int], C.c_int),
         "sc_fire_minimize": ([sysp, _dp, C.c_double, _ip, _dp], C.c_int),
         "sc_get_neighbors": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
         "sc_get_contacts": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
@@ -387,6 +389,15 @@ class System:
         self._ck(self.lib.sc_resize_steps(self.h, nsteps, int(shrink), int(corrected_image),
                                           se_every, _p(out)))
         return out[:nrec * self.npack * 9].reshape(nrec, self.npack, 9)
+
+    def pure_strain_steps(self, nsteps, se_every=0):
+        nrec = nsteps // se_every if se_every > 0 else 0
+        out = np.zeros(max(1, nrec * self.npack * 9))
+        self._ck(self.lib.sc_pure_strain_steps(self.h, nsteps, se_every, _p(out)))
+        return out[:nrec * self.npack * 9].reshape(nrec, self.npack, 9)
+
+    def flip_shear(self, k=-1):
+        self._ck(self.lib.sc_flip_shear(self.h, k))
 
     def gd_steps(self, nmax, emin=0.0):
         done = C.c_int()
