@@ -673,6 +673,8 @@ void launch_update(sc_system* s, int mode, double max_move) {
 }
 
 int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, double>>* Lhint);
+// systems up to this many particles step through captured graphs
+constexpr int kGraphMaxParticles = 65536;
 
 }  // namespace
 
@@ -1115,8 +1117,26 @@ int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out) {
   if (nrec > 0) ensure_se_ring(s, nrec);
   reset_step_counters(s);
   clear_other_buffers(s);
+  // Small systems are launch-bound: their steps run as a captured HIP graph of
+  // 48 steps, replayed.  The bucket, flag and scalar buffers rotate with
+  // periods 3, 3 and 2 steps, so 48 steps end where they began and one capture
+  // serves every chunk.  Graphs skip the locality re-sort (irrelevant at this
+  // size), stress records (their passes fall on particular steps) and event
+  // timing; the steps left over run eagerly.
   const int resort_every = 1000;  // locality only: cells are exact every pass
-  for (int st = 0; st < nsteps; ++st) {
+  int st = 0;
+  if (s->use_graphs && !s->timing && se_every == 0 && s->ntot <= kGraphMaxParticles) {
+    const int chunk = 48;
+    GraphExec g(s);
+    for (; st + chunk <= nsteps; st += chunk) {
+      if (!g.exec)
+        g.capture([&]() {
+          for (int m = 0; m < chunk; ++m) heun_substep(s, false, false, false, 0, m, 1.0);
+        });
+      g.launch();
+    }
+  }
+  for (; st < nsteps; ++st) {
     heun_substep(s, false, false, false, se_every, st, 1.0);
     if ((st + 1) % resort_every == 0 && st + 1 < nsteps) resort(s);
   }
