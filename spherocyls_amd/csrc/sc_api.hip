@@ -119,6 +119,14 @@ struct sc_system {
   int2* d_pblocks = nullptr;
   int2* d_tile_ranges = nullptr;
   int2* d_list_ranges = nullptr;
+  // Launch lists restricted to the packings a minimiser still advances
+  // (list path): force workgroups (indices into pblocks), packings, and
+  // update workgroups.  Used while `compact` is set.
+  bool compact = false;
+  int* d_act_blocks = nullptr;
+  int* d_act_packs = nullptr;
+  int2* d_act_ublocks = nullptr;
+  int n_act_blocks = 0, n_act_packs = 0, n_act_ublocks = 0;
   int* d_cnt[3] = {nullptr, nullptr, nullptr};
   int* d_slot[3] = {nullptr, nullptr, nullptr};
   int cur_buf = 0;
@@ -551,7 +559,8 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     const int xi = s->prec == SC_PREC_MIXED ? 1 : 0;
     Fn fn = table[xi][a.two_point ? 1 : 0][ps.mode][li];
     KernelTimer t(s, ps.kernel_id);
-    hipLaunchKernelGGL(fn, dim3(s->npblocks), dim3(s->list_block),
+    a.block_idx = s->compact ? s->d_act_blocks : nullptr;
+    hipLaunchKernelGGL(fn, dim3(s->compact ? s->n_act_blocks : s->npblocks), dim3(s->list_block),
                        sizeof(int) * kHitCap * s->list_block, s->stream, a);
     s->kseq++;
   }
@@ -574,7 +583,10 @@ void launch_reduce(sc_system* s, const PassSpec& ps, const FireParams* fp) {
   r.do_fire = ps.do_fire;
   r.do_gd = ps.do_gd;
   r.npack = s->npack;
-  hipLaunchKernelGGL(k_reduce, dim3(s->npack), dim3(256), 0, s->stream, r);
+  const bool compact = s->compact && s->path == SC_PATH_LIST;
+  r.pack_idx = compact ? s->d_act_packs : nullptr;
+  hipLaunchKernelGGL(k_reduce, dim3(compact ? s->n_act_packs : s->npack), dim3(256), 0, s->stream,
+                     r);
 }
 
 // Plain force pass at the current state (calc_forces / calc_se), reduced.
@@ -639,6 +651,30 @@ void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se
   s->cur_scal = 1 - s->cur_scal;
 }
 
+// Restrict the minimiser's launches to the packings that have not finished
+// (list path): their force workgroups, their reductions and their update
+// workgroups.  Finished packings keep their last state; their cells are
+// filled again before the final evaluation.
+void set_compact(sc_system* s, const std::vector<int>& active) {
+  std::vector<int> blocks, packs;
+  std::vector<int2> ublocks;
+  for (int k : active) {
+    const int2 r = s->list_ranges[k];
+    for (int b = r.x; b < r.x + r.y; ++b) blocks.push_back(b);
+    packs.push_back(k);
+    const PackInfo& P = s->packs[k];
+    for (int st = P.poff; st < P.poff + P.n; st += 256) ublocks.push_back(make_int2(k, st));
+  }
+  upload_vec(s, s->d_act_blocks, blocks);
+  upload_vec(s, s->d_act_packs, packs);
+  upload_vec(s, s->d_act_ublocks, ublocks);
+  HIPCK(hipStreamSynchronize(s->stream));  // the host vectors go out of scope
+  s->n_act_blocks = (int)blocks.size();
+  s->n_act_packs = (int)packs.size();
+  s->n_act_ublocks = (int)ublocks.size();
+  s->compact = true;
+}
+
 // Gradient-descent (mode 0) / FIRE (mode 1) update after a reduced force pass.
 void launch_update(sc_system* s, int mode, double max_move) {
   const int b1 = (s->cur_buf + 1) % 3, b2 = (s->cur_buf + 2) % 3;
@@ -662,10 +698,13 @@ void launch_update(sc_system* s, int mode, double max_move) {
   u.ntot = s->ntot;
   u.npack = s->npack;
   u.mode = mode;
+  const bool compact = s->compact && list;
+  u.ublocks = compact ? s->d_act_ublocks : nullptr;
   {
     KernelTimer t(s, SC_KERNEL_FIRE_UPDATE);
-    hipLaunchKernelGGL(k_update, dim3(grid_for(std::max(s->ntot, s->npack), 256)), dim3(256), 0,
-                       s->stream, u);
+    hipLaunchKernelGGL(k_update,
+                       dim3(compact ? s->n_act_ublocks : grid_for(std::max(s->ntot, s->npack), 256)),
+                       dim3(256), 0, s->stream, u);
   }
   if (list) s->kseq++;
   s->cur_buf = b1;
@@ -1270,6 +1309,9 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
   for (auto& S : s->scal) done = std::max(done, S.iters);
   if (steps_done) *steps_done = done;
   // positions are final; leave forces, energy and stress of the final state
+  // (the updates skip finished packings: fill every packing's cells first)
+  s->cur_buf = 0;
+  bin_all(s, 0, 0);
   force_pass(s, true, 0);
   return check_errors(s);
   API_END
@@ -1357,6 +1399,15 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       launch_update(s, 1, fp.max_move);
     }
   };
+  // Once half of the packings still running have finished, later launches
+  // cover only the rest (list path): the long tail of a few slow packings then
+  // costs their own work, not a pass over every packing.  The full lists come
+  // back however this function exits.
+  struct CompactOff {
+    sc_system* s;
+    ~CompactOff() { s->compact = false; }
+  } compact_off{s};
+  int last_active = s->npack;
   while (true) {
     if (graph) {
       if (!g.exec) g.capture([&]() { iterations(chunk); });
@@ -1371,13 +1422,24 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     int e = check_errors(s);
     if (e != SC_OK) {
       s->velocity_live = false;
+      s->compact = false;
       return e;
     }
     download_scal(s);
-    bool all = true;
-    for (auto& S : s->scal) all &= S.converged != 0;
-    if (all) break;
+    std::vector<int> active;
+    for (int k = 0; k < s->npack; ++k)
+      if (!s->scal[k].converged) active.push_back(k);
+    if (active.empty()) break;
+    if (s->path == SC_PATH_LIST && 2 * (int)active.size() <= last_active) {
+      set_compact(s, active);
+      last_active = (int)active.size();
+      if (g.exec) {  // the captured launches have the old lists
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+      }
+    }
   }
+  s->compact = false;
   s->velocity_live = false;
   int status = SC_OK;
   for (int k = 0; k < s->npack; ++k) {
@@ -1385,9 +1447,12 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     if (frms) frms[k] = s->scal[k].frms;
     if (s->scal[k].converged != 1) status = SC_ERR_NOT_CONVERGED;
   }
-  // forces / energy / stress of the final state in FP64
+  // forces / energy / stress of the final state in FP64; the updates skip
+  // finished packings, so the cells of every packing are filled again first
   set_prec_all(s, kFP64);
   upload_scal(s);
+  s->cur_buf = 0;
+  bin_all(s, 0, 0);
   force_pass(s, true, 0);
   int e = check_errors(s);
   download_scal(s);

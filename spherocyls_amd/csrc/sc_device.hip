@@ -157,7 +157,8 @@ __device__ void finalize_pack(const ReduceArgs& a, int pk, const PackInfo& P, co
 // so the partials are visible without device-scope fences in the force kernel.
 __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
   __shared__ double s_red[4 * kNSum];
-  const int pk = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+  const int pk = a.pack_idx ? a.pack_idx[blockIdx.x] : (int)blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
   const PackInfo P = a.packs[pk];
   const PackScal S = a.scal[pk];
   if ((a.do_fire || a.do_gd) && S.converged) {
@@ -607,16 +608,27 @@ __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, con
 __global__ void k_update(UpdateArgs u) {
 #pragma clang fp contract(off)
   const Bank& b = u.b;
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lin = blockIdx.x * blockDim.x + threadIdx.x;
   if (u.cnt_clear)
-    for (int i = g; i < u.ncells_total; i += gridDim.x * blockDim.x) u.cnt_clear[i] = 0;
-  if (u.flag_clear && g < u.npack) u.flag_clear[g] = 0;
+    for (int i = lin; i < u.ncells_total; i += gridDim.x * blockDim.x) u.cnt_clear[i] = 0;
+  if (u.flag_clear)
+    for (int i = lin; i < u.npack; i += gridDim.x * blockDim.x) u.flag_clear[i] = 0;
+  int g = lin;
+  if (u.ublocks) {  // a packing's run of particles per workgroup
+    const int2 ub = u.ublocks[blockIdx.x];
+    g = ub.y + threadIdx.x;
+    if (g >= u.packs[ub.x].poff + u.packs[ub.x].n) return;
+  }
   if (g >= u.ntot) return;
   int pk = u.pack_of[g];
+  const PackScal& S = u.scal[pk];
+  // A finished packing is left as it is: no move, no store, no binning.  Its
+  // force passes return at once, and the buckets of every packing are filled
+  // again before the state is evaluated after the minimiser.
+  if (S.converged) return;
   const PackInfo P = u.packs[pk];
-  const PackScal S = u.scal[pk];
   double x = b.x[g], y = b.y[g], p = b.phi[g];
-  const bool moved = !S.converged;
+  const bool moved = true;
   if (moved) {
     if (u.mode == 0) {
       double dt = P.dt;

@@ -152,6 +152,7 @@ struct ListArgs {
   int resize, shrink, corrected_image;
   int pure;                // pure shear: x expands, y contracts at the packing's rate / 2
   int two_point;           // two-point contact model instead of the closest point
+  const int* block_idx;    // launched workgroup -> workgroup of pblocks (null: identity)
 };
 
 // Arguments of the gradient-descent / FIRE update kernel.
@@ -170,6 +171,7 @@ struct UpdateArgs {
   int* flag_clear;
   double skin, max_move;   // skin unused: the packing's pad is the skin
   int ntot, npack, mode;
+  const int2* ublocks;     // per workgroup (packing, first particle); null: all particles
 };
 
 // Arguments of the per-packing reduction / finalisation kernel.
@@ -184,6 +186,7 @@ struct ReduceArgs {
   FireParams fire;
   int se_every, se_always, do_fire, do_gd;
   int npack;
+  const int* pack_idx;    // launched workgroup -> packing (null: identity)
 };
 
 // Device view of a system's current state, for other device code of this

@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   if (a.flag_clear && blockIdx.x == 0)
     for (int i = tid; i < a.npack; i += nthr) a.flag_clear[i] = 0;
 
-  const int2 pb = a.pblocks[blockIdx.x];
+  const int bslot = a.block_idx ? a.block_idx[blockIdx.x] : (int)blockIdx.x;
+  const int2 pb = a.pblocks[bslot];
   const int pk = pb.x;
   const PackInfo P = a.packs[pk];
   const PackScal S = a.scal[pk];
@@ -471,7 +472,7 @@ __global__ void __launch_bounds__(256) k_list(ListArgs a) {
   if constexpr (kSums) {
     if (!do_sum) return;
     block_sum<kNSum>(acc, s_red, nthr);
-    if (tid < kNSum) a.part[(size_t)blockIdx.x * kNSum + tid] = s_red[tid];
+    if (tid < kNSum) a.part[(size_t)bslot * kNSum + tid] = s_red[tid];
   }
 }
 
