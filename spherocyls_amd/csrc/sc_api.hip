@@ -1431,12 +1431,12 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       if (!s->scal[k].converged) active.push_back(k);
     if (active.empty()) break;
     if (s->path == SC_PATH_LIST && 2 * (int)active.size() <= last_active) {
-      set_compact(s, active);
-      last_active = (int)active.size();
-      if (g.exec) {  // the captured launches have the old lists
-        (void)hipGraphExecDestroy(g.exec);
+      if (g.exec) {  // the captured launches name the old lists
+        HIPCK(hipGraphExecDestroy(g.exec));
         g.exec = nullptr;
       }
+      set_compact(s, active);
+      last_active = (int)active.size();
     }
   }
   s->compact = false;

@@ -31,9 +31,66 @@
 
 namespace sc {
 
+// Ascending bitonic sort by key of the first W (key, value) pairs held in
+// registers (every index is a compile-time constant once the loops unroll).
+template <int W>
+__device__ __forceinline__ void sort_pairs(int (&key)[kCap], int (&val)[kCap]) {
+  static_assert((W & (W - 1)) == 0 && W <= kCap, "width must be a power of two <= kCap");
+#pragma unroll
+  for (int k = 2; k <= W; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const bool swap = up ? key[i] > key[l] : key[i] < key[l];
+          const int ki = key[i], vi = val[i];
+          key[i] = swap ? key[l] : ki;
+          val[i] = swap ? val[l] : vi;
+          key[l] = swap ? ki : key[l];
+          val[l] = swap ? vi : val[l];
+        }
+      }
+}
+
+// Append the members of one bucket (m <= W of them) that pass the list's
+// selection rule to particle g's row, in increasing particle id.  The bucket
+// is read once (its slots share one 64-byte line) and sorted in registers.
+template <int W>
+__device__ __forceinline__ void row_from_bucket(const ListArgs& a, const int* sl, int m, int g,
+                                                double xi, double yi, double reach_i, double Lx,
+                                                double Ly, double s, double sx, double glo,
+                                                double ghi, int& n) {
+#pragma clang fp contract(off)
+  int val[kCap], key[kCap];
+#pragma unroll
+  for (int k = 0; k < W; ++k) val[k] = k < m ? sl[k] : g;
+#pragma unroll
+  for (int k = 0; k < W; ++k) key[k] = k < m ? a.b.id[val[k]] : 0x7fffffff;
+  sort_pairs<W>(key, val);
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int q = val[k];
+    if (k < m && q != g) {
+      const double reach = reach_i + a.b.R[q] + a.b.A[q];
+      const double dy = min_image(yi - a.ey[q], Ly);
+      const double dx = min_image(xi - a.ex[q], Lx);
+      const double flo = dx + glo * dy, fhi = dx + ghi * dy;
+      if (fabs(dy) < reach + s && (fabs(flo) < reach + sx || fabs(fhi) < reach + sx)) {
+        if (n < kNbrCap) a.nbr[(size_t)n * a.ntot + g] = q;
+        ++n;
+      }
+    }
+  }
+}
+
 // Build particle g's row of the list from the cells of the evaluated state.
-// Cells are visited in a fixed order and members in increasing id, so the row
-// (and every sum over it) is independent of the atomic order of the buckets.
+// Cells are visited in a fixed order and each bucket's members in increasing
+// particle id, so the row (and every sum over it) depends neither on the
+// atomic order of the buckets nor on the storage order of the bank.  The
+// sorting width is the wavefront's largest bucket, rounded up to 4, 8 or 16.
 __device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double Ly, int g,
                          double xi, double yi, double reach_i, double glo, double ghi) {
 #pragma clang fp contract(off)
@@ -61,30 +118,12 @@ __device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double
       const int cell = P.coff + rs[i] * P.cols + cs[j];
       const int m = min(a.cnt[cell], kCap);
       const int* sl = a.slot + (size_t)cell * kCap;
-      int last = -1;
-      for (int it = 0; it < m; ++it) {
-        int best = -1, bid = 0x7fffffff;
-        for (int k = 0; k < m; ++k) {
-          int q = sl[k];
-          int idq = a.b.id[q];
-          if (idq > last && idq < bid) {
-            bid = idq;
-            best = q;
-          }
-        }
-        last = bid;
-        if (best == g || best < 0) continue;
-        const int q = best;
-        const double reach = reach_i + a.b.R[q] + a.b.A[q];
-        const double dy = min_image(yi - a.ey[q], Ly);
-        if (!(fabs(dy) < reach + s)) continue;
-        const double dx = min_image(xi - a.ex[q], Lx);
-        const double flo = dx + glo * dy, fhi = dx + ghi * dy;
-        if (fabs(flo) < reach + sx || fabs(fhi) < reach + sx) {
-          if (n < kNbrCap) a.nbr[(size_t)n * a.ntot + g] = q;
-          ++n;
-        }
-      }
+      if (__all(m <= 4))
+        row_from_bucket<4>(a, sl, m, g, xi, yi, reach_i, Lx, Ly, s, sx, glo, ghi, n);
+      else if (__all(m <= 8))
+        row_from_bucket<8>(a, sl, m, g, xi, yi, reach_i, Lx, Ly, s, sx, glo, ghi, n);
+      else
+        row_from_bucket<kCap>(a, sl, m, g, xi, yi, reach_i, Lx, Ly, s, sx, glo, ghi, n);
     }
   if (n > kNbrCap) {
     atomicOr(a.err, kErrList);
