@@ -127,6 +127,9 @@ struct sc_system {
   int* d_act_packs = nullptr;
   int2* d_act_ublocks = nullptr;
   int n_act_blocks = 0, n_act_packs = 0, n_act_ublocks = 0;
+  // update workgroups of every packing: (packing, first particle), runs of <= 256
+  std::vector<int2> ublocks;
+  int2* d_ublocks = nullptr;
   int* d_cnt[3] = {nullptr, nullptr, nullptr};
   int* d_slot[3] = {nullptr, nullptr, nullptr};
   int cur_buf = 0;
@@ -336,6 +339,16 @@ void alloc_grid(sc_system* s) {
   s->d_tile_part = s->dalloc<double>((size_t)s->ntiles * kNSum);
   HIPCK(hipMemcpyAsync(s->d_packs, s->packs.data(), sizeof(PackInfo) * s->npack,
                        hipMemcpyHostToDevice, s->stream));
+}
+
+// Update workgroups: each covers up to 256 particles of one packing.
+std::vector<int2> update_blocks(const sc_system* s, const std::vector<int>& packs) {
+  std::vector<int2> ub;
+  for (int k : packs) {
+    const PackInfo& P = s->packs[k];
+    for (int st = P.poff; st < P.poff + P.n; st += 256) ub.push_back(make_int2(k, st));
+  }
+  return ub;
 }
 
 void alloc_pblocks(sc_system* s) {
@@ -657,14 +670,12 @@ void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se
 // filled again before the final evaluation.
 void set_compact(sc_system* s, const std::vector<int>& active) {
   std::vector<int> blocks, packs;
-  std::vector<int2> ublocks;
   for (int k : active) {
     const int2 r = s->list_ranges[k];
     for (int b = r.x; b < r.x + r.y; ++b) blocks.push_back(b);
     packs.push_back(k);
-    const PackInfo& P = s->packs[k];
-    for (int st = P.poff; st < P.poff + P.n; st += 256) ublocks.push_back(make_int2(k, st));
   }
+  const std::vector<int2> ublocks = update_blocks(s, active);
   upload_vec(s, s->d_act_blocks, blocks);
   upload_vec(s, s->d_act_packs, packs);
   upload_vec(s, s->d_act_ublocks, ublocks);
@@ -675,35 +686,59 @@ void set_compact(sc_system* s, const std::vector<int>& active) {
   s->compact = true;
 }
 
-// Gradient-descent (mode 0) / FIRE (mode 1) update after a reduced force pass.
-void launch_update(sc_system* s, int mode, double max_move) {
+// A packing whose force pass has more workgroups than this is reduced once by
+// k_reduce; below it every update workgroup reduces the partials itself.
+constexpr int kFuseMaxBlocks = 512;
+
+// Gradient-descent (mode 0) / FIRE (mode 1) step after a force pass: the
+// per-packing reduction and decision (fused into the update when every
+// packing's partials are few, else a k_reduce launch first), then the update.
+void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int mode) {
   const int b1 = (s->cur_buf + 1) % 3, b2 = (s->cur_buf + 2) % 3;
+  const bool list = s->path == SC_PATH_LIST;
+  const bool compact = s->compact && list;
+  const std::vector<int2>& ranges = list ? s->list_ranges : s->tile_ranges;
+  int most = 0;
+  for (const int2& r : ranges) most = std::max(most, r.y);
+  const bool fused = most <= kFuseMaxBlocks;
+  if (!fused) launch_reduce(s, ps, &fp);
   UpdateArgs u{};
   u.b = s->B();
   u.pack_of = s->d_pack_of;
   u.packs = s->d_packs;
-  u.scal = s->d_scal[1 - s->cur_scal];
+  ReduceArgs& r = u.red;
+  r.packs = s->d_packs;
+  r.scal = s->d_scal[s->cur_scal];
+  r.scal_next = s->d_scal[1 - s->cur_scal];
+  r.part = list ? s->d_list_part : s->d_tile_part;
+  r.ranges = list ? s->d_list_ranges : s->d_tile_ranges;
+  r.pack_sum = s->d_pack_sum;
+  r.se_ring = s->d_se_ring;
+  r.fire = fp;
+  r.se_every = ps.se_every;
+  r.se_always = ps.se_always;
+  r.do_fire = ps.do_fire;
+  r.do_gd = ps.do_gd;
+  r.npack = s->npack;
+  u.fused = fused ? 1 : 0;
   u.cnt = s->d_cnt[b1];
   u.slot = s->d_slot[b1];
   u.cnt_clear = s->d_cnt[b2];
   u.ncells_total = s->ncells;
   u.err = s->d_err;
-  const bool list = s->path == SC_PATH_LIST;
   u.xb = s->d_xb;
   u.yb = s->d_yb;
   u.flag_out = list ? s->flag_out() : nullptr;
   u.flag_clear = list ? s->flag_clear() : nullptr;
   u.skin = 0;  // per packing: the cell padding (read in the kernel)
-  u.max_move = max_move;
+  u.max_move = mode == 1 ? fp.max_move : 0.0;
   u.ntot = s->ntot;
   u.npack = s->npack;
   u.mode = mode;
-  const bool compact = s->compact && list;
-  u.ublocks = compact ? s->d_act_ublocks : nullptr;
+  u.ublocks = compact ? s->d_act_ublocks : s->d_ublocks;
   {
     KernelTimer t(s, SC_KERNEL_FIRE_UPDATE);
-    hipLaunchKernelGGL(k_update,
-                       dim3(compact ? s->n_act_ublocks : grid_for(std::max(s->ntot, s->npack), 256)),
+    hipLaunchKernelGGL(k_update, dim3(compact ? s->n_act_ublocks : (int)s->ublocks.size()),
                        dim3(256), 0, s->stream, u);
   }
   if (list) s->kseq++;
@@ -853,6 +888,12 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
     }
     alloc_grid(s);
     alloc_pblocks(s);
+    {
+      std::vector<int> all(npack);
+      for (int k = 0; k < npack; ++k) all[k] = k;
+      s->ublocks = update_blocks(s, all);
+      upload_vec(s, s->d_ublocks, s->ublocks);
+    }
     upload_scal(s);
 
     Bank& b = s->bank[0];
@@ -1295,8 +1336,7 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
       ps.read_buf = s->cur_buf;
       ps.kernel_id = SC_KERNEL_FIRE_FORCE;
       launch_force(s, ps);
-      launch_reduce(s, ps, &fp);
-      launch_update(s, 0, 0.0);
+      launch_update(s, ps, fp, 0);
     }
     int e = check_errors(s);
     if (e != SC_OK) return e;
@@ -1377,8 +1417,9 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     for (double* p : {s->bank[b].vx, s->bank[b].vy, s->bank[b].vpsi})
       HIPCK(hipMemsetAsync(p, 0, sizeof(double) * s->ntot, s->stream));
   s->velocity_live = true;
-  // One FIRE iteration is three launches (force pass, per-packing reduction,
-  // update).  The rotating bucket, flag and scalar buffers repeat every 6
+  // One FIRE iteration is two launches (force pass; reduction, decision and
+  // update), or three for packings of many force workgroups, whose reduction
+  // runs once in its own launch.  The rotating bucket, flag and scalar buffers repeat every 6
   // iterations, so a chunk of 30 is captured once as a HIP graph and replayed;
   // finished packings make every kernel return early.  Event timing needs
   // eager launches, and eager chunks re-sort the bank for locality.
@@ -1395,8 +1436,7 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       ps.read_buf = s->cur_buf;
       ps.kernel_id = SC_KERNEL_FIRE_FORCE;
       launch_force(s, ps);
-      launch_reduce(s, ps, &fp);
-      launch_update(s, 1, fp.max_move);
+      launch_update(s, ps, fp, 1);
     }
   };
   // Once half of the packings still running have finished, later launches

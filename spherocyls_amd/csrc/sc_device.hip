@@ -78,31 +78,16 @@ size_t tile_lds_bytes(int cap, int groups, int nthr) {
   return off;
 }
 
-// Finalisation of the reduced sums of packing pk (thread 0 of k_reduce):
-// stress/energy records, FIRE decisions, gradient-descent stop test.
-__device__ void finalize_pack(const ReduceArgs& a, int pk, const PackInfo& P, const PackScal& S,
-                              const double* tot, bool do_se) {
+// Minimiser decision for a packing from its reduced sums: the FIRE step
+// (velocity mixing or retreat, time step and mixing factor, convergence) or
+// the gradient-descent stop test.  S is the state the force pass evaluated.
+__device__ PackScal decide(const ReduceArgs& a, const PackInfo& P, const PackScal& S,
+                           const double* tot) {
 #pragma clang fp contract(off)
-  double* ps = a.pack_sum + (size_t)pk * kNSum;
-  double area = S.Lx * S.Ly;
-  double frms = sqrt(tot[kSumR2] / (3.0 * P.n));
-  if (do_se) {
-    ps[kSeE] = tot[kSeE] * P.inv_2n;
-    for (int i = kSePxx; i <= kSeCPyx; ++i) ps[i] = tot[i] / area;
-    ps[kSeNc] = tot[kSeNc];
-    if (a.se_ring && a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0) {
-      int rec = S.step / a.se_every - 1;
-      double* r = a.se_ring + ((size_t)rec * a.npack + pk) * kNSe;
-      for (int i = 0; i < kNSe; ++i) r[i] = ps[i];
-    }
-  }
-  ps[kSumP] = tot[kSumP];
-  ps[kSumG] = tot[kSumG];
-  ps[kSumV] = tot[kSumV];
-  ps[kSumR2] = tot[kSumR2];
+  PackScal S2 = S;
+  const double frms = sqrt(tot[kSumR2] / (3.0 * P.n));
   if (a.do_fire) {
     const FireParams& fp = a.fire;
-    PackScal S2 = S;
     S2.frms = frms;
     if (frms < fp.ftol) {
       S2.converged = 1;
@@ -138,34 +123,49 @@ __device__ void finalize_pack(const ReduceArgs& a, int pk, const PackInfo& P, co
       S2.step = S.step + 1;
       if (S.prec == kFP32 && frms < fp.switch_frms) S2.prec = kFP64;
     }
-    a.scal_next[pk] = S2;
   } else if (a.do_gd) {
-    PackScal S2 = S;
+    // gradient-descent passes always record energy: E = tot[kSeE] / (2N)
     S2.frms = frms;
-    if (ps[kSeE] <= a.fire.emin || S.step >= (int)a.fire.max_iter) {
+    if (tot[kSeE] * P.inv_2n <= a.fire.emin || S.step >= (int)a.fire.max_iter) {
       S2.converged = 1;
       S2.iters = S.step;
     } else {
       S2.step = S.step + 1;
     }
-    a.scal_next[pk] = S2;
   }
+  return S2;
 }
 
-// One workgroup per packing: sum the packing's block partials in block order
-// (deterministic), then finalise.  Runs as its own launch after the force pass,
-// so the partials are visible without device-scope fences in the force kernel.
-__global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
-  __shared__ double s_red[4 * kNSum];
-  const int pk = a.pack_idx ? a.pack_idx[blockIdx.x] : (int)blockIdx.x;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const PackInfo P = a.packs[pk];
-  const PackScal S = a.scal[pk];
-  if ((a.do_fire || a.do_gd) && S.converged) {
-    if (tid == 0) a.scal_next[pk] = S;
-    return;
+// Finalisation of the reduced sums of packing pk (one thread): stress/energy
+// records, and the minimiser decision into scal_next.
+__device__ void finalize_pack(const ReduceArgs& a, int pk, const PackInfo& P, const PackScal& S,
+                              const double* tot, bool do_se) {
+#pragma clang fp contract(off)
+  double* ps = a.pack_sum + (size_t)pk * kNSum;
+  double area = S.Lx * S.Ly;
+  if (do_se) {
+    ps[kSeE] = tot[kSeE] * P.inv_2n;
+    for (int i = kSePxx; i <= kSeCPyx; ++i) ps[i] = tot[i] / area;
+    ps[kSeNc] = tot[kSeNc];
+    if (a.se_ring && a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0) {
+      int rec = S.step / a.se_every - 1;
+      double* r = a.se_ring + ((size_t)rec * a.npack + pk) * kNSe;
+      for (int i = 0; i < kNSe; ++i) r[i] = ps[i];
+    }
   }
-  const bool do_se = a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0);
+  ps[kSumP] = tot[kSumP];
+  ps[kSumG] = tot[kSumG];
+  ps[kSumV] = tot[kSumV];
+  ps[kSumR2] = tot[kSumR2];
+  if (a.do_fire || a.do_gd) a.scal_next[pk] = decide(a, P, S, tot);
+}
+
+// Sum packing pk's workgroup partials in workgroup order: thread t takes
+// partials t, t + nthr, ..., then the fixed-order block sum.  Every thread
+// calls it; the totals are in s_red[0 .. kNSum) afterwards.
+__device__ void reduce_partials(const ReduceArgs& a, int pk, double* s_red) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, nthr = blockDim.x;
   const int2 rg = a.ranges[pk];
   double tot[kNSum];
 #pragma unroll
@@ -176,10 +176,30 @@ __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
     for (int i = 0; i < kNSum; ++i) tot[i] += tp[i];
   }
   block_sum<kNSum>(tot, s_red, nthr);
-  if (tid == 0) {
+}
+
+__device__ bool records_se(const ReduceArgs& a, const PackScal& S) {
+  return a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0);
+}
+
+// One workgroup per packing: reduce, then finalise.  Runs as its own launch
+// after the force pass, so the partials are visible without device-scope
+// fences in the force kernel.  (Minimisers over packings of few workgroups
+// reduce inside k_update instead.)
+__global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
+  __shared__ double s_red[4 * kNSum];
+  const int pk = a.pack_idx ? a.pack_idx[blockIdx.x] : (int)blockIdx.x;
+  const PackInfo P = a.packs[pk];
+  const PackScal S = a.scal[pk];
+  if ((a.do_fire || a.do_gd) && S.converged) {
+    if (threadIdx.x == 0) a.scal_next[pk] = S;
+    return;
+  }
+  reduce_partials(a, pk, s_red);
+  if (threadIdx.x == 0) {
     double fin[kNSum];
     for (int i = 0; i < kNSum; ++i) fin[i] = s_red[i];
-    finalize_pack(a, pk, P, S, fin, do_se);
+    finalize_pack(a, pk, P, S, fin, records_se(a, S));
   }
 }
 
@@ -600,88 +620,108 @@ __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, con
   bin_into(g, x, y, S.Lx, S.Ly, P, cnt, slot, err);
 }
 
-// Gradient-descent / FIRE position update (one thread per particle) + binning.
+// Gradient-descent / FIRE step after a force pass (one thread per particle,
+// each workgroup within one packing) + binning.  With u.fused every workgroup
+// first reduces its packing's partials and takes the minimiser decision itself
+// (the same sums in the same order in every workgroup, so the same decision);
+// the packing's first workgroup records it.  That saves a launch per
+// iteration, and the previous kernel's partials are visible without fences.
+// Packings of many force workgroups are reduced once by k_reduce instead
+// (u.fused = 0), whose decision is read here.
 // mode 0: gradient_relax_step (src/cpp/relaxation_dynamics.cpp:15-19)
 // mode 1: FIRE update (specification in oracle/sc_oracle.cpp)
 // With neighbour lists (flag_out != null) the update raises the packing's
 // rebuild flag when a particle leaves its skin/2 box around the build position.
-__global__ void k_update(UpdateArgs u) {
+__global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
 #pragma clang fp contract(off)
+  __shared__ double s_red[4 * kNSum];
+  __shared__ PackScal s_dec;
   const Bank& b = u.b;
-  const int lin = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int lin = blockIdx.x * blockDim.x + tid;
   if (u.cnt_clear)
     for (int i = lin; i < u.ncells_total; i += gridDim.x * blockDim.x) u.cnt_clear[i] = 0;
   if (u.flag_clear)
     for (int i = lin; i < u.npack; i += gridDim.x * blockDim.x) u.flag_clear[i] = 0;
-  int g = lin;
-  if (u.ublocks) {  // a packing's run of particles per workgroup
-    const int2 ub = u.ublocks[blockIdx.x];
-    g = ub.y + threadIdx.x;
-    if (g >= u.packs[ub.x].poff + u.packs[ub.x].n) return;
-  }
-  if (g >= u.ntot) return;
-  int pk = u.pack_of[g];
-  const PackScal& S = u.scal[pk];
+  // this workgroup's packing and its decision (uniform over the workgroup)
+  const int2 ub = u.ublocks[blockIdx.x];
+  const int pk = ub.x;
+  const PackInfo P = u.packs[pk];
+  const PackScal S0 = u.red.scal[pk];
+  const bool lead = ub.y == P.poff;  // the packing's first workgroup stores the decision
   // A finished packing is left as it is: no move, no store, no binning.  Its
   // force passes return at once, and the buckets of every packing are filled
   // again before the state is evaluated after the minimiser.
-  if (S.converged) return;
-  const PackInfo P = u.packs[pk];
-  double x = b.x[g], y = b.y[g], p = b.phi[g];
-  const bool moved = true;
-  if (moved) {
-    if (u.mode == 0) {
-      double dt = P.dt;
-      x += dt * b.fx[g];
-      y += dt * b.fy[g];
-      p += dt * b.tau[g] / b.I[g];
-    } else {
-      double sq = sqrt(b.I[g]);
-      double gx = b.fx[g], gy = b.fy[g], gp = b.tau[g] / sq;
-      double vx = b.vx[g], vy = b.vy[g], vp = b.vpsi[g];
-      double ux, uy, up;
-      if (S.fire_mix) {
-        ux = S.fire_c_keep * vx + S.fire_c_force * gx;
-        uy = S.fire_c_keep * vy + S.fire_c_force * gy;
-        up = S.fire_c_keep * vp + S.fire_c_force * gp;
-      } else {
-        x -= 0.5 * S.fire_dt_old * vx;
-        y -= 0.5 * S.fire_dt_old * vy;
-        p -= 0.5 * S.fire_dt_old * vp / sq;
-        ux = uy = up = 0.0;
-      }
-      double dtn = S.fire_dt_new;
-      ux += dtn * gx;
-      uy += dtn * gy;
-      up += dtn * gp;
-      b.vx[g] = ux;
-      b.vy[g] = uy;
-      b.vpsi[g] = up;
-      double mx = dtn * ux, my = dtn * uy;
-      double m2 = mx * mx + my * my;
-      if (m2 > u.max_move * u.max_move) {
-        double sc = u.max_move / sqrt(m2);
-        mx *= sc;
-        my *= sc;
-      }
-      x += mx;
-      y += my;
-      p += dtn * up / sq;
+  if (S0.converged) {
+    if (u.fused && lead && tid == 0) u.red.scal_next[pk] = S0;
+    return;
+  }
+  if (u.fused) {
+    reduce_partials(u.red, pk, s_red);
+    if (tid == 0) {
+      double fin[kNSum];
+      for (int i = 0; i < kNSum; ++i) fin[i] = s_red[i];
+      if (lead) finalize_pack(u.red, pk, P, S0, fin, records_se(u.red, S0));
+      s_dec = decide(u.red, P, S0, fin);
     }
+  } else if (tid == 0) {
+    s_dec = u.red.scal_next[pk];
+  }
+  __syncthreads();
+  const PackScal S = s_dec;
+  if (S.converged) return;  // converged at the state just evaluated
+  const int g = ub.y + tid;
+  if (g >= P.poff + P.n) return;
+  double x = b.x[g], y = b.y[g], p = b.phi[g];
+  if (u.mode == 0) {
+    double dt = P.dt;
+    x += dt * b.fx[g];
+    y += dt * b.fy[g];
+    p += dt * b.tau[g] / b.I[g];
+  } else {
+    double sq = sqrt(b.I[g]);
+    double gx = b.fx[g], gy = b.fy[g], gp = b.tau[g] / sq;
+    double vx = b.vx[g], vy = b.vy[g], vp = b.vpsi[g];
+    double ux, uy, up;
+    if (S.fire_mix) {
+      ux = S.fire_c_keep * vx + S.fire_c_force * gx;
+      uy = S.fire_c_keep * vy + S.fire_c_force * gy;
+      up = S.fire_c_keep * vp + S.fire_c_force * gp;
+    } else {
+      x -= 0.5 * S.fire_dt_old * vx;
+      y -= 0.5 * S.fire_dt_old * vy;
+      p -= 0.5 * S.fire_dt_old * vp / sq;
+      ux = uy = up = 0.0;
+    }
+    double dtn = S.fire_dt_new;
+    ux += dtn * gx;
+    uy += dtn * gy;
+    up += dtn * gp;
+    b.vx[g] = ux;
+    b.vy[g] = uy;
+    b.vpsi[g] = up;
+    double mx = dtn * ux, my = dtn * uy;
+    double m2 = mx * mx + my * my;
+    if (m2 > u.max_move * u.max_move) {
+      double sc = u.max_move / sqrt(m2);
+      mx *= sc;
+      my *= sc;
+    }
+    x += mx;
+    y += my;
+    p += dtn * up / sq;
   }
   bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
   if (!okx || !oky || !isfinite(p)) atomicOr(u.err, kErrNonFinite);
   b.x[g] = x;
   b.y[g] = y;
-  if (moved) {
-    b.phi[g] = p;
-    double sn, cs;
-    sincos(p, &sn, &cs);
-    b.nx[g] = b.A[g] * cs;
-    b.ny[g] = b.A[g] * sn;
-  }
+  b.phi[g] = p;
+  double sn, cs;
+  sincos(p, &sn, &cs);
+  b.nx[g] = b.A[g] * cs;
+  b.ny[g] = b.A[g] * sn;
   bin_into(g, x, y, S.Lx, S.Ly, P, u.cnt, u.slot, u.err);
-  if (u.flag_out && moved) {
+  if (u.flag_out) {
     const double hs = 0.5 * P.pad;  // list skin = the packing's cell padding
     if (fabs(min_image(x - u.xb[g], S.Lx)) >= hs || fabs(min_image(y - u.yb[g], S.Ly)) >= hs)
       u.flag_out[pk] = 1;

@@ -155,25 +155,6 @@ struct ListArgs {
   const int* block_idx;    // launched workgroup -> workgroup of pblocks (null: identity)
 };
 
-// Arguments of the gradient-descent / FIRE update kernel.
-struct UpdateArgs {
-  Bank b;
-  const int* pack_of;
-  const PackInfo* packs;
-  const PackScal* scal;   // decisions of the preceding reduction
-  int* cnt;               // buckets filled with the new positions
-  int* slot;
-  int* cnt_clear;
-  int ncells_total;
-  int* err;
-  const double *xb, *yb;  // list build positions (list path)
-  int* flag_out;          // list path: raise the rebuild flag
-  int* flag_clear;
-  double skin, max_move;   // skin unused: the packing's pad is the skin
-  int ntot, npack, mode;
-  const int2* ublocks;     // per workgroup (packing, first particle); null: all particles
-};
-
 // Arguments of the per-packing reduction / finalisation kernel.
 struct ReduceArgs {
   const PackInfo* packs;
@@ -187,6 +168,26 @@ struct ReduceArgs {
   int se_every, se_always, do_fire, do_gd;
   int npack;
   const int* pack_idx;    // launched workgroup -> packing (null: identity)
+};
+
+// Arguments of the gradient-descent / FIRE update kernel.
+struct UpdateArgs {
+  Bank b;
+  const int* pack_of;
+  const PackInfo* packs;
+  int* cnt;               // buckets filled with the new positions
+  int* slot;
+  int* cnt_clear;
+  int ncells_total;
+  int* err;
+  const double *xb, *yb;  // list build positions (list path)
+  int* flag_out;          // list path: raise the rebuild flag
+  int* flag_clear;
+  double skin, max_move;   // skin unused: the packing's pad is the skin
+  int ntot, npack, mode;
+  const int2* ublocks;     // per workgroup: packing, first particle (runs of <= 256)
+  ReduceArgs red;          // the force pass's partials, scalars and minimiser settings
+  int fused;               // 1: reduce and decide here; 0: k_reduce already wrote red.scal_next
 };
 
 // Device view of a system's current state, for other device code of this
