@@ -724,7 +724,6 @@ void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int m
   u.cnt = s->d_cnt[b1];
   u.slot = s->d_slot[b1];
   u.cnt_clear = s->d_cnt[b2];
-  u.ncells_total = s->ncells;
   u.err = s->d_err;
   u.xb = s->d_xb;
   u.yb = s->d_yb;

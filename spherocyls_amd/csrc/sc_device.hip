@@ -639,14 +639,20 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
   const Bank& b = u.b;
   const int tid = threadIdx.x;
   const int lin = blockIdx.x * blockDim.x + tid;
-  if (u.cnt_clear)
-    for (int i = lin; i < u.ncells_total; i += gridDim.x * blockDim.x) u.cnt_clear[i] = 0;
   if (u.flag_clear)
     for (int i = lin; i < u.npack; i += gridDim.x * blockDim.x) u.flag_clear[i] = 0;
   // this workgroup's packing and its decision (uniform over the workgroup)
   const int2 ub = u.ublocks[blockIdx.x];
   const int pk = ub.x;
   const PackInfo P = u.packs[pk];
+  // The packing's workgroups clear its cells of the bucket buffer the next
+  // pass fills (workgroup w of nw takes cells w*256 + tid, stepping nw*256).
+  // Packings left out of a compacted launch keep their buckets: they have
+  // finished, and every bucket is refilled before the state is evaluated again.
+  if (u.cnt_clear) {
+    const int w = (ub.y - P.poff) >> 8, nw = (P.n + 255) >> 8, ncell = P.rows * P.cols;
+    for (int c = w * 256 + tid; c < ncell; c += nw * 256) u.cnt_clear[P.coff + c] = 0;
+  }
   const PackScal S0 = u.red.scal[pk];
   const bool lead = ub.y == P.poff;  // the packing's first workgroup stores the decision
   // A finished packing is left as it is: no move, no store, no binning.  Its
@@ -655,6 +661,25 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
   if (S0.converged) {
     if (u.fused && lead && tid == 0) u.red.scal_next[pk] = S0;
     return;
+  }
+  // The particle's state is loaded before the reduction, so these loads are in
+  // flight while the partials are summed.
+  const int g = ub.y + tid;
+  const bool own = g < P.poff + P.n;
+  const int gl = own ? g : P.poff;
+  double x = b.x[gl], y = b.y[gl], p = b.phi[gl];
+  const double Fx = b.fx[gl], Fy = b.fy[gl], Tq = b.tau[gl], Ig = b.I[gl];
+  const double Ag = b.A[gl];
+  double vx = 0.0, vy = 0.0, vp = 0.0;
+  if (u.mode != 0) {
+    vx = b.vx[gl];
+    vy = b.vy[gl];
+    vp = b.vpsi[gl];
+  }
+  double xbg = 0.0, ybg = 0.0;
+  if (u.flag_out) {
+    xbg = u.xb[gl];
+    ybg = u.yb[gl];
   }
   if (u.fused) {
     reduce_partials(u.red, pk, s_red);
@@ -670,62 +695,62 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
   __syncthreads();
   const PackScal S = s_dec;
   if (S.converged) return;  // converged at the state just evaluated
-  const int g = ub.y + tid;
-  if (g >= P.poff + P.n) return;
-  double x = b.x[g], y = b.y[g], p = b.phi[g];
-  if (u.mode == 0) {
-    double dt = P.dt;
-    x += dt * b.fx[g];
-    y += dt * b.fy[g];
-    p += dt * b.tau[g] / b.I[g];
-  } else {
-    double sq = sqrt(b.I[g]);
-    double gx = b.fx[g], gy = b.fy[g], gp = b.tau[g] / sq;
-    double vx = b.vx[g], vy = b.vy[g], vp = b.vpsi[g];
-    double ux, uy, up;
-    if (S.fire_mix) {
-      ux = S.fire_c_keep * vx + S.fire_c_force * gx;
-      uy = S.fire_c_keep * vy + S.fire_c_force * gy;
-      up = S.fire_c_keep * vp + S.fire_c_force * gp;
+  int bin_c = -1;
+  if (own) {
+    if (u.mode == 0) {
+      double dt = P.dt;
+      x += dt * Fx;
+      y += dt * Fy;
+      p += dt * Tq / Ig;
     } else {
-      x -= 0.5 * S.fire_dt_old * vx;
-      y -= 0.5 * S.fire_dt_old * vy;
-      p -= 0.5 * S.fire_dt_old * vp / sq;
-      ux = uy = up = 0.0;
+      double sq = sqrt(Ig);
+      double gx = Fx, gy = Fy, gp = Tq / sq;
+      double ux, uy, up;
+      if (S.fire_mix) {
+        ux = S.fire_c_keep * vx + S.fire_c_force * gx;
+        uy = S.fire_c_keep * vy + S.fire_c_force * gy;
+        up = S.fire_c_keep * vp + S.fire_c_force * gp;
+      } else {
+        x -= 0.5 * S.fire_dt_old * vx;
+        y -= 0.5 * S.fire_dt_old * vy;
+        p -= 0.5 * S.fire_dt_old * vp / sq;
+        ux = uy = up = 0.0;
+      }
+      double dtn = S.fire_dt_new;
+      ux += dtn * gx;
+      uy += dtn * gy;
+      up += dtn * gp;
+      b.vx[g] = ux;
+      b.vy[g] = uy;
+      b.vpsi[g] = up;
+      double mx = dtn * ux, my = dtn * uy;
+      double m2 = mx * mx + my * my;
+      if (m2 > u.max_move * u.max_move) {
+        double sc = u.max_move / sqrt(m2);
+        mx *= sc;
+        my *= sc;
+      }
+      x += mx;
+      y += my;
+      p += dtn * up / sq;
     }
-    double dtn = S.fire_dt_new;
-    ux += dtn * gx;
-    uy += dtn * gy;
-    up += dtn * gp;
-    b.vx[g] = ux;
-    b.vy[g] = uy;
-    b.vpsi[g] = up;
-    double mx = dtn * ux, my = dtn * uy;
-    double m2 = mx * mx + my * my;
-    if (m2 > u.max_move * u.max_move) {
-      double sc = u.max_move / sqrt(m2);
-      mx *= sc;
-      my *= sc;
+    bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
+    if (!okx || !oky || !isfinite(p)) atomicOr(u.err, kErrNonFinite);
+    b.x[g] = x;
+    b.y[g] = y;
+    b.phi[g] = p;
+    double sn, cs;
+    sincos(p, &sn, &cs);
+    b.nx[g] = Ag * cs;
+    b.ny[g] = Ag * sn;
+    bin_c = cell_of(x, y, S.Lx, S.Ly, P);
+    if (u.flag_out) {
+      const double hs = 0.5 * P.pad;  // list skin = the packing's cell padding
+      if (fabs(min_image(x - xbg, S.Lx)) >= hs || fabs(min_image(y - ybg, S.Ly)) >= hs)
+        u.flag_out[pk] = 1;
     }
-    x += mx;
-    y += my;
-    p += dtn * up / sq;
   }
-  bool okx = wrap_coord(x, S.Lx), oky = wrap_coord(y, S.Ly);
-  if (!okx || !oky || !isfinite(p)) atomicOr(u.err, kErrNonFinite);
-  b.x[g] = x;
-  b.y[g] = y;
-  b.phi[g] = p;
-  double sn, cs;
-  sincos(p, &sn, &cs);
-  b.nx[g] = b.A[g] * cs;
-  b.ny[g] = b.A[g] * sn;
-  bin_into(g, x, y, S.Lx, S.Ly, P, u.cnt, u.slot, u.err);
-  if (u.flag_out) {
-    const double hs = 0.5 * P.pad;  // list skin = the packing's cell padding
-    if (fabs(min_image(x - u.xb[g], S.Lx)) >= hs || fabs(min_image(y - u.yb[g], S.Ly)) >= hs)
-      u.flag_out[pk] = 1;
-  }
+  bin_wave<1>(bin_c, g, u.cnt, u.slot, u.err);
 }
 
 // Affine rescale (x, y) *= scale[pack] (quasi-static compression step).

@@ -177,8 +177,7 @@ struct UpdateArgs {
   const PackInfo* packs;
   int* cnt;               // buckets filled with the new positions
   int* slot;
-  int* cnt_clear;
-  int ncells_total;
+  int* cnt_clear;         // buckets cleared for the pass after next (each packing's own cells)
   int* err;
   const double *xb, *yb;  // list build positions (list path)
   int* flag_out;          // list path: raise the rebuild flag
