@@ -193,9 +193,13 @@ constexpr int kGather = 4;
 // MIXED: the kernel carries the FP32-force pair path besides the FP64 one and
 // picks per packing (a mixed-mode FIRE moves packings to FP64 one by one).
 // FP64 systems launch the MIXED = false kernel, whose single pair path needs
-// fewer registers: 4 wavefronts per SIMD instead of 3 for the Heun passes.
+// fewer registers.  Its force pass and corrector are held to 128 VGPRs (4
+// wavefronts per SIMD instead of 3, at the cost of a few spilled dwords
+// outside the pair loop): 1.2 us off the corrector at N = 262144 and 5% off
+// the minimiser's force pass.  The predictor fits in 118 without the bound.
 template <int MODE, int LANES, bool TWO_POINT, bool MIXED>
-__global__ void __launch_bounds__(256) k_list(ListArgs a) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(MODE != kModePred && !MIXED ? 4 : 1))) k_list(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double s_red[4 * kNSum];
   extern __shared__ int s_hit[];  // [slot][thread] partner columns, kHitCap x blockDim
