@@ -99,3 +99,28 @@ def minimize_one_by_one(work: Workload, params, precision="fp64", limit=None):
         for s in systems:
             s.close()
     return t1 - t0, np.array(its), np.array(frs)
+
+
+def main(argv=None):
+    """Minimise the ragged benchmark ensemble once, all packings in one system:
+    python -m spherocyls_amd.ensemble [--packings 256] [--precision fp64|mixed]"""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--packings", type=int, default=256)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--max-iter", type=int, default=20000)
+    args = ap.parse_args(argv)
+    work = prepare(configs.ensemble(args.packings), heun_steps=2000, strain_rate=1e-4, dt=0.01)
+    sec, it, fr, st = minimize_batched(work, native.fire_params(max_iter=args.max_iter),
+                                       precision=args.precision)
+    conv = int(np.sum(fr < 1e-10))
+    print(json.dumps({"packings": args.packings, "particles": work.particles,
+                      "precision": args.precision, "seconds": sec, "converged": conv,
+                      "packings_per_s": conv / sec, "max_iterations": int(it.max()),
+                      "mean_iterations": float(it.mean())}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -211,6 +211,33 @@ def test_fire_first_iterations_match_oracle(path):
     assert np.max(np.abs(wrap_diff(phi, p["phi"], 2 * np.pi))) < 1e-9
 
 
+@pytest.mark.parametrize("block,lanes", [(256, 1), (64, 8)])
+def test_fire_reduction_fused_and_separate(block, lanes):
+    """FIRE iterations of a packing whose force pass has few workgroups (the
+    update kernel reduces the partial sums itself) and many (more than 512:
+    a separate reduction launch first) both follow the oracle."""
+    R, A = configs.bidisperse(4200)
+    pk = configs.make_packing(R, A, 0.80, 57)
+    k = 40
+    params = native.fire_params(**FIRE, max_iter=k)
+    s = native.System(pk, pads=0.1)
+    s.set_path("list")
+    s.set_list_block(block, lanes)
+    try:
+        it, fr, st = s.fire(params, check=False)
+        assert st == native.SC_ERR_NOT_CONVERGED and it[0] == k
+        x, y, phi = s.positions()
+    finally:
+        s.close()
+    nblocks = -(-pk.n // (block // lanes))
+    assert (nblocks > 512) == (lanes == 8)
+    ob = oracle.OracleBox.from_packing(pk, 0.1)
+    ob.fire(params)
+    p = ob.particles()
+    assert np.max(np.abs(wrap_diff(x, p["x"], pk.L))) < 1e-9
+    assert np.max(np.abs(wrap_diff(phi, p["phi"], 2 * np.pi))) < 1e-9
+
+
 def test_fire_converges_and_oracle_agrees():
     R, A = configs.bidisperse(2048)
     pk = configs.make_packing(R, A, 0.70, 53)  # well below jamming: overlap-free minimum
