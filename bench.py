@@ -11,6 +11,12 @@ Workloads (seeded synthetic packings, spherocyls_amd/configs.py):
                strain rate 1e-6, dt 0.01  (default; largest single-GPU case)
   compress     N=16384 bidisperse aspect 2, phi=0.80, strain rate 1e-5
   shear_small  N=512 bidisperse aspect 2, phi=0.85, strain rate 1e-5
+
+Beside the headline fields the line carries, unless switched off: the
+reference CPU path on the same packing (cpu_baseline), packings/s minimised
+over the ragged 256-packing ensemble (minimization), the quasi-static
+compression curve (quasistatic), and the large shear in mixed precision with
+minimisation checkpoints every 1000 steps (checkpoints).
 """
 import argparse
 import json
@@ -284,6 +290,9 @@ def main():
                     help="shear_large with this many particles (HBM-resident sizes)")
     ap.add_argument("--no-quasistatic", action="store_true",
                     help="skip the quasi-static compression measurement")
+    ap.add_argument("--checkpoints", type=int, default=5, metavar="K",
+                    help="large near-jamming shear in mixed precision with K minimisation "
+                         "checkpoints, one every 1000 steps (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -366,6 +375,12 @@ def main():
                                            0.0 if args.no_cpu_baseline else 15.0)
     if not args.no_quasistatic:
         out["quasistatic"] = quasistatic_compression()
+    if args.checkpoints > 0:
+        from spherocyls_amd import checkpoints
+
+        chk = checkpoints.run(checkpoints=args.checkpoints, every=1000, precision="mixed")
+        chk.pop("records")
+        out["checkpoints"] = chk
     print(json.dumps(out), flush=True)
 
 

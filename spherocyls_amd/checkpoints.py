@@ -1,0 +1,118 @@
+"""Long shear of one large packing with periodic minimisation checkpoints.
+
+The large near-jamming workload: N = 262144, R uniform in [0.5, 0.7],
+A = 0.5 R, phi = 0.845, sheared Heun steps at strain rate 1e-6 (dt 0.01) under
+Lees-Edwards boundaries with FP32 pair forces and FP64 sums.  Every `every`
+steps the current state is copied out and minimised (learned warm start, then
+FIRE to F_rms < 1e-10 with the two-point contact model, the plain FIRE run
+from the unmodified state as fallback), while the shear continues from the
+unminimised state.  The same copy is also minimised by plain FIRE, without the
+warm start, for comparison.  At each checkpoint the stress and energy of the
+sheared state are evaluated in the run's precision and in FP64 on the same
+state, and their relative difference is recorded (for the shear stress also
+relative to the pressure, since Pxy is small beside Pxx and Pyy).
+
+    python -m spherocyls_amd.checkpoints [--checkpoints 10] [--every 1000]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import configs, ensemble, native
+
+SE_KEYS = ("E", "Pxx", "Pyy", "Pxy")
+
+
+def run(pk=None, checkpoints=10, every=1000, strain_rate=1e-6, dt=0.01, precision="mixed",
+        warm_start=True, params=None, log=None):
+    """Shear pk for checkpoints * every steps with a minimisation every `every`
+    steps.  Returns a dict: Heun particle-steps/s (the shear alone), per
+    checkpoint the precision-vs-FP64 stress error and the minimisation's
+    seconds, iterations and F_rms."""
+    from . import warmstart
+
+    pk = configs.shear_large(1) if pk is None else pk
+    params = native.fire_params(max_iter=200000) if params is None else params
+    net = warmstart.load() if warm_start and os.path.exists(warmstart.DEFAULT_WEIGHTS) else None
+    s = native.System(pk, pads=0.1)
+    recs = []
+    heun_s = 0.0
+    try:
+        s.set_dynamics(strain_rate=strain_rate, step=dt)
+        s.set_precision(precision)
+        s.autotune(reps=5)
+        for c in range(checkpoints):
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.strain_steps(every)
+            heun_s += time.perf_counter() - t0
+            # the sheared state's stress and energy: run precision against FP64
+            s.calc_se()
+            b_run = s.box()
+            s.set_precision("fp64")
+            s.calc_se()
+            b64 = s.box()
+            s.set_precision(precision)
+            err = {k: abs(b_run[k] - b64[k]) / max(abs(b64[k]), 1e-300) for k in SE_KEYS}
+            pressure = 0.5 * abs(b64["Pxx"] + b64["Pyy"])
+            err["Pxy_vs_pressure"] = abs(b_run["Pxy"] - b64["Pxy"]) / max(pressure, 1e-300)
+            # minimisation checkpoint of a copy of the state
+            x, y, phi = s.positions()
+            work = ensemble.Workload([configs.Packing(b64["Lx"], x, y, phi, pk.A, pk.R,
+                                                      pk.packing_fraction, pk.name)],
+                                     [b64["gamma"]])
+            psec, pit, pfr, _ = ensemble.minimize_batched(work, params, precision=precision)
+            if net is not None:
+                sec, it, fr, fb = warmstart.minimize_with_warm_start(work, params, net,
+                                                                     precision=precision)
+            else:
+                sec, it, fr, fb = psec, pit, pfr, []
+            rec = {"step": (c + 1) * every, "gamma_total": b64["gamma_total"],
+                   "se_rel_error_vs_fp64": err, "E": b64["E"], "Pxx": b64["Pxx"],
+                   "Pyy": b64["Pyy"], "Pxy": b64["Pxy"], "min_seconds": sec,
+                   "min_iterations": int(it[0]), "min_frms": float(fr[0]),
+                   "min_converged": bool(fr[0] < params[8]), "fallback": bool(fb),
+                   "plain_seconds": psec, "plain_iterations": int(pit[0]),
+                   "plain_converged": bool(pfr[0] < params[8])}
+            recs.append(rec)
+            if log:
+                log(json.dumps(rec))
+    finally:
+        s.close()
+        if net is not None:
+            net.close()
+    steps = checkpoints * every
+    worst = {k: max(r["se_rel_error_vs_fp64"][k] for r in recs)
+             for k in (*SE_KEYS, "Pxy_vs_pressure")}
+    return {"workload": pk.name, "n_particles": pk.n, "precision": precision,
+            "steps": steps, "checkpoints": checkpoints, "heun_seconds": heun_s,
+            "particle_steps_per_s": pk.n * steps / heun_s,
+            "se_rel_error_vs_fp64_max": worst,
+            "minimisations_converged": sum(r["min_converged"] for r in recs),
+            "min_seconds_mean": float(np.mean([r["min_seconds"] for r in recs])),
+            "min_iterations_mean": float(np.mean([r["min_iterations"] for r in recs])),
+            "plain_seconds_mean": float(np.mean([r["plain_seconds"] for r in recs])),
+            "plain_iterations_mean": float(np.mean([r["plain_iterations"] for r in recs])),
+            "warm_start": net is not None, "records": recs}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--checkpoints", type=int, default=10)
+    ap.add_argument("--every", type=int, default=1000)
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--no-warm-start", action="store_true")
+    args = ap.parse_args(argv)
+    out = run(checkpoints=args.checkpoints, every=args.every, precision=args.precision,
+              warm_start=not args.no_warm_start, log=lambda m: print(m, flush=True))
+    out.pop("records")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

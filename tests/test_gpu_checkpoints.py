@@ -1,0 +1,24 @@
+"""The long-shear protocol with minimisation checkpoints, on a small packing:
+the mixed-precision stress and energy of each sheared state against FP64 on
+the same state, and every checkpoint minimised to F_rms < 1e-10 (with and
+without the learned warm start)."""
+import pytest
+
+from spherocyls_amd import checkpoints, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_shear_with_checkpoints_small():
+    pk = configs.shear_large(3, n=4096)
+    out = checkpoints.run(pk, checkpoints=2, every=200, precision="mixed")
+    assert out["steps"] == 400 and out["n_particles"] == 4096
+    assert out["particle_steps_per_s"] > 0
+    err = out["se_rel_error_vs_fp64_max"]
+    # FP32 pair forces, FP64 sums: the error against FP64 on the same state
+    for k in ("E", "Pxx", "Pyy", "Pxy_vs_pressure"):
+        assert err[k] < 1e-4, (k, err[k])
+    assert out["minimisations_converged"] == 2
+    for r in out["records"]:
+        assert r["min_frms"] < 1e-10 and r["plain_converged"]
+        assert r["step"] in (200, 400)
