@@ -686,9 +686,13 @@ void set_compact(sc_system* s, const std::vector<int>& active) {
   s->compact = true;
 }
 
-// A packing whose force pass has more workgroups than this is reduced once by
-// k_reduce; below it every update workgroup reduces the partials itself.
-constexpr int kFuseMaxBlocks = 512;
+// Every update workgroup reducing its packing's partials itself reads
+// (update workgroups) x (force workgroups) x kNSum doubles per packing, which
+// grows with N^2.  Up to this many bytes in all (and force workgroups per
+// packing) the update kernel reduces; beyond it k_reduce reduces each packing
+// once, in a launch of its own.
+constexpr double kFuseMaxRereadBytes = 32.0 * 1024 * 1024;
+constexpr int kFuseMaxBlocks = 4096;
 
 // Gradient-descent (mode 0) / FIRE (mode 1) step after a force pass: the
 // per-packing reduction and decision (fused into the update when every
@@ -699,8 +703,12 @@ void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int m
   const bool compact = s->compact && list;
   const std::vector<int2>& ranges = list ? s->list_ranges : s->tile_ranges;
   int most = 0;
-  for (const int2& r : ranges) most = std::max(most, r.y);
-  const bool fused = most <= kFuseMaxBlocks;
+  double reread = 0.0;
+  for (int k = 0; k < s->npack; ++k) {
+    most = std::max(most, ranges[k].y);
+    reread += (double)((s->packs[k].n + 255) / 256) * ranges[k].y * kNSum * sizeof(double);
+  }
+  const bool fused = most <= kFuseMaxBlocks && reread <= kFuseMaxRereadBytes;
   if (!fused) launch_reduce(s, ps, &fp);
   UpdateArgs u{};
   u.b = s->B();

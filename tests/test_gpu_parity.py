@@ -211,12 +211,13 @@ def test_fire_first_iterations_match_oracle(path):
     assert np.max(np.abs(wrap_diff(phi, p["phi"], 2 * np.pi))) < 1e-9
 
 
-@pytest.mark.parametrize("block,lanes", [(256, 1), (64, 8)])
-def test_fire_reduction_fused_and_separate(block, lanes):
-    """FIRE iterations of a packing whose force pass has few workgroups (the
-    update kernel reduces the partial sums itself) and many (more than 512:
-    a separate reduction launch first) both follow the oracle."""
-    R, A = configs.bidisperse(4200)
+@pytest.mark.parametrize("n,block,lanes", [(4200, 256, 1), (4200, 64, 8), (33000, 64, 8)])
+def test_fire_reduction_fused_and_separate(n, block, lanes):
+    """FIRE iterations of a packing whose partial sums the update kernel
+    reduces itself (every update workgroup re-reads them: fused while that is
+    at most 4096 force workgroups and 32 MiB in all) and of one too large for
+    that (a separate reduction launch first) both follow the oracle."""
+    R, A = configs.bidisperse(n)
     pk = configs.make_packing(R, A, 0.80, 57)
     k = 40
     params = native.fire_params(**FIRE, max_iter=k)
@@ -230,7 +231,9 @@ def test_fire_reduction_fused_and_separate(block, lanes):
     finally:
         s.close()
     nblocks = -(-pk.n // (block // lanes))
-    assert (nblocks > 512) == (lanes == 8)
+    reread = -(-pk.n // 256) * nblocks * 13 * 8
+    fused = nblocks <= 4096 and reread <= 32 * 1024 * 1024
+    assert fused == (n < 10000)
     ob = oracle.OracleBox.from_packing(pk, 0.1)
     ob.fire(params)
     p = ob.particles()
