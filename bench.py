@@ -339,6 +339,25 @@ def main():
     corr_ms, corr_n = s.kernel_time(native.SC_KERNEL_CORRECT)
     s.timing(False)
     s.close()
+
+    # the same steps from the caller's host arrays to host results: system
+    # creation (upload, cell sort, lists) + steps + download of the positions
+    t0 = time.perf_counter()
+    s2 = native.System(pk, pads=0.1)
+    try:
+        s2.set_dynamics(strain_rate=strain_rate, step=dt)
+        if args.precision == "mixed":
+            s2.set_precision("mixed")
+        s2.set_path(path)
+        if path == "tile":
+            s2.set_tiling(shape, block)
+        else:
+            s2.set_list_block(block, shape)
+        s2.strain_steps(args.steps)
+        s2.positions()
+    finally:
+        s2.close()
+    host_s = time.perf_counter() - t0
     corr_avg = corr_ms / max(corr_n, 1) * 1e-3
     pred_avg = pred_ms / max(pred_n, 1) * 1e-3
     achieved = BYTES_CORR * pk.n / corr_avg / 1e9
@@ -367,6 +386,10 @@ def main():
                      "kernel": f"k_{path}<corrector>", "avg_us": corr_avg * 1e6,
                      "predictor_avg_us": pred_avg * 1e6,
                      "bytes_per_particle": BYTES_CORR},
+        "host_to_host": {"value": pk.n * args.steps / host_s, "unit": "particle-steps/s",
+                         "seconds": host_s,
+                         "includes": "system creation from host arrays (upload, cell sort, "
+                                     "lists), the same steps, download of the positions"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pk, strain_rate, dt)
