@@ -269,12 +269,10 @@ __global__ void __launch_bounds__(256)
     // Every lane of the particle builds the same row (identical stores), so a
     // lane only ever reads back what it wrote itself; rebuilds are rare.
     const bool rebuild = a.flag_in[pk] != 0;
-    double center, xbi, ybi;
+    double center;
     int n;
     if (rebuild) {
       center = gam;
-      xbi = xi;
-      ybi = yi;
       n = build_row(a, P, Lx, Ly, g, xi, yi, reach_i, gam - a.gamma_window, gam + a.gamma_window);
       if (lane == 0) {
         a.xb[g] = xi;
@@ -284,8 +282,6 @@ __global__ void __launch_bounds__(256)
     } else {
       n = a.nbr_n[g];
       center = a.list_center[pk];
-      xbi = a.xb[g];
-      ybi = a.yb[g];
     }
 
     // ---- forces over the row, in two phases.  Phase 1 walks the row with
@@ -343,6 +339,13 @@ __global__ void __launch_bounds__(256)
     }
     // ---- per-particle epilogue (lane 0)
     const double half_skin = 0.5 * P.pad;
+    // build position of the row (written above by this lane when it rebuilt
+    // the row), loaded only here so it is not live across the pair loop
+    double xbi = 0.0, ybi = 0.0;
+    if (MODE != kModeForces && lane == 0) {
+      xbi = a.xb[g];
+      ybi = a.yb[g];
+    }
     bool raise = false;
     if (lane != 0) {
     } else if (MODE == kModeForces) {
