@@ -18,7 +18,7 @@ namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
 template <int MODE, bool TWO_POINT>
 __global__ void k_tile(TileArgs a);
-template <int MODE, int LANES, bool TWO_POINT, bool MIXED>
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE>
 __global__ void k_list(ListArgs a);
 __global__ void k_reduce(ReduceArgs a);
 __global__ void k_segvec(Bank b, int ntot);
@@ -557,20 +557,24 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.pure = ps.pure;
     a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     using Fn = void (*)(ListArgs);
-#define SC_LIST_ROW(M, T, X) \
-  {&k_list<M, 1, T, X>, &k_list<M, 2, T, X>, &k_list<M, 4, T, X>, &k_list<M, 8, T, X>}
-#define SC_LIST_MODES(T, X)                                            \
-  {SC_LIST_ROW(kModeForces, T, X), SC_LIST_ROW(kModePred, T, X), \
-   SC_LIST_ROW(kModeCorr, T, X)}
-    // [pair precision: FP64 only / mixed][contact model][mode][lanes]
-    static const Fn table[2][2][3][4] = {
+#define SC_LIST_ROW(M, T, X, E)                                                   \
+  {&k_list<M, 1, T, X, E>, &k_list<M, 2, T, X, E>, &k_list<M, 4, T, X, E>, \
+   &k_list<M, 8, T, X, E>}
+#define SC_LIST_MODES(T, X)                                                          \
+  {SC_LIST_ROW(kModeForces, T, X, true), SC_LIST_ROW(kModePred, T, X, false),        \
+   SC_LIST_ROW(kModeCorr, T, X, false), SC_LIST_ROW(kModeForces, T, X, false)}
+    // [pair precision: FP64 only / mixed][contact model][mode][lanes]; mode 3
+    // is the force pass that records no stress / energy (minimiser passes)
+    static const Fn table[2][2][4][4] = {
         {SC_LIST_MODES(false, false), SC_LIST_MODES(true, false)},
         {SC_LIST_MODES(false, true), SC_LIST_MODES(true, true)}};
 #undef SC_LIST_MODES
 #undef SC_LIST_ROW
     const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
     const int xi = s->prec == SC_PREC_MIXED ? 1 : 0;
-    Fn fn = table[xi][a.two_point ? 1 : 0][ps.mode][li];
+    const bool se = ps.se_always || ps.se_every > 0;
+    const int mi = ps.mode == kModeForces && !se ? 3 : ps.mode;
+    Fn fn = table[xi][a.two_point ? 1 : 0][mi][li];
     KernelTimer t(s, ps.kernel_id);
     a.block_idx = s->compact ? s->d_act_blocks : nullptr;
     hipLaunchKernelGGL(fn, dim3(s->compact ? s->n_act_blocks : s->npblocks), dim3(s->list_block),

@@ -197,7 +197,11 @@ constexpr int kGather = 4;
 // wavefronts per SIMD instead of 3, at the cost of a few spilled dwords
 // outside the pair loop): 1.2 us off the corrector at N = 262144 and 5% off
 // the minimiser's force pass.  The predictor fits in 118 without the bound.
-template <int MODE, int LANES, bool TWO_POINT, bool MIXED>
+// SE: the plain force pass accumulates stress and energy over its contacts
+// (calc_se, recorded steps).  The minimiser's passes need only the residual
+// and FIRE sums, which are formed after the pair loop; their SE = false
+// kernel keeps no accumulators live across the loop.
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(MODE != kModePred && !MIXED ? 4 : 1))) k_list(ListArgs a) {
 #pragma clang fp contract(off)
@@ -224,8 +228,8 @@ __global__ void __launch_bounds__(256)
   // residual); the Heun passes carry no accumulators, which keeps their
   // register footprint and occupancy where the hot loop needs it.
   constexpr bool kSums = MODE == kModeForces;
-  const bool do_se =
-      kSums && (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
+  const bool do_se = kSums && SE &&
+                     (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
   const bool do_sum = kSums && (do_se || a.do_fire || a.do_gd || a.do_resid);
 
   const double dt = P.dt, gd = P.strain_rate;
@@ -323,7 +327,7 @@ __global__ void __launch_bounds__(256)
       }
       for (int h = 0; h < nh; ++h) {
         const int q = s_hit[h * nthr + tid];
-        pair_into<kSums, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, a.ex[q], a.ey[q], a.enx[q],
+        pair_into<kSums && SE, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, a.ex[q], a.ey[q], a.enx[q],
                                     a.eny[q], a.b.R[q], a.b.A[q], a.b.id[q], Lx, Ly, gam, fp64,
                                     do_se, Fx, Fy, Tq, acc);
       }
@@ -522,10 +526,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-#define SC_LIST_INST(L, T, X)                                       \
-  template __global__ void k_list<kModeForces, L, T, X>(ListArgs); \
-  template __global__ void k_list<kModePred, L, T, X>(ListArgs);   \
-  template __global__ void k_list<kModeCorr, L, T, X>(ListArgs);
+#define SC_LIST_INST(L, T, X)                                              \
+  template __global__ void k_list<kModeForces, L, T, X, true>(ListArgs);  \
+  template __global__ void k_list<kModeForces, L, T, X, false>(ListArgs); \
+  template __global__ void k_list<kModePred, L, T, X, false>(ListArgs);   \
+  template __global__ void k_list<kModeCorr, L, T, X, false>(ListArgs);
 #define SC_LIST_INST_LANES(T, X) \
   SC_LIST_INST(1, T, X)          \
   SC_LIST_INST(2, T, X)          \
