@@ -18,7 +18,7 @@ namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
 template <int MODE, bool TWO_POINT>
 __global__ void k_tile(TileArgs a);
-template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE>
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE, bool BOX>
 __global__ void k_list(ListArgs a);
 __global__ void k_reduce(ReduceArgs a);
 __global__ void k_segvec(Bank b, int ntot);
@@ -557,15 +557,18 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.pure = ps.pure;
     a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     using Fn = void (*)(ListArgs);
-#define SC_LIST_ROW(M, T, X, E)                                                   \
-  {&k_list<M, 1, T, X, E>, &k_list<M, 2, T, X, E>, &k_list<M, 4, T, X, E>, \
-   &k_list<M, 8, T, X, E>}
-#define SC_LIST_MODES(T, X)                                                          \
-  {SC_LIST_ROW(kModeForces, T, X, true), SC_LIST_ROW(kModePred, T, X, false),        \
-   SC_LIST_ROW(kModeCorr, T, X, false), SC_LIST_ROW(kModeForces, T, X, false)}
-    // [pair precision: FP64 only / mixed][contact model][mode][lanes]; mode 3
-    // is the force pass that records no stress / energy (minimiser passes)
-    static const Fn table[2][2][4][4] = {
+#define SC_LIST_ROW(M, T, X, E, B)                                                   \
+  {&k_list<M, 1, T, X, E, B>, &k_list<M, 2, T, X, E, B>, &k_list<M, 4, T, X, E, B>, \
+   &k_list<M, 8, T, X, E, B>}
+#define SC_LIST_MODES(T, X)                                                            \
+  {SC_LIST_ROW(kModeForces, T, X, true, false), SC_LIST_ROW(kModePred, T, X, false, false), \
+   SC_LIST_ROW(kModeCorr, T, X, false, false), SC_LIST_ROW(kModeForces, T, X, false, false), \
+   SC_LIST_ROW(kModePred, T, X, false, true), SC_LIST_ROW(kModeCorr, T, X, false, true)}
+    // [pair precision: FP64 only / mixed][contact model][mode][lanes].  Modes
+    // 0-2: force pass, sheared predictor, sheared corrector; 3: force pass
+    // that records no stress / energy (minimiser passes); 4, 5: predictor and
+    // corrector of steps that change the box (resize, pure shear)
+    static const Fn table[2][2][6][4] = {
         {SC_LIST_MODES(false, false), SC_LIST_MODES(true, false)},
         {SC_LIST_MODES(false, true), SC_LIST_MODES(true, true)}};
 #undef SC_LIST_MODES
@@ -573,7 +576,8 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
     const int xi = s->prec == SC_PREC_MIXED ? 1 : 0;
     const bool se = ps.se_always || ps.se_every > 0;
-    const int mi = ps.mode == kModeForces && !se ? 3 : ps.mode;
+    const bool box = ps.resize || ps.pure;
+    const int mi = ps.mode == kModeForces ? (se ? 0 : 3) : (box ? ps.mode + 3 : ps.mode);
     Fn fn = table[xi][a.two_point ? 1 : 0][mi][li];
     KernelTimer t(s, ps.kernel_id);
     a.block_idx = s->compact ? s->d_act_blocks : nullptr;

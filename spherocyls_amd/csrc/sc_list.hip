@@ -201,7 +201,10 @@ constexpr int kGather = 4;
 // (calc_se, recorded steps).  The minimiser's passes need only the residual
 // and FIRE sums, which are formed after the pair loop; their SE = false
 // kernel keeps no accumulators live across the loop.
-template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE>
+// BOX: the Heun passes change the box (resize, pure shear).  Sheared Heun
+// steps run the BOX = false instance, whose epilogue carries only the
+// Lees-Edwards path (corrector 114 VGPRs instead of 128 plus spills).
+template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE, bool BOX>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(MODE != kModePred && !MIXED ? 4 : 1))) k_list(ListArgs a) {
 #pragma clang fp contract(off)
@@ -232,25 +235,28 @@ __global__ void __launch_bounds__(256)
                      (a.se_always || (a.se_every > 0 && S.step > 0 && S.step % a.se_every == 0));
   const bool do_sum = kSums && (do_se || a.do_fire || a.do_gd || a.do_resid);
 
+  // box changes of this pass (compile-time absent in the BOX = false kernels)
+  const bool box_resize = BOX && a.resize, box_pure = BOX && a.pure;
+  const bool box_shrink = BOX && a.shrink, box_corrected_image = BOX && a.corrected_image;
   const double dt = P.dt, gd = P.strain_rate;
   double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
   double L_new_x = Lx, L_new_y = Ly;
-  if (MODE != kModeForces && a.resize) {
-    L_new_x = a.shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
-    L_new_y = a.shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
+  if (MODE != kModeForces && box_resize) {
+    L_new_x = box_shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
+    L_new_y = box_shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
   }
-  if (MODE != kModeForces && a.pure) {
+  if (MODE != kModeForces && box_pure) {
     // the box of pure_strain_step_2p (src/cuda/2point_dynamics.cu:2273-2274)
     L_new_x = Lx + 0.5 * dt * gd * Lx;
     L_new_y = Ly - 0.5 * dt * gd * Ly;
   }
   if (MODE == kModeCorr) {
-    if (a.pure) {  // forces of the predicted state in the new box, strain unchanged
+    if (box_pure) {  // forces of the predicted state in the new box, strain unchanged
       Lx = L_new_x;
       Ly = L_new_y;
     } else {
       gam = S.gamma + dt * gd;
-      if (a.resize && a.corrected_image) {
+      if (box_resize && box_corrected_image) {
         Lx = L_new_x;
         Ly = L_new_y;
       }
@@ -368,7 +374,7 @@ __global__ void __launch_bounds__(256)
     } else if (MODE == kModePred) {
       const double p0 = a.b.phi[g];
       double v0x, v0y, v0t, qx, qy, qp;
-      if (a.pure) {
+      if (box_pure) {
         // pure_euler_est_2p (src/cuda/2point_dynamics.cu:1885-1890), mobility 1:
         // the affine field (+x/2, -y/2) times the rate, and the rod's rotation
         // by the strain, -RC sin(2 phi) / 2 times the rate; v0 keeps the force
@@ -379,7 +385,7 @@ __global__ void __launch_bounds__(256)
         qx = xi + dt * (Fx + 0.5 * gd * xi);
         qy = yi + dt * (Fy - 0.5 * gd * yi);
         qp = p0 + dt * (v0t + gd * (-0.5 * a.b.RC[g] * sin(2.0 * p0)));
-      } else if (!a.resize) {
+      } else if (!box_resize) {
         // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
         v0x = Fx - S.gamma * Fy;
         v0y = Fy;
@@ -395,7 +401,7 @@ __global__ void __launch_bounds__(256)
         qx = xi + dt * v0x;
         qy = yi + dt * v0y;
         qp = p0 + dt * v0t / a.b.I[g];
-        if (a.shrink) {
+        if (box_shrink) {
           qx *= a.resize_scale;
           qy *= a.resize_scale;
         } else {
@@ -413,8 +419,8 @@ __global__ void __launch_bounds__(256)
       a.v0x[g] = v0x;
       a.v0y[g] = v0y;
       a.v0t[g] = v0t;
-      const double Lbx = (a.resize || a.pure) ? L_new_x : Lx;
-      const double Lby = (a.resize || a.pure) ? L_new_y : Ly;
+      const double Lbx = (box_resize || box_pure) ? L_new_x : Lx;
+      const double Lby = (box_resize || box_pure) ? L_new_y : Ly;
       double wx = qx, wy = qy;
       bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
       if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
@@ -424,12 +430,12 @@ __global__ void __launch_bounds__(256)
       const double gnext = S.gamma + dt * gd;
       raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
               fabs(min_image(qy - ybi, Lby)) >= half_skin ||
-              fabs(gnext - center) >= a.gamma_window || a.resize || a.pure;
+              fabs(gnext - center) >= a.gamma_window || box_resize || box_pure;
     } else {  // kModeCorr
       double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
       double Lbx = Lx, Lby = Ly;
       bool remap = false;
-      if (a.pure) {
+      if (box_pure) {
         // pure_heun_corr_2p (src/cuda/2point_dynamics.cu:2141-2166): the
         // predicted state plus half the change of the force part
         x1 = a.px[g] + 0.5 * dt * (Fx - a.v0x[g]);
@@ -437,7 +443,7 @@ __global__ void __launch_bounds__(256)
         p1 = a.pphi[g] + 0.5 * dt * (Tq / a.b.I[g] - a.v0t[g]);
         Lbx = L_new_x;
         Lby = L_new_y;
-      } else if (!a.resize) {
+      } else if (!box_resize) {
         // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
         double v1x = Fx - gam * Fy;
         double v1y = Fy;
@@ -464,7 +470,7 @@ __global__ void __launch_bounds__(256)
         x1 += 0.5 * dt * (a.v0x[g] + Fx);
         y1 += 0.5 * dt * (a.v0y[g] + Fy);
         p1 += 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
-        if (a.shrink) {
+        if (box_shrink) {
           x1 *= a.resize_scale;
           y1 *= a.resize_scale;
         } else {
@@ -484,9 +490,9 @@ __global__ void __launch_bounds__(256)
       a.b.nx[g] = Ai * cs;
       a.b.ny[g] = Ai * sn;
       bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
-      double gnext = a.pure ? S.gamma : S.gamma + gd * dt;
+      double gnext = box_pure ? S.gamma : S.gamma + gd * dt;
       if (remap) gnext -= 1.0;
-      raise = remap || a.resize || a.pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
+      raise = remap || box_resize || box_pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
               fabs(min_image(y1 - ybi, Lby)) >= half_skin || fabs(gnext - center) >= a.gamma_window;
     }
     if (raise && a.flag_out) a.flag_out[pk] = 1;
@@ -501,11 +507,11 @@ __global__ void __launch_bounds__(256)
     for (int w = 0; w < (int)(sizeof(PackScal) / sizeof(int)); ++w) dst[w] = src[w];
     PackScal& S2 = a.scal_next[pk];
     S2.step = S2.step + 1;
-    if (a.pure) {
+    if (box_pure) {
       S2.Lx = L_new_x;
       S2.Ly = L_new_y;
       S2.gamma_total = S2.gamma_total + dt * gd;
-    } else if (!a.resize) {
+    } else if (!box_resize) {
       double g2 = S2.gamma + gd * dt, gt = S2.gamma_total + gd * dt;
       if (g2 >= 0.5) {
         g2 -= 1.0;
@@ -526,11 +532,13 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-#define SC_LIST_INST(L, T, X)                                              \
-  template __global__ void k_list<kModeForces, L, T, X, true>(ListArgs);  \
-  template __global__ void k_list<kModeForces, L, T, X, false>(ListArgs); \
-  template __global__ void k_list<kModePred, L, T, X, false>(ListArgs);   \
-  template __global__ void k_list<kModeCorr, L, T, X, false>(ListArgs);
+#define SC_LIST_INST(L, T, X)                                                    \
+  template __global__ void k_list<kModeForces, L, T, X, true, false>(ListArgs);  \
+  template __global__ void k_list<kModeForces, L, T, X, false, false>(ListArgs); \
+  template __global__ void k_list<kModePred, L, T, X, false, false>(ListArgs);   \
+  template __global__ void k_list<kModeCorr, L, T, X, false, false>(ListArgs);   \
+  template __global__ void k_list<kModePred, L, T, X, false, true>(ListArgs);    \
+  template __global__ void k_list<kModeCorr, L, T, X, false, true>(ListArgs);
 #define SC_LIST_INST_LANES(T, X) \
   SC_LIST_INST(1, T, X)          \
   SC_LIST_INST(2, T, X)          \
