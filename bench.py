@@ -40,11 +40,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PRED = 88
 BYTES_CORR = 144
 # HBM bytes per corrector launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# (list path, 2 lanes per particle, 64-thread workgroups, median of the last
-# 100 launches after 1000 steps), scaled by a plain copy's readings
+# (list path, 2 lanes per particle, 256-thread workgroups, median of the 100
+# event-timed launches after 1000 steps), scaled by a plain copy's readings
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; profiles/k_list_heun_shear_N262144.md):
 # (workload, path, precision) -> bytes
-MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (29856.8 * 2.0 + 33936.9) * 1024}
+MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (23777.2 * 2.0 + 17984.2) * 1024}
 
 WORKLOADS = {
     "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
