@@ -77,8 +77,24 @@ def _load_oracle():
         lib.orc_fire_iter.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         lib.orc_frms.argtypes = [C.c_void_p]
         lib.orc_frms.restype = C.c_double
+        lib.orc_pair_contacts.argtypes = [C.c_int] + [C.c_double] * 12 + [_dp]
+        lib.orc_pair_contacts.restype = C.c_int
         _olib = lib
     return _olib
+
+
+def pair_contacts(p, q, model=1, L=1000.0, gamma=0.0):
+    """Contacts of one pair, p = (x, y, phi, A, R) as A and q as B, in an open box:
+    a list of dicts with s, t, D = (dx, dy), dist, the force (fx, fy) on p and
+    the torques tau_p, tau_q."""
+    lib = _load_oracle()
+    out = np.zeros(18)
+    xp, yp, pp, Ap, Rp = (float(v) for v in p)
+    xq, yq, pq, Aq, Rq = (float(v) for v in q)
+    k = lib.orc_pair_contacts(int(model), xp, yp, pp, Rp, Ap, xq, yq, pq, Rq, Aq, float(L),
+                              float(gamma), _ptr(out))
+    keys = ("s", "t", "dx", "dy", "dist", "fx", "fy", "tau_p", "tau_q")
+    return [dict(zip(keys, out[9 * j:9 * j + 9])) for j in range(k)]
 
 
 def ref_available() -> bool:

@@ -1010,6 +1010,28 @@ int orc_brute_contacts(orc_box* b, int* p, int* q, int cap) {
   return k;
 }
 
+// Contacts of one pair (p = A, q = B) under contact model `model` in an open
+// square box of side L at strain gamma: out[9 k .. 9 k + 8] = s t Dx Dy dist fx
+// fy tau_p tau_q of contact k (the force acts on p).  Returns the count.
+int orc_pair_contacts(int model, double xp, double yp, double php, double Rp, double Ap,
+                      double xq, double yq, double phq, double Rq, double Aq, double L,
+                      double gamma, double* out) {
+  PairOut pc[2];
+  int k;
+  if (model == 1) {
+    k = pair_eval_2p(xp, yp, php, Rp, Ap, xq, yq, phq, Rq, Aq, L, L, gamma, pc);
+  } else {
+    pc[0] = pair_eval(xp, yp, php, Rp, Ap, xq, yq, phq, Rq, Aq, L, L, gamma);
+    k = pc[0].contact ? 1 : 0;
+  }
+  for (int j = 0; j < k; ++j) {
+    const PairOut& o = pc[j];
+    const double v[9] = {o.s, o.t, o.dx, o.dy, o.dist, o.fx, o.fy, o.tau_p, o.tau_q};
+    for (int i = 0; i < 9; ++i) out[9 * j + i] = v[i];
+  }
+  return k;
+}
+
 double orc_frms(orc_box* b) {
   forces_main(b);
   return residual_rms(b);

@@ -59,6 +59,11 @@ int orc_fire(orc_box* b, const double* params, double* frms_out);
 // fire_state[0..3] = dt alpha npos converged (updated in place).
 void orc_fire_iter(orc_box* b, const double* params, double* vx, double* vy, double* vpsi,
                    double* fire_state);
+// One pair's contacts (model 0: closest point, 1: two-point) in an open box of
+// side L: out[9 k ..] = s t Dx Dy dist fx fy tau_p tau_q; returns the count.
+int orc_pair_contacts(int model, double xp, double yp, double php, double Rp, double Ap,
+                      double xq, double yq, double phq, double Rq, double Aq, double L,
+                      double gamma, double* out);
 double orc_frms(orc_box* b);  // sqrt(sum(Fx^2+Fy^2+tau^2)/(3N)) at the current state
 
 #ifdef __cplusplus

@@ -141,3 +141,43 @@ def test_two_point_fire_jammed_stress(precision):
     for k in ("E", "cPxx", "cPyy", "Pxx", "Pyy"):
         assert bx[k] == pytest.approx(o[k], rel=1e-8), k
     assert bx["cPxy"] == pytest.approx(o["cPxy"], rel=1e-8, abs=1e-8 * abs(o["cPxx"]))
+
+
+@pytest.mark.parametrize("path", ["list", "tile"])
+def test_second_contact_branch_kats(path):
+    """The hand-derived two-body cases of every second-contact branch
+    (tests/test_two_point_cpu.py) as one batch of 2-rod packings: forces,
+    torques and energy of each packing against the oracle's contacts."""
+    from test_two_point_cpu import TWO_POINT_KATS
+
+    pks = []
+    for name, p, q, _ in TWO_POINT_KATS:
+        pts = [(p[0] + 10.0, p[1] + 10.0) + p[2:], (q[0] + 10.0, q[1] + 10.0) + q[2:]]
+        x, y, phi, A, R = (np.array(v, dtype=float) for v in zip(*pts))
+        pks.append(configs.Packing(40.0, x, y, phi, A, R, 0.0, name))
+    print("two-point branch KATs", path, len(pks), flush=True)
+    s = native.System(pks, pads=0.1)
+    try:
+        s.set_path(path)
+        s.set_contact_model("two_point")
+        s.calc_se()
+        fx, fy, tau = s.forces()
+        boxes = [s.box(k) for k in range(len(pks))]
+    finally:
+        s.close()
+    for k, pk in enumerate(pks):
+        a = (pk.x[0], pk.y[0], pk.phi[0], pk.A[0], pk.R[0])
+        b = (pk.x[1], pk.y[1], pk.phi[1], pk.A[1], pk.R[1])
+        cs = oracle.pair_contacts(a, b, model=1, L=40.0)
+        assert len(cs) == 2, pk.name
+        scale = sum(abs(c["fx"]) + abs(c["fy"]) for c in cs)
+        want = (sum(c["fx"] for c in cs), sum(c["fy"] for c in cs))
+        assert abs(fx[2 * k] - want[0]) <= TOL * scale, pk.name
+        assert abs(fy[2 * k] - want[1]) <= TOL * scale, pk.name
+        assert abs(fx[2 * k + 1] + want[0]) <= TOL * scale, pk.name
+        tq = sum(c["tau_q"] for c in cs)
+        tp = sum(c["tau_p"] for c in cs)
+        tsc = sum(abs(c["tau_p"]) + abs(c["tau_q"]) for c in cs)
+        assert abs(tau[2 * k] - tp) <= TOL * tsc and abs(tau[2 * k + 1] - tq) <= TOL * tsc, pk.name
+        E = sum((1 - c["dist"]) ** 2 for c in cs) / 4
+        assert boxes[k]["E"] == pytest.approx(E, rel=TOL), pk.name

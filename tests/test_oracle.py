@@ -495,3 +495,62 @@ def test_fire_alpha_one_is_gradient_flow():
     dt = params[0]
     np.testing.assert_allclose(wrap_diff(p1["x"], p0["x"], pk.L), dt * dt * p0["fx"], atol=1e-15)
     np.testing.assert_allclose(p1["phi"] - p0["phi"], dt * dt * p0["tau"] / p0["I"], atol=1e-15)
+
+
+def _fire_start(n=32, phi=0.84, seed=47):
+    R, A = configs.bidisperse(n)
+    pk = configs.make_packing(R, A, phi, seed)
+    b = oracle.OracleBox.from_packing(pk, 0.1)
+    b.calc_forces()
+    return pk, b, b.particles()
+
+
+def test_fire_uphill_step_retreats_half_a_step():
+    """P = g.v < 0: FIRE 2.0's retreat first moves every coordinate back by
+    dt v / 2 (dt the step that produced v), then restarts from v = 0 with the
+    halved step: v = dt' g, displacement dt' v = dt'^2 g.  With v = -0.3 g the
+    new position is x0 + (0.15 dt + dt'^2) g, and psi = phi sqrt(I) likewise
+    (tau / sqrt(I) is its generalised force)."""
+    pk, b, p0 = _fire_start()
+    sq = np.sqrt(p0["I"])
+    g = [p0["fx"], p0["fy"], p0["tau"] / sq]
+    v = [-0.3 * gi for gi in g]
+    dt = 0.04
+    st = np.array([dt, 0.07, 3.0, 0.0])
+    b.fire_iter(FIRE, *[vi.copy() for vi in v], st)
+    p1 = b.particles()
+    dtn = dt * FIRE[4]
+    assert st[0] == dtn and st[1] == FIRE[5] and st[2] == 0
+    assert np.max(dtn * dtn * np.hypot(g[0], g[1])) < FIRE[7]  # no clamp in this step
+    # (coordinates of ~10 carry ~2e-15 of rounding; the retreat term is ~1e-3)
+    np.testing.assert_allclose(wrap_diff(p1["x"], p0["x"], pk.L), (0.15 * dt + dtn * dtn) * g[0],
+                               rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(wrap_diff(p1["y"], p0["y"], pk.L), (0.15 * dt + dtn * dtn) * g[1],
+                               rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose((p1["phi"] - p0["phi"]) * sq, (0.15 * dt + dtn * dtn) * g[2],
+                               rtol=1e-9, atol=1e-13)
+
+
+def test_fire_displacement_is_clamped_to_max_move():
+    """A step whose translation dt v is longer than max_move moves the particle
+    by exactly max_move along dt v; the rotation is not clamped.  From v = 0
+    with a large step dt, v = dt g and the translation is dt^2 g."""
+    pk, b, p0 = _fire_start()
+    sq = np.sqrt(p0["I"])
+    g = [p0["fx"], p0["fy"], p0["tau"] / sq]
+    v = [np.zeros(pk.n) for _ in range(3)]
+    dt = 0.2
+    st = np.array([dt, FIRE[5], 0.0, 0.0])
+    b.fire_iter(FIRE, *v, st)
+    p1 = b.particles()
+    dxy = np.hypot(g[0], g[1]) * dt * dt
+    big, small = dxy > FIRE[7], (dxy > 0) & (dxy < FIRE[7])
+    assert big.sum() >= 4 and small.sum() >= 4
+    mx = wrap_diff(p1["x"], p0["x"], pk.L)
+    my = wrap_diff(p1["y"], p0["y"], pk.L)
+    np.testing.assert_allclose(np.hypot(mx, my)[big], FIRE[7], rtol=1e-10)
+    # direction along g
+    np.testing.assert_allclose((mx * g[1] - my * g[0])[big], 0.0, atol=1e-13)
+    assert np.all(mx[big] * g[0][big] + my[big] * g[1][big] > 0)
+    np.testing.assert_allclose(mx[small], dt * dt * g[0][small], rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose((p1["phi"] - p0["phi"]) * sq, dt * dt * g[2], rtol=1e-9, atol=1e-13)

@@ -149,3 +149,123 @@ def test_fire_converges_where_closest_point_stalls(aspect, phi, seed):
     # the relaxed two-point state is overlap-free here (below jamming)
     b2.calc_se()
     assert b2.scalars()["E"] < 1e-18
+
+
+# ---------------------------------------------------------------------------
+# Known answers for each non-parallel branch of the second-contact rule
+# (src/cuda/2point_dynamics.cu:412-456), derived by hand.
+#
+# Geometry: rod A centred at the origin along x with half-length a, so
+# A(s) = (s a, 0); rod B centred at (X, Y) at angle th with half-length b,
+# B(t) = (X, Y) + t b (cos th, sin th).  The foot of A(s) on B's line is
+# t = ((s a - X) cos th - Y sin th) / b, the foot of B(t) on A's line is
+# s = (X + t b cos th) / a; each is clamped to [-1, 1] when it falls outside.
+# Contact 1 is the closest pair of points; contact 2 is the other end of the
+# overlap: of the two far ends (A's or B's) the one whose foot lies closer.
+# All radii are 0.5 (sigma = 1).  Each case states which end wins and why; the
+# expected (s, t) are closed forms of those choices.
+# ---------------------------------------------------------------------------
+C1, S1 = math.cos(0.1), math.sin(0.1)
+C2, S2 = math.cos(0.2), math.sin(0.2)
+
+TWO_POINT_KATS = [
+    # 1. Branch sarg >= 1, targ <= 1, |s3| < 1, b < 0 (rods aligned).  a = 1,
+    #    B: (2, 0.2), th = 0.2, b = 3.  Closest: A's end s = 1 and its foot on B.
+    #    Far ends: A's s = -1 (foot t = (-3 C2 - 0.2 S2) / 3 inside B,
+    #    distance |0.2 C2 - 3 S2| = 0.39999) and B's t = -1 (point
+    #    (2 - 3 C2, 0.2 - 3 S2), foot s = 2 - 3 C2 inside A, distance
+    #    3 S2 - 0.2 = 0.39601).  B's end is closer.
+    ("sarg>=1 third taken, aligned", (0.0, 0.0, 0.0, 1.0, 0.5), (2.0, 0.2, 0.2, 3.0, 0.5),
+     [(1.0, (-C2 - 0.2 * S2) / 3.0), (2.0 - 3.0 * C2, -1.0)]),
+    # 2. The mirror image x -> -x of case 1 (b > 0, rods anti-aligned):
+    #    B: (-2, 0.2), th = pi - 0.2.  Closest: A's end s = -1; B's end t = -1
+    #    now lies at (-2 + 3 C2, 0.2 - 3 S2), foot s = 3 C2 - 2.
+    ("sarg>=1 third taken, anti-aligned", (0.0, 0.0, 0.0, 1.0, 0.5),
+     (-2.0, 0.2, math.pi - 0.2, 3.0, 0.5),
+     [(-1.0, (-C2 - 0.2 * S2) / 3.0), (3.0 * C2 - 2.0, -1.0)]),
+    # 3. Branch sarg < 1, |s2| <= 1, |t3| < 1, b < 0.  a = 1, B: (0, 0.2),
+    #    th = 0.1, b = 1.  Closest: B's end t = -1 at (-C1, 0.2 - S1), foot
+    #    s = -C1.  Far ends: B's t = 1 at (C1, 0.2 + S1), foot s = C1, distance
+    #    0.2 + S1 = 0.29983; A's s = 1, foot t = C1 - 0.2 S1 inside B, distance
+    #    S1 + 0.2 C1 = 0.29883.  A's end is closer.
+    ("sarg<1 third taken, aligned", (0.0, 0.0, 0.0, 1.0, 0.5), (0.0, 0.2, 0.1, 1.0, 0.5),
+     [(-C1, -1.0), (1.0, C1 - 0.2 * S1)]),
+    # 4. Mirror image of case 3 (b > 0): B at angle pi - 0.1.  Closest: B's
+    #    end t = -1 at (C1, 0.2 - S1), foot s = C1; A's end s = -1 wins, foot
+    #    t = C1 - 0.2 S1.
+    ("sarg<1 third taken, anti-aligned", (0.0, 0.0, 0.0, 1.0, 0.5),
+     (0.0, 0.2, math.pi - 0.1, 1.0, 0.5),
+     [(C1, -1.0), (-1.0, C1 - 0.2 * S1)]),
+    # 5. A third candidate that loses (branch sarg >= 1, targ <= 1).  a = 1,
+    #    B: (-0.5, 0.2), th = 0.1, b = 1.5.  Closest: A's end s = -1, foot
+    #    t = (-0.5 C1 - 0.2 S1) / 1.5.  Far ends: A's s = 1, foot
+    #    t = (1.5 C1 - 0.2 S1) / 1.5 inside B, distance 1.5 S1 + 0.2 C1 = 0.34875;
+    #    B's t = 1 at (-0.5 + 1.5 C1, 0.2 + 1.5 S1), distance 0.34975.  A's
+    #    end is closer, so the second contact stays at A's end.
+    ("sarg>=1 third loses", (0.0, 0.0, 0.0, 1.0, 0.5), (-0.5, 0.2, 0.1, 1.5, 0.5),
+     [(-1.0, (-0.5 * C1 - 0.2 * S1) / 1.5), (1.0, (1.5 * C1 - 0.2 * S1) / 1.5)]),
+    # 6. Branch sarg >= 1, targ > 1 (A's other end projects past B, so B's end
+    #    t = 1 is used with its foot on A).  a = 1, B: (-1.75, 0.2), th = 0.1,
+    #    b = 1.  Closest: A's end s = -1, foot t = 0.75 C1 - 0.2 S1.  A's s = 1
+    #    has foot 2.75 C1 - 0.2 S1 > 1: B's end t = 1 at (-1.75 + C1, 0.2 + S1),
+    #    foot s = C1 - 1.75.
+    ("sarg>=1 targ>1", (0.0, 0.0, 0.0, 1.0, 0.5), (-1.75, 0.2, 0.1, 1.0, 0.5),
+     [(-1.0, 0.75 * C1 - 0.2 * S1), (C1 - 1.75, 1.0)]),
+    # 7. Branch sarg < 1, |s2| > 1 (B's other end projects past A, so A's end
+    #    s = -1 is used with its foot on B).  a = 1, B: (-1.75, -0.3), th = 0.1,
+    #    b = 1.  Closest: B's end t = 1 at (-1.75 + C1, -0.3 + S1), foot
+    #    s = C1 - 1.75.  B's t = -1 has foot -1.75 - C1 < -1: A's end s = -1,
+    #    foot t = 0.75 C1 + 0.3 S1.
+    ("sarg<1 s2 past the end", (0.0, 0.0, 0.0, 1.0, 0.5), (-1.75, -0.3, 0.1, 1.0, 0.5),
+     [(C1 - 1.75, 1.0), (-1.0, 0.75 * C1 + 0.3 * S1)]),
+]
+
+
+def _expected_contact(p, q, s, t):
+    """Harmonic contact at parameters (s, t) of pair (p = A, q = B): D, force
+    on p and both torques, from the force law the golden vectors pin."""
+    xp, yp, pp, Ap, Rp = p
+    xq, yq, pq, Aq, Rq = q
+    nA = np.array([Ap * math.cos(pp), Ap * math.sin(pp)])
+    nB = np.array([Aq * math.cos(pq), Aq * math.sin(pq)])
+    D = np.array([xp, yp]) + s * nA - np.array([xq, yq]) - t * nB
+    dist = float(np.hypot(*D))
+    sig = Rp + Rq
+    F = (1.0 - dist / sig) / sig * D / dist
+    return {"s": s, "t": t, "dx": D[0], "dy": D[1], "dist": dist, "fx": F[0], "fy": F[1],
+            "tau_p": s * (nA[0] * F[1] - nA[1] * F[0]),
+            "tau_q": t * (nB[1] * F[0] - nB[0] * F[1])}
+
+
+@pytest.mark.parametrize("case", TWO_POINT_KATS, ids=[c[0] for c in TWO_POINT_KATS])
+def test_kat_second_contact_branches(case):
+    _, p, q, want = case
+    # the same pair translated away from the box edges
+    pt = (p[0] + 10.0, p[1] + 10.0) + p[2:]
+    qt = (q[0] + 10.0, q[1] + 10.0) + q[2:]
+    got = oracle.pair_contacts(pt, qt, model=1, L=40.0)
+    assert len(got) == 2
+    for g, (s, t) in zip(got, want):
+        e = _expected_contact(pt, qt, s, t)
+        for k in ("s", "t", "dx", "dy", "dist", "fx", "fy", "tau_p", "tau_q"):
+            assert g[k] == pytest.approx(e[k], rel=1e-12, abs=1e-13), k
+    # the box sums agree with the two contacts
+    pp, sc = two_body(pt, qt)
+    fx = sum(c["fx"] for c in got)
+    assert pp["fx"][0] == pytest.approx(fx, rel=1e-13, abs=1e-15)
+    assert pp["tau"][1] == pytest.approx(sum(c["tau_q"] for c in got), rel=1e-13, abs=1e-15)
+    assert sc["E"] == pytest.approx(sum((1 - c["dist"]) ** 2 for c in got) / 4, rel=1e-12)
+
+
+def test_kat_antiparallel_partial_overlap():
+    # The parallel-overlap case above with q reversed (phi = pi, b = +A_p A_q):
+    # q(t) = (10.5 - t, 10.8), so the overlap [9.5, 11] now ends at q's t = 1
+    # (with p's s = -1/2) and at p's s = 1 (with q's t = -1/2).
+    p, q = (10.0, 10.0, 0.0, 1.0, 0.5), (10.5, 10.8, math.pi, 1.0, 0.5)
+    got = oracle.pair_contacts(p, q, model=1, L=40.0)
+    assert len(got) == 2
+    for g, (s, t) in zip(got, [(-0.5, 1.0), (1.0, -0.5)]):
+        e = _expected_contact(p, q, s, t)
+        for k in ("s", "t", "dist", "fx", "fy", "tau_p", "tau_q"):
+            assert g[k] == pytest.approx(e[k], rel=1e-12, abs=1e-13), k
+    assert got[0]["fy"] == pytest.approx(-0.2, rel=1e-12)
