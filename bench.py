@@ -64,29 +64,29 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(pk, strain_rate, dt, budget_s=15.0):
-    """Reference CPU path (oracle/_ref, else the oracle restatement), 1 thread,
-    on a bounded number of steps of the same packing."""
+def cpu_baseline(pk, gamma, strain_rate, dt, steps=200, runs=3):
+    """Reference CPU path (oracle/_ref: the reference's own src/cpp compiled
+    here; else the oracle restatement), 1 thread, on the same relaxed state the
+    GPU line is timed from: `runs` runs of `steps` Heun steps each, each from
+    that state; the median run's rate."""
     import oracle
 
     kind = "reference" if oracle.ref_available() else "port"
     cls = oracle.RefBox if kind == "reference" else oracle.OracleBox
-    b = cls.from_packing(pk, 0.1)
-    b.set_params(0.0, 0.0, strain_rate, 0.0, dt)
-    b.calc_forces()
-    b.run_steps(1, 0)
-    steps, elapsed = 0, 0.0
-    chunk = 1
-    while elapsed < budget_s:
+    rates = []
+    for _ in range(runs):
+        b = cls.from_packing(pk, 0.1)
+        b.set_params(gamma, 0.0, strain_rate, 0.0, dt)
+        b.find_neighbors()
+        b.calc_forces()
         t0 = time.perf_counter()
-        b.run_steps(chunk, 0)
-        elapsed += time.perf_counter() - t0
-        steps += chunk
-        chunk = min(chunk * 2, 64)
-    return {"value": pk.n * steps / elapsed, "unit": "particle-steps/s", "cores": 1,
-            "kind": kind,
-            "sample": f"{steps} Heun steps of the same N={pk.n} packing on 1 core "
-                      f"({cpu_model()}, nproc={os.cpu_count()})"}
+        b.run_steps(steps, 0)
+        rates.append(pk.n * steps / (time.perf_counter() - t0))
+        del b
+    return {"value": float(np.median(rates)), "unit": "particle-steps/s", "cores": 1,
+            "kind": kind, "runs": [float(r) for r in rates],
+            "sample": f"median of {runs} runs of {steps} Heun steps of the same relaxed "
+                      f"N={pk.n} packing, 1 thread ({cpu_model()}, nproc={os.cpu_count()})"}
 
 
 # FIRE, per particle-iteration, non-fused force + update passes: 112 B for a
@@ -283,6 +283,11 @@ def main():
     ap.add_argument("--lanes", type=int, default=1, help="list path: lanes per particle")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=200,
+                    help="CPU baseline: Heun steps per run (from the relaxed state)")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs (median)")
+    ap.add_argument("--relax-steps", type=int, default=2000,
+                    help="untimed Heun steps in setup before every timed leg")
     ap.add_argument("--no-minimization", action="store_true",
                     help="skip the packings/s minimisation measurement")
     ap.add_argument("--packings", type=int, default=256, help="minimisation ensemble size")
@@ -306,11 +311,23 @@ def main():
         sys.exit(2)
 
     make, strain_rate, dt = WORKLOADS[args.workload]
-    pk = configs.shear_large(1, n=args.n) if args.n and args.workload == "shear_large" else make()
-    s = native.System(pk, pads=0.1)
+    pk0 = configs.shear_large(1, n=args.n) if args.n and args.workload == "shear_large" else make()
+    s = native.System(pk0, pads=0.1)
     s.set_dynamics(strain_rate=strain_rate, step=dt)
     if args.precision == "mixed":
         s.set_precision("mixed")
+    # The seeded placement only avoids crossings: its overlaps relax over the
+    # first steps, which move particles fast and rebuild the neighbour rows
+    # often.  Setup runs those steps untimed; every timed leg (GPU, host to
+    # host, CPU baseline) starts from the relaxed state.
+    t0 = time.perf_counter()
+    if args.relax_steps:
+        s.strain_steps(args.relax_steps)
+    relax_s = time.perf_counter() - t0
+    xr, yr, pr = s.positions()
+    b0 = s.box()
+    gamma_r = b0["gamma"]
+    pk = configs.Packing(pk0.L, xr, yr, pr, pk0.A, pk0.R, pk0.packing_fraction, pk0.name)
     if args.path == "auto":
         tuned = s.autotune(reps=10)  # the project's autotuner: path and launch shape
     else:
@@ -345,6 +362,7 @@ def main():
     t0 = time.perf_counter()
     s2 = native.System(pk, pads=0.1)
     try:
+        s2.set_gamma(gamma_r, 0.0)
         s2.set_dynamics(strain_rate=strain_rate, step=dt)
         if args.precision == "mixed":
             s2.set_precision("mixed")
@@ -374,15 +392,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-sum",
-        "data": "synthetic (seeded non-crossing random packing)",
+        "data": f"synthetic (seeded non-crossing random packing, relaxed by {args.relax_steps} "
+                "untimed Heun steps in setup)",
         "config": {"workload": pk.name, "n_particles": pk.n, "strain_rate": strain_rate, "dt": dt,
-                   "cell_pad": 0.1, "force_path": path,
+                   "cell_pad": 0.1, "force_path": path, "relax_steps": args.relax_steps,
+                   "relax_s": relax_s,
                    ("tile_cols" if path == "tile" else "lanes_per_particle"): shape,
                    "block_threads": block, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if args.n else MEASURED_TRAFFIC.get((args.workload, path,
                                                                              args.precision)),
+                     "traffic_source": "steady-state constant: rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                                       "per corrector launch after the relaxation, scaled by a "
+                                       "plain copy's readings (profiles/), not this run",
                      "kernel": f"k_{path}<corrector>", "avg_us": corr_avg * 1e6,
                      "predictor_avg_us": pred_avg * 1e6,
                      "bytes_per_particle": BYTES_CORR},
@@ -392,7 +415,8 @@ def main():
                                      "lists), the same steps, download of the positions"},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(pk, strain_rate, dt)
+        out["cpu_baseline"] = cpu_baseline(pk, gamma_r, strain_rate, dt, steps=args.cpu_steps,
+                                           runs=args.cpu_runs)
     if not args.no_minimization:
         out["minimization"] = minimization(args.packings, 1,
                                            0.0 if args.no_cpu_baseline else 15.0)

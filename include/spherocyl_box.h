@@ -145,6 +145,9 @@ class SpherocylBox {
   // flip_shear_direction (src/cuda/find_neighbors.cu:565-579): x -> L - x,
   // phi -> -phi, gamma -> -gamma
   void flip_shear_direction();
+  // rotate_by_gamma (src/cuda/find_neighbors.cu:624-645): rotate the frame so
+  // the sheared box vector is the new x axis; the box becomes non-square
+  void rotate_by_gamma();
   long unsigned int compress_qs2p(double dMaxPacking, double dResizeStep);
   long unsigned int expand_qs2p(double dMinPacking, double dResizeStep);
   long unsigned int find_jam_2p(double dJamE, double dResizeStep);

@@ -36,6 +36,8 @@ extern "C" {
 #define SC_ERR_LIST_OVERFLOW 7  /* more than the per-particle candidate list holds */
 #define SC_ERR_NOT_CONVERGED 8  /* minimiser hit its iteration limit */
 #define SC_ERR_STATE 9          /* call not valid in the current state */
+#define SC_ERR_HOST 10          /* host-side failure: out of host memory or an internal
+                                   error (message in sc_last_error) */
 
 /* precision modes (sc_set_precision) */
 #define SC_PREC_FP64 0  /* all pair math in FP64 */
@@ -179,6 +181,14 @@ int sc_pure_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out)
 /* flip_shear_direction (src/cuda/find_neighbors.cu:552-579): x -> Lx - x,
  * phi -> -phi, gamma -> -gamma for packing k (k = -1: all). */
 int sc_flip_shear(sc_system* s, int k);
+/* rotate_by_gamma (src/cuda/find_neighbors.cu:580-645): a rigid rotation of the
+ * lab frame that takes the sheared box vector (gamma Ly, Ly) to the new x axis.
+ * Positions x' = sgn(gamma) y sqrt(1+gamma^2), y' = -sgn(gamma) x / sqrt(1+gamma^2)
+ * (wrapped), phi' = phi - atan(1/gamma) (pi/2 at gamma = 0), box
+ * Lx' = Ly sqrt(1+gamma^2), Ly' = Lx / sqrt(1+gamma^2), strain -gamma, for
+ * packing k (k = -1: all).  Later shear steps remap at gamma = Lx/(2 Ly) with
+ * x += (Lx/Ly) y (the general Lees-Edwards rule). */
+int sc_rotate_by_gamma(sc_system* s, int k);
 /* gradient_descent_minimize loop (src/cpp/relaxation_dynamics.cpp:33-85,
  * gradient_relax_step :11-31): at most nmax steps, stopping when E <= emin. */
 int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done);

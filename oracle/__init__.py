@@ -56,7 +56,7 @@ def _load_oracle():
         lib.orc_new.argtypes = [C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double]
         for name in ("orc_free", "orc_find_neighbors", "orc_calc_forces", "orc_calc_se",
                      "orc_strain_step", "orc_gd_step", "orc_set_back_gamma", "orc_pure_step",
-                     "orc_flip_shear"):
+                     "orc_flip_shear", "orc_rotate_by_gamma"):
             getattr(lib, name).argtypes = [C.c_void_p]
             getattr(lib, name).restype = None
         lib.orc_set_params.argtypes = [C.c_void_p] + [C.c_double] * 5
@@ -191,6 +191,7 @@ class OracleBox(_BoxBase):
     def set_back_gamma(self): self.lib.orc_set_back_gamma(self.h)
     def pure_step(self): self.lib.orc_pure_step(self.h)
     def flip_shear(self): self.lib.orc_flip_shear(self.h)
+    def rotate_by_gamma(self): self.lib.orc_rotate_by_gamma(self.h)
 
     def resize_step(self, shrink=True, corrected_image=False):
         self.lib.orc_resize_step(self.h, int(shrink), int(corrected_image))

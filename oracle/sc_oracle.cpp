@@ -17,6 +17,7 @@
 //       step that keeps the velocity, reduces to gradient descent for alpha=1)
 #include "sc_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -72,9 +73,13 @@ inline void wrap_angle(double& p) {
   while (p < -kPi) p += 2 * kPi;
 }
 
-// configure_cells, spherocyl_box.cpp:223-268
+// configure_cells, spherocyl_box.cpp:223-268.  For Lx != Ly (which the CPU
+// constructors never produce) the column width grows to cover the remap strain
+// Lx / (2 Ly): 2 sqrt(1 + (Lx / 2 Ly)^2) (Rmax + Amax) + pad when above the
+// reference's 2.24 (Rmax + Amax) + pad.
 void configure_cells(orc_box* b) {
-  double wmin = 2.24 * (b->Rmax + b->Amax) + b->pad;
+  const double r = b->Lx / b->Ly;
+  double wmin = std::max(2.24, 2.0 * std::sqrt(1.0 + 0.25 * r * r)) * (b->Rmax + b->Amax) + b->pad;
   double hmin = 2 * (b->Rmax + b->Amax) + b->pad;
   Cells& c = b->cg;
   c.rows = std::max(static_cast<int>(b->Ly / hmin), 1);
@@ -125,7 +130,9 @@ bool candidate(const orc_box* b, int p, int q) {
   if (std::fabs(dy) < reach + b->pad) {
     double dx = b->x[p] - b->x[q];
     dx += b->Lx * ((dx < -0.5 * b->Lx) - (dx > 0.5 * b->Lx));
-    double dx_max_shear = dx + 0.5 * dy;
+    // the largest strain before the remap: 0.5 (Lx / Ly), which is the
+    // reference's 0.5 in its square boxes
+    double dx_max_shear = dx + (0.5 * (b->Lx / b->Ly)) * dy;
     dx += b->gamma * dy;
     if (std::fabs(dx) < reach + b->pad || std::fabs(dx_max_shear) < reach + b->pad) return true;
   }
@@ -537,18 +544,24 @@ inline bool moved_too_far(const orc_box* b, int i) {
          b->ym[i] < -0.5 * b->pad;
 }
 
-// set_back_gamma, spherocyl_box.cpp:524-545 (Lx = Ly as the constructors enforce)
+// set_back_gamma, spherocyl_box.cpp:524-545, in the general form of the GPU
+// code for Lx != Ly (set_back_coords, src/cuda/find_neighbors.cu:320-330, and
+// :369): x += (Lx/Ly) y and gamma -= Lx/Ly, a lattice translation of the
+// sheared box.  With Lx = Ly (the CPU constructors) the ratio is exactly 1 and
+// this is the CPU code's x += y, gamma -= 1; gamma_total is re-synchronised
+// as the CPU code does only then.
 void lees_edwards_remap(orc_box* b) {
+  const double ratio = b->Lx / b->Ly;
   for (int i = 0; i < b->n; ++i) {
     if (b->y[i] < 0) b->y[i] += b->Ly;
     else if (b->y[i] > b->Ly) b->y[i] -= b->Ly;
-    b->x[i] += b->y[i];
+    b->x[i] += ratio * b->y[i];
     if (b->x[i] < 0) b->x[i] += b->Lx;
     else
       while (b->x[i] > b->Lx) b->x[i] -= b->Lx;
   }
-  b->gamma -= 1.0;
-  b->gamma_total = int(b->gamma_total + 1) + b->gamma;
+  b->gamma -= ratio;
+  if (ratio == 1.0) b->gamma_total = int(b->gamma_total + 1) + b->gamma;
   build_neighbors(b);
 }
 
@@ -583,7 +596,8 @@ void heun_step(orc_box* b) {
   }
   b->gamma += b->strain_rate * b->dt;
   b->gamma_total += b->strain_rate * b->dt;
-  if (b->gamma >= 0.5) lees_edwards_remap(b);
+  // the remap threshold 0.5 Lx/Ly (src/cuda/strain_dynamics.cu:712); 0.5 when Lx = Ly
+  if (b->gamma >= 0.5 * (b->Lx / b->Ly)) lees_edwards_remap(b);
 }
 
 // Pure shear: pure_strain_step_2p of the reference's GPU code
@@ -645,6 +659,33 @@ void flip_shear(orc_box* b) {
 
 // reconfigure_cells, resize_dynamics.cpp:106-121 (the reference compares rows
 // with Lx/hmin and cols with Ly/wmin; identical for Lx = Ly).
+// rotate_by_gamma / gamma_rotate, src/cuda/find_neighbors.cu:580-645: the lab
+// frame turns so that the sheared box vector (gamma Ly, Ly) is the new x axis.
+void rotate_by_gamma(orc_box* b) {
+  const double g = b->gamma, root = std::sqrt(1 + g * g);
+  const double Lx = b->Lx, Ly = b->Ly;
+  b->Lx = Ly * root;
+  b->Ly = Lx / root;
+  for (int i = 0; i < b->n; ++i) {
+    const double x = b->x[i], y = b->y[i];
+    double nx = (1 - 2 * std::signbit(g)) * y * root;
+    double ny = (2 * std::signbit(g) - 1) * x / root;
+    double np = g != 0.0 ? b->phi[i] - std::atan(1 / g) : b->phi[i] - kPi / 2;
+    if (np > kPi) np -= 2 * kPi;
+    else if (np < -kPi) np += 2 * kPi;
+    if (ny > b->Ly) ny -= b->Ly;
+    else if (ny < 0) ny += b->Ly;
+    if (nx < 0) nx += b->Lx;
+    else if (nx > b->Lx) nx -= b->Lx;
+    b->x[i] = nx;
+    b->y[i] = ny;
+    b->phi[i] = np;
+  }
+  b->gamma = -g;
+  configure_cells(b);
+  build_neighbors(b);
+}
+
 void reconfigure_cells(orc_box* b) {
   double wmin = 2.24 * (b->Rmax + b->Amax) + b->pad;
   double hmin = 2 * (b->Rmax + b->Amax) + b->pad;
@@ -908,6 +949,7 @@ void orc_resize_step(orc_box* b, int shrink, int corrected_image) {
 void orc_gd_step(orc_box* b) { gd_step(b); }
 void orc_pure_step(orc_box* b) { pure_step(b); }
 void orc_flip_shear(orc_box* b) { flip_shear(b); }
+void orc_rotate_by_gamma(orc_box* b) { rotate_by_gamma(b); }
 void orc_set_back_gamma(orc_box* b) { lees_edwards_remap(b); }
 
 int orc_run_steps(orc_box* b, int nsteps, int se_every, double* se_out) {

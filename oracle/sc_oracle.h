@@ -32,6 +32,7 @@ void orc_resize_step(orc_box* b, int shrink, int corrected_image);  // resize_dy
 void orc_gd_step(orc_box* b);          // relaxation_dynamics.cpp:11-31
 void orc_pure_step(orc_box* b);        // src/cuda/2point_dynamics.cu:1663-2285, CPU conventions
 void orc_flip_shear(orc_box* b);       // src/cuda/find_neighbors.cu:552-579
+void orc_rotate_by_gamma(orc_box* b);  // src/cuda/find_neighbors.cu:580-645
 void orc_set_back_gamma(orc_box* b);   // spherocyl_box.cpp:524-545
 // 0: closest-point contact (src/cpp); 1: two-point contact model
 // (src/cuda/2point_dynamics.cu:391-546), each pair evaluated once, lower id first.

@@ -13,6 +13,7 @@
 
 #include "../../include/spherocyls.h"
 #include "sc_internal.h"
+#include "sc_status.h"
 
 namespace sc {
 // kernels (sc_device.hip, sc_list.hip)
@@ -55,14 +56,19 @@ struct HipError {
     if (e_ != hipSuccess) throw HipError{e_, #call}; \
   } while (0)
 
-struct StatusError {
-  int status;
-  std::string msg;
-};
+// Cell grid of configure_cells (src/cpp/spherocyl_box.cpp:228-241).  Columns
+// are 2.24 (Rmax + Amax) + pad wide: a pair in contact is at most
+// 2 (Rmax + Amax) sqrt(1 + gamma^2) apart in sheared x, and |gamma| <= 1/2 in
+// a square box.  A box with Lx != Ly remaps at |gamma| = Lx / (2 Ly), so its
+// columns are widened to 2 sqrt(1 + (Lx / 2 Ly)^2) (Rmax + Amax) + pad when
+// that is larger (Lx / Ly > 1.0087); square boxes keep the reference's rule.
+double cell_width_factor(double Lx, double Ly) {
+  const double r = Lx / Ly;
+  return std::max(2.24, 2.0 * std::sqrt(1.0 + 0.25 * r * r));
+}
 
-// Cell grid of configure_cells (src/cpp/spherocyl_box.cpp:228-241).
 void cell_grid(double Lx, double Ly, double Rmax, double Amax, double& pad, int& rows, int& cols) {
-  double wmin = 2.24 * (Rmax + Amax) + pad;
+  double wmin = cell_width_factor(Lx, Ly) * (Rmax + Amax) + pad;
   double hmin = 2 * (Rmax + Amax) + pad;
   rows = std::max(static_cast<int>(Ly / hmin), 1);
   cols = std::max(static_cast<int>(Lx / wmin), 1);
@@ -450,7 +456,6 @@ struct PassSpec {
   int mode = kModeForces;
   int se_every = 0, se_always = 0, do_fire = 0, do_gd = 0, do_resid = 0;
   int resize = 0, shrink = 0, corrected_image = 0, pure = 0;
-  double resize_scale = 1.0;
   int read_buf = 0, fill_buf = -1, clear_buf = -1;
   int kernel_id = SC_KERNEL_FORCE;
 };
@@ -481,7 +486,6 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.v0t = s->d_v0t;
     a.tile_part = s->d_tile_part;
     a.err = s->d_err;
-    a.resize_scale = ps.resize_scale;
     a.se_every = ps.se_every;
     a.se_always = ps.se_always;
     a.do_fire = ps.do_fire;
@@ -543,7 +547,6 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.gamma_window = s->gamma_window;
     a.part = s->d_list_part;
     a.err = s->d_err;
-    a.resize_scale = ps.resize_scale;
     a.ntot = s->ntot;
     a.npack = s->npack;
     a.se_every = ps.se_every;
@@ -637,7 +640,7 @@ void set_prec_all(sc_system* s, int prec) {
 // rotate: c0 holds the cells of x, the predictor bins x~ into c1 and zeroes
 // c2, the corrector reads c1, bins x' into c2 and zeroes c0.
 void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se_every, int st,
-                  double scale, bool pure = false) {
+                  bool pure = false) {
   const int c0 = s->cur_buf, c1 = (c0 + 1) % 3, c2 = (c0 + 2) % 3;
   if (se_every > 0 && st > 0 && st % se_every == 0) {
     // stress / energy record of the state the step starts from (the reference
@@ -655,7 +658,6 @@ void heun_substep(sc_system* s, bool resize, bool shrink, bool corrected, int se
   p.shrink = shrink;
   p.corrected_image = corrected;
   p.pure = pure;
-  p.resize_scale = scale;
   p.read_buf = c0;
   p.fill_buf = c1;
   p.clear_buf = c2;
@@ -762,6 +764,7 @@ void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int m
 }
 
 int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, double>>* Lhint);
+bool regrid(sc_system* s);
 // systems up to this many particles step through captured graphs
 constexpr int kGraphMaxParticles = 65536;
 
@@ -785,6 +788,7 @@ const char* sc_status_string(int status) {
     case SC_ERR_LIST_OVERFLOW: return "candidate list overflow";
     case SC_ERR_NOT_CONVERGED: return "minimiser did not converge";
     case SC_ERR_STATE: return "invalid state";
+    case SC_ERR_HOST: return "host memory or internal error";
     default: return "unknown status";
   }
 }
@@ -822,7 +826,8 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
   if (ndev <= 0) return SC_ERR_NO_DEVICE;
   if (device < 0 || device >= ndev) return SC_ERR_ARG;
 
-  sc_system* s = new sc_system();
+  sc_system* s = new (std::nothrow) sc_system();
+  if (!s) return SC_ERR_HOST;
   try {
     s->device = device;
     HIPCK(hipSetDevice(device));
@@ -846,6 +851,7 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
       P.dt = 0.05;  // SpherocylBox::set_defaults, src/cpp/spherocyl_box.cpp:24
       P.strain_rate = 0;
       P.resize_rate = 0;
+      P.resize_scale = 1.0;
       P.inv_2n = 1.0 / (2.0 * P.n);
       double filled = 0;
       for (int i = P.poff; i < P.poff + P.n; ++i) {
@@ -940,6 +946,12 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
     std::fprintf(stderr, "spherocyls: %s failed: %s\n", e.what, hipGetErrorString(e.e));
     sc_system_destroy(s);
     return SC_ERR_HIP;
+  } catch (...) {
+    std::string m;
+    const int st = current_exception_status(&m);
+    std::fprintf(stderr, "spherocyls: sc_system_create: %s\n", m.c_str());
+    sc_system_destroy(s);
+    return st;
   }
   *out = s;
   return SC_OK;
@@ -968,8 +980,10 @@ int sc_system_destroy(sc_system* s) {
   catch (const HipError& e) {                                                         \
     return fail(s, SC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e)); \
   }                                                                                   \
-  catch (const StatusError& e) {                                                      \
-    return fail(s, e.status, e.msg);                                                  \
+  catch (...) {                                                                       \
+    std::string m_;                                                                   \
+    const int st_ = current_exception_status(&m_);                                    \
+    return fail(s, st_, m_);                                                          \
   }
 
 int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate, double step) {
@@ -982,6 +996,7 @@ int sc_set_dynamics(sc_system* s, int k, double strain_rate, double resize_rate,
     s->packs[i].strain_rate = strain_rate;
     s->packs[i].resize_rate = resize_rate;
     s->packs[i].dt = step;
+    s->packs[i].resize_scale = std::pow(1.0 - resize_rate, step);
   }
   HIPCK(hipMemcpyAsync(s->d_packs, s->packs.data(), sizeof(PackInfo) * s->npack,
                        hipMemcpyHostToDevice, s->stream));
@@ -1226,13 +1241,13 @@ int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out) {
     for (; st + chunk <= nsteps; st += chunk) {
       if (!g.exec)
         g.capture([&]() {
-          for (int m = 0; m < chunk; ++m) heun_substep(s, false, false, false, 0, m, 1.0);
+          for (int m = 0; m < chunk; ++m) heun_substep(s, false, false, false, 0, m);
         });
       g.launch();
     }
   }
   for (; st < nsteps; ++st) {
-    heun_substep(s, false, false, false, se_every, st, 1.0);
+    heun_substep(s, false, false, false, se_every, st);
     if ((st + 1) % resort_every == 0 && st + 1 < nsteps) resort(s);
   }
   force_pass(s, false, se_every);
@@ -1256,10 +1271,11 @@ int sc_resize_steps(sc_system* s, int nsteps, int shrink, int corrected_image, i
   std::vector<std::pair<double, double>> Lm(s->npack);
   for (int k = 0; k < s->npack; ++k) Lm[k] = {s->scal[k].Lx, s->scal[k].Ly};
   for (int st = 0; st < nsteps; ++st) {
-    // all packings share one resize rate and step in this call
-    double scale = std::pow(1.0 - s->packs[0].resize_rate, s->packs[0].dt);
-    heun_substep(s, true, shrink != 0, corrected_image != 0, se_every, st, scale);
-    for (auto& v : Lm) {
+    // each packing resizes by its own (1 - rate)^dt
+    heun_substep(s, true, shrink != 0, corrected_image != 0, se_every, st);
+    for (int k = 0; k < s->npack; ++k) {
+      const double scale = s->packs[k].resize_scale;
+      auto& v = Lm[k];
       v.first = shrink ? v.first * scale : v.first / scale;
       v.second = shrink ? v.second * scale : v.second / scale;
     }
@@ -1287,7 +1303,7 @@ int sc_pure_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out)
   std::vector<std::pair<double, double>> Lm(s->npack);
   for (int k = 0; k < s->npack; ++k) Lm[k] = {s->scal[k].Lx, s->scal[k].Ly};
   for (int st = 0; st < nsteps; ++st) {
-    heun_substep(s, false, false, false, se_every, st, 1.0, true);
+    heun_substep(s, false, false, false, se_every, st, true);
     for (int k = 0; k < s->npack; ++k) {
       const double h = 0.5 * s->packs[k].dt * s->packs[k].strain_rate;
       Lm[k].first = Lm[k].first + h * Lm[k].first;
@@ -1306,14 +1322,11 @@ int sc_pure_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out)
 
 int sc_flip_shear(sc_system* s, int k) {
   if (!s || k < -1 || k >= s->npack) return SC_ERR_ARG;
+  API_BEGIN
   std::vector<double> x(s->ntot), y(s->ntot), phi(s->ntot);
   int st = sc_get_positions(s, x.data(), y.data(), phi.data());
   if (st != SC_OK) return st;
-  try {
-    download_scal(s);
-  } catch (const HipError& e) {
-    return fail(s, SC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
-  }
+  download_scal(s);
   std::vector<double> gam(s->npack);
   for (int i = 0; i < s->npack; ++i) {
     gam[i] = s->scal[i].gamma;
@@ -1330,6 +1343,46 @@ int sc_flip_shear(sc_system* s, int k) {
   for (int i = 0; st == SC_OK && i < s->npack; ++i)
     if (k < 0 || i == k) st = sc_set_gamma(s, i, gam[i], s->scal[i].gamma_total);
   return st;
+  API_END
+}
+
+int sc_rotate_by_gamma(sc_system* s, int k) {
+  if (!s || k < -1 || k >= s->npack) return SC_ERR_ARG;
+  API_BEGIN
+  std::vector<double> x(s->ntot), y(s->ntot), phi(s->ntot);
+  int st = sc_get_positions(s, x.data(), y.data(), phi.data());
+  if (st != SC_OK) return st;
+  download_scal(s);
+  for (int i = 0; i < s->npack; ++i) {
+    if (k >= 0 && i != k) continue;
+    // Spherocyl_Box::rotate_by_gamma / gamma_rotate (src/cuda/find_neighbors.cu:
+    // 580-645): rotate the lab frame so that the box vector (gamma Ly, Ly)
+    // becomes the new x axis; the box becomes Ly sqrt(1+gamma^2) by
+    // Lx / sqrt(1+gamma^2) at strain -gamma.
+    PackScal& S = s->scal[i];
+    const double g = S.gamma, root = std::sqrt(1.0 + g * g);
+    const double sgn = std::signbit(g) ? -1.0 : 1.0;
+    const double Lx = S.Lx, Ly = S.Ly;
+    const double nLx = Ly * root, nLy = Lx / root;
+    const double dphi = g != 0.0 ? std::atan(1.0 / g) : 0.5 * kRefPi;
+    const PackInfo& P = s->packs[i];
+    for (int j = P.poff; j < P.poff + P.n; ++j) {
+      const double nx = sgn * y[j] * root, ny = -sgn * x[j] / root;
+      double np = phi[j] - dphi;
+      if (np > kRefPi) np -= 2 * kRefPi;
+      else if (np < -kRefPi) np += 2 * kRefPi;
+      x[j] = nx < 0 ? nx + nLx : (nx > nLx ? nx - nLx : nx);
+      y[j] = ny < 0 ? ny + nLy : (ny > nLy ? ny - nLy : ny);
+      phi[j] = np;
+    }
+    S.Lx = nLx;
+    S.Ly = nLy;
+    S.gamma = -g;
+  }
+  upload_scal(s);
+  regrid(s);  // the cell grid of the new box; set_positions bins and re-sorts
+  return sc_set_positions(s, x.data(), y.data(), phi.data());
+  API_END
 }
 
 int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
@@ -1471,7 +1524,12 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     } else {
       for (int m = 0; m < chunk; ++m, ++it) {
         iterations(1);
-        if ((it + 1) % resort_every == 0) resort(s);
+        if ((it + 1) % resort_every == 0) {
+          // finished packings are no longer binned by the updates: bin every
+          // packing into the current buffer so the sort permutes all of them
+          bin_all(s, s->cur_buf, 0);
+          resort(s);
+        }
       }
     }
     int e = check_errors(s);
@@ -1741,6 +1799,17 @@ int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, doub
     if (!any) return SC_OK;
   }
   download_scal(s);  // device L is authoritative
+  if (!regrid(s)) return SC_OK;
+  s->cur_buf = 0;
+  bin_all(s, 0, 0);
+  resort(s);
+  return check_errors(s);
+}
+
+// The cell grid of every packing for its current box (host copy of the
+// scalars); reallocates the grid buffers when a grid changed.  Returns
+// whether one did.  The caller bins and re-sorts.
+bool regrid(sc_system* s) {
   bool changed = false;
   for (int k = 0; k < s->npack; ++k) {
     PackInfo& P = s->packs[k];
@@ -1753,12 +1822,9 @@ int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, doub
       changed = true;
     }
   }
-  if (!changed) return SC_OK;
+  if (!changed) return false;
   build_tiles(s);
   alloc_grid(s);
-  s->cur_buf = 0;
-  bin_all(s, 0, 0);
-  resort(s);
-  return check_errors(s);
+  return true;
 }
 }  // namespace

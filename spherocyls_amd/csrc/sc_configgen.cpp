@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/spherocyls.h"
+#include "sc_status.h"
 
 namespace {
 
@@ -74,9 +75,11 @@ double seg_dist2(double xp, double yp, double pp, double Ap, double xq, double y
 
 }  // namespace
 
-extern "C" int sc_random_configuration(int n, double L, double gamma, const double* A,
-                                       const double* R, int config, unsigned long long seed,
-                                       int max_tries, double* x, double* y, double* phi) {
+namespace {
+
+int random_configuration(int n, double L, double gamma, const double* A, const double* R,
+                         int config, unsigned long long seed, int max_tries, double* x, double* y,
+                         double* phi) {
   if (n <= 0 || !(L > 0) || !(std::fabs(gamma) <= 0.5) || !A || !R || !x || !y || !phi ||
       config < SC_CONFIG_RANDOM || config > SC_CONFIG_GRID_ALIGNED || max_tries < 1)
     return SC_ERR_ARG;
@@ -156,6 +159,18 @@ extern "C" int sc_random_configuration(int n, double L, double gamma, const doub
     }
   }
   return SC_OK;
+}
+
+}  // namespace
+
+extern "C" int sc_random_configuration(int n, double L, double gamma, const double* A,
+                                       const double* R, int config, unsigned long long seed,
+                                       int max_tries, double* x, double* y, double* phi) {
+  try {
+    return random_configuration(n, L, gamma, A, R, config, seed, max_tries, x, y, phi);
+  } catch (...) {
+    return sc::current_exception_status(nullptr);
+  }
 }
 
 extern "C" int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,

@@ -89,7 +89,11 @@ __device__ PackScal decide(const ReduceArgs& a, const PackInfo& P, const PackSca
   if (a.do_fire) {
     const FireParams& fp = a.fire;
     S2.frms = frms;
-    if (frms < fp.ftol) {
+    // Convergence is judged on FP64 pair forces only: a packing still on FP32
+    // pair math whose residual falls below max(switch_frms, ftol) moves to FP64
+    // and keeps iterating.
+    const bool fp32 = S.prec == kFP32;
+    if (!fp32 && frms < fp.ftol) {
       S2.converged = 1;
       S2.iters = S.step;
     } else if (S.step >= (int)fp.max_iter) {
@@ -121,7 +125,7 @@ __device__ PackScal decide(const ReduceArgs& a, const PackInfo& P, const PackSca
       S2.fire_alpha = alpha;
       S2.fire_npos = npos;
       S2.step = S.step + 1;
-      if (S.prec == kFP32 && frms < fp.switch_frms) S2.prec = kFP64;
+      if (fp32 && frms < fmax(fp.switch_frms, fp.ftol)) S2.prec = kFP64;
     }
   } else if (a.do_gd) {
     // gradient-descent passes always record energy: E = tot[kSeE] / (2N)
@@ -328,8 +332,8 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
   double gam = S.gamma;
   double L_new_x = Lx, L_new_y = Ly;
   if (MODE != kModeForces && a.resize) {
-    L_new_x = a.shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
-    L_new_y = a.shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
+    L_new_x = a.shrink ? Lx * P.resize_scale : Lx / P.resize_scale;
+    L_new_y = a.shrink ? Ly * P.resize_scale : Ly / P.resize_scale;
   }
   if (MODE == kModeCorr) {
     gam = S.gamma + dt * gd;
@@ -489,11 +493,11 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
         qy = y0 + dt * v0y;
         qp = p0 + dt * v0t / a.b.I[g];
         if (a.shrink) {
-          qx *= a.resize_scale;
-          qy *= a.resize_scale;
+          qx *= P.resize_scale;
+          qy *= P.resize_scale;
         } else {
-          qx /= a.resize_scale;
-          qy /= a.resize_scale;
+          qx /= P.resize_scale;
+          qy /= P.resize_scale;
         }
       }
       a.px[g] = qx;
@@ -524,10 +528,11 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
         y1 += ddy;
         p1 += ddp;
         double gnew = S.gamma + gd * dt;
-        if (gnew >= 0.5) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
+        const double ratio = S.Lx / S.Ly;  // general Lees-Edwards remap, as k_list
+        if (gnew >= 0.5 * ratio) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
           if (y1 < 0) y1 += S.Ly;
           else if (y1 > S.Ly) y1 -= S.Ly;
-          x1 += y1;
+          x1 += ratio * y1;
           if (x1 < 0) x1 += S.Lx;
           else {
             int guard = 0;
@@ -545,11 +550,11 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
         y1 += ddy;
         p1 += ddp;
         if (a.shrink) {
-          x1 *= a.resize_scale;
-          y1 *= a.resize_scale;
+          x1 *= P.resize_scale;
+          y1 *= P.resize_scale;
         } else {
-          x1 /= a.resize_scale;
-          y1 /= a.resize_scale;
+          x1 /= P.resize_scale;
+          y1 /= P.resize_scale;
         }
         Lbx = L_new_x;
         Lby = L_new_y;
@@ -568,11 +573,12 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
     PackScal S2 = S;
     S2.step = S.step + 1;
     if (!a.resize) {
+      const double ratio = S.Lx / S.Ly;
       S2.gamma = S.gamma + gd * dt;
       S2.gamma_total = S.gamma_total + gd * dt;
-      if (S2.gamma >= 0.5) {
-        S2.gamma -= 1.0;
-        S2.gamma_total = (double)(int)(S2.gamma_total + 1) + S2.gamma;
+      if (S2.gamma >= 0.5 * ratio) {
+        S2.gamma -= ratio;
+        if (ratio == 1.0) S2.gamma_total = (double)(int)(S2.gamma_total + 1) + S2.gamma;
       }
     } else {
       S2.Lx = L_new_x;
@@ -898,7 +904,7 @@ __global__ void k_export(Bank b, const PackInfo* packs, const PackScal* scal, co
           hit = false;
           if (fabs(dy) < reach + P.pad) {
             double dx = min_image(b.x[g] - b.x[q], S.Lx);
-            double dxp = dx + 0.5 * dy;
+            double dxp = dx + (0.5 * (S.Lx / S.Ly)) * dy;  // remap strain 0.5 Lx/Ly
             dx += S.gamma * dy;
             hit = fabs(dx) < reach + P.pad || fabs(dxp) < reach + P.pad;
           }

@@ -39,6 +39,7 @@ struct PackInfo {
   int tile_cols;
   double pad;            // cell padding
   double dt, strain_rate, resize_rate;
+  double resize_scale;   // (1 - resize_rate)^dt, the box scale of one resize step
   double inv_2n;         // 1 / (2N), energy normalisation
 };
 
@@ -96,7 +97,6 @@ struct TileArgs {
   double* tile_part;      // [tiles][kNSum] per-tile partial sums
   int* err;
   FireParams fire;
-  double resize_scale;    // (1 - rate)^dt of the resize step
   int se_every;           // pred/forces: record when step % se_every == 0 (0: never)
   int se_always;          // forces: reduce stress/energy unconditionally
   int do_fire;            // forces: FIRE sums + decisions in the last block
@@ -146,7 +146,6 @@ struct ListArgs {
   double gamma_window;     // lists cover strains center +- gamma_window
   double* part;            // [workgroups][kNSum] partial sums
   int* err;
-  double resize_scale;
   int ntot, npack;
   int se_every, se_always, do_fire, do_gd, do_resid;
   int resize, shrink, corrected_image;

@@ -239,11 +239,16 @@ __global__ void __launch_bounds__(256)
   const bool box_resize = BOX && a.resize, box_pure = BOX && a.pure;
   const bool box_shrink = BOX && a.shrink, box_corrected_image = BOX && a.corrected_image;
   const double dt = P.dt, gd = P.strain_rate;
+  // Lees-Edwards remap in the general form (src/cuda/find_neighbors.cu:320-330,
+  // :369; src/cuda/strain_dynamics.cu:712): at gamma >= Lx/(2 Ly), x += (Lx/Ly) y
+  // and gamma -= Lx/Ly.  For the square box of the CPU code the ratio is exactly
+  // 1 and this is its set_back_gamma.
+  const double le_ratio = S.Lx / S.Ly;
   double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
   double L_new_x = Lx, L_new_y = Ly;
   if (MODE != kModeForces && box_resize) {
-    L_new_x = box_shrink ? Lx * a.resize_scale : Lx / a.resize_scale;
-    L_new_y = box_shrink ? Ly * a.resize_scale : Ly / a.resize_scale;
+    L_new_x = box_shrink ? Lx * P.resize_scale : Lx / P.resize_scale;
+    L_new_y = box_shrink ? Ly * P.resize_scale : Ly / P.resize_scale;
   }
   if (MODE != kModeForces && box_pure) {
     // the box of pure_strain_step_2p (src/cuda/2point_dynamics.cu:2273-2274)
@@ -402,11 +407,11 @@ __global__ void __launch_bounds__(256)
         qy = yi + dt * v0y;
         qp = p0 + dt * v0t / a.b.I[g];
         if (box_shrink) {
-          qx *= a.resize_scale;
-          qy *= a.resize_scale;
+          qx *= P.resize_scale;
+          qy *= P.resize_scale;
         } else {
-          qx /= a.resize_scale;
-          qy /= a.resize_scale;
+          qx /= P.resize_scale;
+          qy /= P.resize_scale;
         }
       }
       a.px[g] = qx;
@@ -452,11 +457,11 @@ __global__ void __launch_bounds__(256)
         y1 += 0.5 * dt * (a.v0y[g] + v1y);
         p1 += 0.5 * dt * (a.v0t[g] + v1t);
         double gnew = S.gamma + gd * dt;
-        if (gnew >= 0.5) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
+        if (gnew >= 0.5 * le_ratio) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
           remap = true;
           if (y1 < 0) y1 += S.Ly;
           else if (y1 > S.Ly) y1 -= S.Ly;
-          x1 += y1;
+          x1 += le_ratio * y1;
           if (x1 < 0) x1 += S.Lx;
           else {
             int guard = 0;
@@ -471,11 +476,11 @@ __global__ void __launch_bounds__(256)
         y1 += 0.5 * dt * (a.v0y[g] + Fy);
         p1 += 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
         if (box_shrink) {
-          x1 *= a.resize_scale;
-          y1 *= a.resize_scale;
+          x1 *= P.resize_scale;
+          y1 *= P.resize_scale;
         } else {
-          x1 /= a.resize_scale;
-          y1 /= a.resize_scale;
+          x1 /= P.resize_scale;
+          y1 /= P.resize_scale;
         }
         Lbx = L_new_x;
         Lby = L_new_y;
@@ -491,7 +496,7 @@ __global__ void __launch_bounds__(256)
       a.b.ny[g] = Ai * sn;
       bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
       double gnext = box_pure ? S.gamma : S.gamma + gd * dt;
-      if (remap) gnext -= 1.0;
+      if (remap) gnext -= le_ratio;
       raise = remap || box_resize || box_pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
               fabs(min_image(y1 - ybi, Lby)) >= half_skin || fabs(gnext - center) >= a.gamma_window;
     }
@@ -512,10 +517,11 @@ __global__ void __launch_bounds__(256)
       S2.Ly = L_new_y;
       S2.gamma_total = S2.gamma_total + dt * gd;
     } else if (!box_resize) {
+      const double ratio = S2.Lx / S2.Ly;
       double g2 = S2.gamma + gd * dt, gt = S2.gamma_total + gd * dt;
-      if (g2 >= 0.5) {
-        g2 -= 1.0;
-        gt = (double)(int)(gt + 1) + g2;
+      if (g2 >= 0.5 * ratio) {
+        g2 -= ratio;
+        if (ratio == 1.0) gt = (double)(int)(gt + 1) + g2;
       }
       S2.gamma = g2;
       S2.gamma_total = gt;

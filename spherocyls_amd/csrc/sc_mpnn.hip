@@ -36,6 +36,7 @@
 #include "../../include/spherocyls.h"
 #include "../../include/spherocyls_mpnn.h"
 #include "sc_internal.h"
+#include "sc_status.h"
 #include "sc_physics.h"
 
 namespace sc {
@@ -751,10 +752,15 @@ int build_rev(int n, const std::vector<int>& off, const std::vector<int>& src,
   if (!m) return SC_ERR_ARG;     \
   try {                          \
     MPCK(hipSetDevice(m->device));
-#define MP_END                              \
-  }                                         \
-  catch (const std::string& s) {            \
-    return fail(m, SC_ERR_HIP, s);          \
+#define MP_END                                      \
+  }                                                 \
+  catch (const std::string& s) {                    \
+    return fail(m, SC_ERR_HIP, s);                  \
+  }                                                 \
+  catch (...) {                                     \
+    std::string m_;                                 \
+    const int st_ = sc::current_exception_status(&m_); \
+    return fail(m, st_, m_);                        \
   }
 
 extern "C" {
@@ -765,7 +771,8 @@ int sc_mpnn_create(sc_mpnn** out, int device, unsigned long long seed) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return SC_ERR_NO_DEVICE;
   if (device < 0 || device >= count) return SC_ERR_ARG;
-  sc_mpnn* m = new sc_mpnn();
+  sc_mpnn* m = new (std::nothrow) sc_mpnn();
+  if (!m) return SC_ERR_HOST;
   m->device = device;
   try {
     MPCK(hipSetDevice(device));
@@ -781,6 +788,10 @@ int sc_mpnn_create(sc_mpnn** out, int device, unsigned long long seed) {
   } catch (const std::string& s) {
     sc_mpnn_destroy(m);
     return SC_ERR_HIP;
+  } catch (...) {
+    const int st = sc::current_exception_status(nullptr);
+    sc_mpnn_destroy(m);
+    return st;
   }
   *out = m;
   return SC_OK;

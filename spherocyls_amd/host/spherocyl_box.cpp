@@ -839,6 +839,13 @@ void SpherocylBox::flip_shear_direction() {
   read_scalars();
 }
 
+void SpherocylBox::rotate_by_gamma() {
+  ensure_device();
+  check(sc_rotate_by_gamma(m_sys, 0), "rotate_by_gamma");
+  m_bHostStale = true;
+  read_scalars();
+}
+
 long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double dStopAspect,
                                                 double dSvStressGamma, double dSvPosGamma) {
   if (m_dStrainRate == 0.0) throw SpherocylError(SC_ERR_ARG, "run_pure_strain: zero strain rate");

@@ -82,6 +82,7 @@ def load():
         "sc_affine_resize": ([sysp, _dp], C.c_int),
         "sc_pure_strain_steps": ([sysp, C.c_int, C.c_int, _dp], C.c_int),
         "sc_flip_shear": ([sysp, C.c_int], C.c_int),
+        "sc_rotate_by_gamma": ([sysp, C.c_int], C.c_int),
         "sc_fire_minimize": ([sysp, _dp, C.c_double, _ip, _dp], C.c_int),
         "sc_get_neighbors": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
         "sc_get_contacts": ([sysp, C.c_int, _ip, _ip, C.c_int, _ip], C.c_int),
@@ -399,6 +400,9 @@ class System:
     def flip_shear(self, k=-1):
         self._ck(self.lib.sc_flip_shear(self.h, k))
 
+    def rotate_by_gamma(self, k=-1):
+        self._ck(self.lib.sc_rotate_by_gamma(self.h, k))
+
     def gd_steps(self, nmax, emin=0.0):
         done = C.c_int()
         self._ck(self.lib.sc_gd_steps(self.h, nmax, emin, C.byref(done)))
@@ -408,7 +412,7 @@ class System:
         s = _f64(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.npack,)))
         self._ck(self.lib.sc_affine_resize(self.h, _p(s)))
 
-    def fire(self, params=None, switch_frms=0.0, check=True):
+    def fire(self, params=None, switch_frms=1e-6, check=True):
         p = fire_params() if params is None else _f64(params)
         it = np.zeros(self.npack, dtype=np.int32)
         fr = np.zeros(self.npack)

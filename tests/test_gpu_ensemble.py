@@ -1,0 +1,74 @@
+"""The ragged 256-packing ensemble (N log-uniform in [64, 4096], phi U[0.80,
+0.86], aspect {1.2, 2, 4}, bidisperse radii) on the GPU, judged by the oracle:
+2000 sheared Heun steps of the whole batch, then batched FIRE (two-point
+contacts) in FP64 and in mixed precision.  Every packing's final state is
+re-evaluated by the oracle, which must find F_rms < 1e-10; energy and
+corrected stress of each packing agree with the oracle's on that state."""
+import numpy as np
+import pytest
+
+import oracle
+from spherocyls_amd import configs, ensemble, native
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def work():
+    pks = configs.ensemble(256)
+    print("ensemble: 256 packings,", sum(p.n for p in pks), "particles; 2000 Heun steps",
+          flush=True)
+    return ensemble.prepare(pks, heun_steps=2000, strain_rate=1e-4, dt=0.01)
+
+
+def _oracle_box(pk, gamma, x, y, phi):
+    b = oracle.OracleBox(pk.L, x, y, phi, pk.A, pk.R, 0.1)
+    b.set_contact_model(1)
+    b.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    b.find_neighbors()
+    return b
+
+
+def test_ensemble_covers_the_configuration(work):
+    sizes = np.array(work.sizes)
+    aspects = {round(float((p.A[0] + p.R[0]) / p.R[0]), 3) for p in work.packings}
+    assert len(sizes) == 256 and sizes.min() < 100 and sizes.max() > 3500
+    assert aspects == {1.2, 2.0, 4.0}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ensemble_minimised_states_pass_the_oracle(work, precision):
+    params = native.fire_params(max_iter=20000)
+    print("ensemble FIRE", precision, flush=True)
+    s = ensemble.system_for(work, precision=precision)
+    try:
+        it, fr, st = s.fire(params, check=False)
+        x, y, phi = s.positions()
+        boxes = [s.box(k) for k in range(s.npack)]
+    finally:
+        s.close()
+    conv = fr < 1e-10
+    assert conv.sum() >= 254, f"{precision}: {int(conv.sum())} of 256 converged"
+    worst = 0.0
+    o = 0
+    for k, pk in enumerate(work.packings):
+        sl = slice(o, o + pk.n)
+        o += pk.n
+        b = _oracle_box(pk, work.gammas[k], x[sl], y[sl], phi[sl])
+        f = b.frms()
+        if conv[k]:
+            # the oracle's FP64 residual of the GPU's final state
+            assert f < 1e-10, (precision, k, pk.name, f)
+            worst = max(worst, f)
+        else:
+            assert f >= 1e-10 * 0.5, (precision, k, pk.name, f)  # reported honestly
+        b.calc_se()
+        sc = b.scalars()
+        bx = boxes[k]
+        mag = abs(sc["cPxx"]) + abs(sc["cPyy"])
+        assert bx["E"] == pytest.approx(sc["E"], rel=1e-8, abs=1e-22), (k, pk.name)
+        for key in ("cPxx", "cPyy", "cPxy"):
+            assert bx[key] == pytest.approx(sc[key], rel=1e-8, abs=1e-8 * mag + 1e-22), (k, key)
+        assert bx["frms"] == pytest.approx(f, rel=1e-6, abs=1e-14)
+    print(f"ensemble {precision}: {int(conv.sum())} converged, oracle worst F_rms {worst:.3g}, "
+          f"iterations mean {it.mean():.0f} max {it.max()}", flush=True)

@@ -151,3 +151,86 @@ def test_box_run_pure_strain_and_flip(tmp_path):
     p = b.particles()
     assert _wrapdiff(flipped[:, 0], p["x"], Lx1) < TOL
     assert _wrapdiff(flipped[:, 2], p["phi"], 2 * math.pi) < TOL
+
+
+def test_rotate_by_gamma_then_shear_through_the_remap():
+    """rotate_by_gamma leaves a box with Lx / Ly = 1 + gamma^2; Heun shear steps
+    then cross the general remap strain Lx / (2 Ly).  Positions, forces and
+    stress against the oracle after the rotation and after the steps."""
+    R, A = configs.bidisperse(300)
+    gamma = 0.3
+    pk = configs.make_packing(R, A, 0.85, 29, "rotate", gamma=gamma)
+    rate, dt, nsteps = 2.0, 0.01, 30
+    print("rotate_by_gamma + shear", pk.n, flush=True)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_gamma(gamma, 0.0)
+        s.rotate_by_gamma()
+        s.calc_se()
+        x0, y0, p0 = s.positions()
+        f0 = s.forces()
+        b0 = s.box()
+        # shear from just below the remap strain, through it
+        r = b0["Lx"] / b0["Ly"]
+        g1 = 0.5 * r - 12 * rate * dt
+        s.set_gamma(g1, 0.0)
+        s.set_dynamics(strain_rate=rate, step=dt)
+        s.strain_steps(nsteps)
+        x1, y1, p1 = s.positions()
+        f1 = s.forces()
+        b1 = s.box()
+    finally:
+        s.close()
+    b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 0.1)
+    b.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    b.rotate_by_gamma()
+    b.calc_se()
+    o, p = b.scalars(), b.particles()
+    assert b0["Lx"] == pytest.approx(o["Lx"], rel=1e-15) and b0["gamma"] == -gamma
+    assert _wrapdiff(x0, p["x"], o["Lx"]) < 1e-12 and _wrapdiff(y0, p["y"], o["Ly"]) < 1e-12
+    assert _wrapdiff(p0, p["phi"], 2 * math.pi) < 1e-12
+    sc = b.force_scale()
+    for a_, r_, c in zip(f0, (p["fx"], p["fy"], p["tau"]), sc):
+        assert np.all(np.abs(a_ - r_) <= TOL * 2 * c + 1e-15)
+    for k in ("E", "cPxx", "cPyy", "cPxy"):
+        assert b0[k] == pytest.approx(o[k], rel=1e-9, abs=1e-12 * abs(o["cPxx"])), k
+    b.set_params(g1, 0.0, rate, 0.0, dt)
+    b.find_neighbors()
+    b.calc_forces()
+    b.run_steps(nsteps, 0)
+    b.calc_se()
+    o, p = b.scalars(), b.particles()
+    assert o["gamma"] < 0 and b1["gamma"] == pytest.approx(o["gamma"], abs=1e-14)  # remapped
+    assert _wrapdiff(x1, p["x"], o["Lx"]) < TOL and _wrapdiff(y1, p["y"], o["Ly"]) < TOL
+    assert _wrapdiff(p1, p["phi"], 2 * math.pi) < TOL
+    sc = b.force_scale()
+    for a_, r_, c in zip(f1, (p["fx"], p["fy"], p["tau"]), sc):
+        assert np.all(np.abs(a_ - r_) <= TOL * 2 * c + 1e-15)
+
+
+def test_resize_with_a_rate_per_packing():
+    """Two packings with different resize rates and steps in one system: each
+    box and trajectory against its own oracle run."""
+    R, A = configs.bidisperse(128)
+    pks = [configs.make_packing(R, A, 0.84, 31, "a"), configs.make_packing(R, A, 0.84, 32, "b")]
+    rates, dts, nsteps = (2e-3, 5e-3), (0.01, 0.02), 25
+    s = native.System(pks, pads=0.1)
+    try:
+        for k in range(2):
+            s.set_dynamics(resize_rate=rates[k], step=dts[k], k=k)
+        s.resize_steps(nsteps, shrink=True, corrected_image=True)
+        x, y, phi = s.positions()
+        boxes = [s.box(k) for k in range(2)]
+    finally:
+        s.close()
+    for k, pk in enumerate(pks):
+        ob = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 1.5)
+        ob.set_params(0.0, 0.0, 0.0, rates[k], dts[k])
+        ob.calc_forces()
+        for _ in range(nsteps):
+            ob.resize_step(shrink=True, corrected_image=True)
+            ob.calc_forces()
+        p, L = ob.particles(), ob.scalars()["Lx"]
+        assert boxes[k]["Lx"] == L, k
+        sl = slice(k * pk.n, (k + 1) * pk.n)
+        assert _wrapdiff(x[sl], p["x"], L) < 1e-11 and _wrapdiff(y[sl], p["y"], L) < 1e-11, k

@@ -126,3 +126,23 @@ def test_driver_reports_missing_gpu_or_bad_input(tmp_path):
                         "0.01", "1e-5", "0.1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "cannot read" in r.stderr
+
+
+def test_exceptions_map_to_status_codes(tmp_path):
+    """Host code inside the library throws; no exception may leave an extern "C"
+    entry point.  The shared mapping turns each kind into a status (built here
+    with g++; it needs no GPU), and every entry-point guard ends in catch (...)."""
+    exe = str(tmp_path / "status_map")
+    subprocess.run(["g++", "-std=c++17", "-O1", os.path.join(ROOT, "tests", "cpp", "status_map.cpp"),
+                    "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    csrc = os.path.join(ROOT, "spherocyls_amd", "csrc")
+    api = open(os.path.join(csrc, "sc_api.hip")).read()
+    mp = open(os.path.join(csrc, "sc_mpnn.hip")).read()
+    gen = open(os.path.join(csrc, "sc_configgen.cpp")).read()
+    for text, macro in ((api, "#define API_END"), (mp, "#define MP_END")):
+        body = text[text.index(macro):].split("\n\n", 1)[0]
+        assert "catch (...)" in body and "current_exception_status" in body, macro
+    assert "catch (...)" in gen
+    assert native.load().sc_status_string(10) == b"host memory or internal error"

@@ -120,3 +120,91 @@ def test_flip_is_a_mirror_image():
     np.testing.assert_allclose(p2["x"], p0["x"], rtol=0, atol=1e-12)
     np.testing.assert_array_equal(p2["phi"], p0["phi"])
     assert b.scalars()["gamma"] == 0.2
+
+
+# ---------------------------------------------------------------------------
+# rotate_by_gamma (src/cuda/find_neighbors.cu:580-645) and the Lees-Edwards
+# remap of a non-square box.  The rotation is a rigid turn of the lab frame, so
+# the physics fixes the answer: the contact set, the energy, |F| and the
+# torques are unchanged, forces turn with the frame and the stress tensor
+# transforms as R s R^T.
+# ---------------------------------------------------------------------------
+def _pairset(pq):
+    return set(zip(pq[0].tolist(), pq[1].tolist()))
+
+
+def _sheared_box(gamma, seed=71, n=200, aspect=2.0, phi=0.84):
+    R, A = configs.bidisperse(n, aspect=aspect)
+    pk = configs.make_packing(R, A, phi, seed, gamma=gamma)
+    b = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 0.1)
+    b.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    b.find_neighbors()
+    b.calc_se()
+    return pk, b
+
+
+@pytest.mark.parametrize("gamma", [0.3, -0.2, 0.0])
+def test_rotate_by_gamma_is_a_rigid_rotation(gamma):
+    pk, b = _sheared_box(gamma)
+    p0, s0, c0 = b.particles(), b.scalars(), _pairset(b.contacts())
+    b.rotate_by_gamma()
+    b.find_neighbors()
+    b.calc_se()
+    p1, s1, c1 = b.particles(), b.scalars(), _pairset(b.contacts())
+    root = math.sqrt(1 + gamma * gamma)
+    assert s1["Lx"] == pytest.approx(pk.L * root, rel=1e-15)
+    assert s1["Ly"] == pytest.approx(pk.L / root, rel=1e-15)
+    assert s1["gamma"] == -gamma
+    assert c1 == c0 and len(c0) > 100
+    assert s1["E"] == pytest.approx(s0["E"], rel=1e-11)
+    # the rotation that takes (gamma, 1) to the x axis (for gamma < 0 the frame
+    # turns the other way: the new x axis is -(gamma, 1))
+    sg = -1.0 if math.copysign(1.0, gamma) < 0 else 1.0
+    Rm = sg * np.array([[gamma, 1.0], [-1.0, gamma]]) / root
+    F0 = np.stack([p0["fx"], p0["fy"]])
+    F1 = np.stack([p1["fx"], p1["fy"]])
+    sx, sy, st = b.force_scale()
+    scale = np.hypot(sx, sy) + 1e-300
+    assert np.all(np.abs(F1 - Rm @ F0) <= 1e-10 * scale)
+    assert np.all(np.abs(p1["tau"] - p0["tau"]) <= 1e-10 * (st + 1e-300))
+    S0 = np.array([[s0["cPxx"], s0["cPxy"]], [s0["cPyx"], s0["cPyy"]]])
+    S1 = np.array([[s1["cPxx"], s1["cPxy"]], [s1["cPyx"], s1["cPyy"]]])
+    np.testing.assert_allclose(S1, Rm @ S0 @ Rm.T, rtol=1e-9, atol=1e-12 * np.abs(S0).max())
+
+
+def test_lees_edwards_remap_of_a_non_square_box():
+    """After the rotation Lx / Ly = 1 + gamma^2: the remap at gamma = Lx / (2 Ly),
+    x += (Lx / Ly) y, gamma -= Lx / Ly is a lattice translation, so every force
+    and the energy are unchanged by it."""
+    pk, b = _sheared_box(0.3, seed=72, phi=0.86)
+    b.rotate_by_gamma()
+    s = b.scalars()
+    r = s["Lx"] / s["Ly"]
+    assert r == pytest.approx(1.09, rel=1e-12)
+    b.set_params(0.5 * r, 0.0, 0.0, 0.0, 0.05)
+    b.find_neighbors()
+    b.calc_se()
+    p0, s0 = b.particles(), b.scalars()
+    b.set_back_gamma()
+    assert b.scalars()["gamma"] == pytest.approx(-0.5 * r, rel=1e-15)
+    b.calc_se()
+    p1 = b.particles()
+    sx, sy, st = b.force_scale()
+    for k, sc in (("fx", sx), ("fy", sy), ("tau", st)):
+        assert np.all(np.abs(p1[k] - p0[k]) <= 1e-11 * (sc + 1e-300)), k
+    assert b.scalars()["E"] == pytest.approx(s0["E"], rel=1e-12)
+    assert len(b.contacts()[0]) == s0["ncontacts"]
+
+
+def test_non_square_cells_cover_contacts_at_the_remap_strain():
+    """In the widened cell grid of a box with Lx / Ly = 1.25 at the largest
+    strain it reaches, Lx / (2 Ly), the neighbour-list contacts equal the O(N^2)
+    brute-force contacts."""
+    pk, b = _sheared_box(0.5, seed=73, n=400, aspect=3.0, phi=0.86)
+    b.rotate_by_gamma()
+    s = b.scalars()
+    r = s["Lx"] / s["Ly"]
+    for g in (0.5 * r - 1e-9, -0.5 * r):
+        b.set_params(g, 0.0, 0.0, 0.0, 0.05)
+        b.find_neighbors()
+        assert _pairset(b.contacts()) == _pairset(b.brute_contacts())
