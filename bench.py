@@ -82,6 +82,8 @@ def cpu_baseline(pk, gamma, strain_rate, dt, steps=200, runs=3):
         t0 = time.perf_counter()
         b.run_steps(steps, 0)
         rates.append(pk.n * steps / (time.perf_counter() - t0))
+        print(f"cpu baseline run {len(rates)}/{runs}: {rates[-1]:.4g} particle-steps/s",
+              file=sys.stderr, flush=True)
         del b
     return {"value": float(np.median(rates)), "unit": "particle-steps/s", "cores": 1,
             "kind": kind, "runs": [float(r) for r in rates],
@@ -142,6 +144,9 @@ def minimization(n_packings, reps, cpu_budget_s):
                                             "converged": conv,
                                             "mean_iterations": float(it.mean()),
                                             "fallbacks": len(fb)}
+    # what a warm start could save at best: exact displacements to the minimum
+    # applied with quality q (q = 0: the plain run)
+    out["warm_start_ceiling"] = warmstart.ceiling(work, params)
     sec, it, fr = ensemble.minimize_one_by_one(work, params, precision="fp64")
     conv = int(np.sum(fr < 1e-10))
     runs["one_by_one_fp64"] = {"value": conv / sec, "seconds": sec, "converged": conv,
@@ -208,6 +213,57 @@ def quasistatic_compression(eps=1e-4, phi0=0.80, phi1=0.86, n=16384, samples=13)
         "median": float(np.median(np.abs(pmx[jam] - p64[jam]) / p64[jam])) if np.any(jam) else None,
         "jammed_steps": int(np.sum(jam))}
     return out
+
+
+def jammed_compression(n=1024, phi0=0.86, phi1=0.90, eps=1e-3, seed=3):
+    """The compression continued past jamming (bidisperse aspect 2, seeded
+    random start at phi0, x, y, L *= 1 - eps per step to phi1, FIRE with the
+    two-point model to F_rms < 1e-10 after each step), in FP64 and in mixed
+    precision.  Reports, in the jammed steps (finite pressure): the mixed run's
+    reported pressure against the oracle's FP64 evaluation of the same state
+    (the precision error of the report), and against the FP64 run's pressure
+    at the same step (the two runs reach different minima once FP32 pair
+    forces set FIRE on another path: the spread of the jammed states)."""
+    import oracle
+    from spherocyls_amd import quasistatic
+
+    R, A = configs.bidisperse(n)
+    pk = configs.make_packing(R, A, phi0, seed, f"jammed compression N={n}")
+    params = native.fire_params(max_iter=400000)
+    curves, final = {}, {}
+    for prec in ("fp64", "mixed"):
+        s = native.System(pk, pads=0.1)
+        try:
+            s.set_contact_model("two_point")
+            s.set_precision(prec)
+            curves[prec] = quasistatic.compress(s, eps, phi1, params)
+            final[prec] = (s.positions(), s.box())
+        finally:
+            s.close()
+    f64, mix = curves["fp64"], curves["mixed"]
+    k = min(len(f64.packing), len(mix.packing))
+    p64, pmx = f64.pressure[:k, 0], mix.pressure[:k, 0]
+    jam = (p64 > 1e-6) & (pmx > 1e-6)
+    (x, y, phi), bx = final["mixed"]
+    ob = oracle.OracleBox(bx["Lx"], x, y, phi, pk.A, pk.R, 0.1)
+    ob.set_contact_model(1)
+    ob.find_neighbors()
+    ob.calc_se()
+    o = ob.scalars()
+    p_orc = 0.5 * (o["cPxx"] + o["cPyy"])
+    p_rep = 0.5 * (bx["cPxx"] + bx["cPyy"])
+    rel = np.abs(pmx[jam] - p64[jam]) / p64[jam]
+    return {"workload": f"N={n} bidisperse aspect 2, phi {phi0} -> {phi1}, eps {eps}, "
+                        "two-point FIRE to 1e-10",
+            "steps": int(k - 1), "jammed_steps": int(np.sum(jam)),
+            "final_pressure_fp64": float(p64[-1]), "final_pressure_mixed": float(pmx[-1]),
+            "mixed_report_vs_oracle_rel_error": abs(p_rep - p_orc) / abs(p_orc),
+            "mixed_final_frms_oracle": float(ob.frms()),
+            "mixed_vs_fp64_run_rel_diff": {"max": float(rel.max()) if len(rel) else None,
+                                           "median": float(np.median(rel)) if len(rel) else None},
+            "converged": {p: int(np.sum(c.frms[:, 0] < 1e-10)) for p, c in curves.items()},
+            "mean_iterations": {p: float(c.iterations[1:, 0].mean()) for p, c in curves.items()},
+            "seconds": {p: c.seconds for p, c in curves.items()}}
 
 
 def _cpu_fire_one(args):
@@ -422,6 +478,7 @@ def main():
                                            0.0 if args.no_cpu_baseline else 15.0)
     if not args.no_quasistatic:
         out["quasistatic"] = quasistatic_compression()
+        out["quasistatic"]["jammed"] = jammed_compression()
     if args.checkpoints > 0:
         from spherocyls_amd import checkpoints
 

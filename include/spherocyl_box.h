@@ -148,6 +148,32 @@ class SpherocylBox {
   // rotate_by_gamma (src/cuda/find_neighbors.cu:624-645): rotate the frame so
   // the sheared box vector is the new x axis; the box becomes non-square
   void rotate_by_gamma();
+  // One-point quasi-static protocols of the reference's GPU code
+  // (src/cuda/energy_minimize.cu:1731-1961).  The reference runs them with the
+  // closest-point contact model and its conjugate-gradient minimiser with
+  // angle damping; here each minimisation is FIRE with the box's contact model
+  // and minimiser parameters (set_minimizer_params: a closest-point packing
+  // that stalls on the alignment kink stops at max_iter, as the reference's
+  // minimiser stops at its step limit).  SE lines "t packing E Pxx Pyy Pxy Pyx"
+  // and sd%010lu.dat position files as for the two-point protocols.
+  //   quasistatic_compress: minimise; expand while E > 0; compress while
+  //                         packing < dMaxPack.
+  //   qs_one_cycle:         minimise; expand while E > 0; compress while
+  //                         E < dMaxE (SE lines to the open file f); returns
+  //                         the last time.
+  //   quasistatic_cycle:    nCycles cycles.
+  //   quasistatic_find_jam: cycles with dMaxE and the step halved after each,
+  //                         while dMaxE > dMinE and the step > Lx 1e-15.
+  //   find_jam_1p:          steps growing by 1.1 towards the energy dJamE,
+  //                         the next search from step 1.1^(steps) / 2, until
+  //                         the step is below Lx 1e-13; final state also to
+  //                         sd%010lu.bin.
+  void quasistatic_compress(double dMaxPack, double dResizeStep);
+  long unsigned int qs_one_cycle(FILE* f, long unsigned int nTime, double dMaxE,
+                                 double dResizeStep);
+  long unsigned int quasistatic_cycle(int nCycles, double dMaxE, double dResizeStep);
+  long unsigned int quasistatic_find_jam(double dMaxE, double dMinE, double dResizeStep);
+  long unsigned int find_jam_1p(double dJamE, double dResizeStep);
   long unsigned int compress_qs2p(double dMaxPacking, double dResizeStep);
   long unsigned int expand_qs2p(double dMinPacking, double dResizeStep);
   long unsigned int find_jam_2p(double dJamE, double dResizeStep);
@@ -191,6 +217,9 @@ class SpherocylBox {
   void qs_step(double dEpsilon);
   void qs_line(FILE* f, const char* fmt, unsigned long t);
   long unsigned int qs_run(double dLimit, double dResizeStep, bool bCompress);
+  FILE* qs_open_se();
+  long unsigned int resize_to_energy_1p(FILE* f, long unsigned int nTime, double dEnergy,
+                                        double dStep);
   long unsigned int resize_to_energy_2p(FILE* f, long unsigned int nTime, double dEnergy,
                                         double dStep);
 

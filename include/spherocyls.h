@@ -41,8 +41,10 @@ extern "C" {
 
 /* precision modes (sc_set_precision) */
 #define SC_PREC_FP64 0  /* all pair math in FP64 */
-#define SC_PREC_MIXED 1 /* FP32 pair math on FP64 relative displacements; FP64 sums,
-                           stress, energy and residual */
+#define SC_PREC_MIXED 1 /* FP32 pair math on FP64 relative displacements for the
+                           dynamics and the minimiser; FP64 sums and residual;
+                           passes that record stress and energy (sc_calc_se,
+                           se_every records) evaluate their pairs in FP64 */
 
 /* sc_get_box fields (double out[SC_BOX_NFIELDS]) */
 #define SC_BOX_LX 0
@@ -238,25 +240,16 @@ int sc_timing_get(sc_system* s, int kernel, double* ms_total, long long* launche
 int sc_timing_reset(sc_system* s);
 int sc_synchronize(sc_system* s);
 
-/* Seeded random placement (counterpart of SpherocylBox::random_configuration,
- * src/cpp/spherocyl_box.cpp:694-790, and its initialConfig modes,
- * src/cpp/spherocyl_box.h:8 plus the grid modes 4/5 of :718-737): centres
- * uniform in [0,L)^2 (sheared coordinates at strain gamma; grid modes: one per
- * site of a rows x cols grid in the particle aspect ratio) and angles uniform in
- * [0,pi) (ALIGNED modes: 0).  RANDOM modes redraw a particle whose segment comes
- * within sqrt(1e-5)(R_p+R_q) of an earlier one under the Lees-Edwards pair
- * separation; ZERO_E and GRID modes redraw any contact.  Host only; needs no
- * GPU.  SC_ERR_PLACEMENT after max_tries draws for one particle, or when the
- * grid sites are too small for the particles. */
-#define SC_CONFIG_RANDOM 0
-#define SC_CONFIG_RANDOM_ALIGNED 1
-#define SC_CONFIG_ZERO_E 2
-#define SC_CONFIG_ZERO_E_ALIGNED 3
-#define SC_CONFIG_GRID 4
-#define SC_CONFIG_GRID_ALIGNED 5
-int sc_random_configuration(int n, double L, double gamma, const double* A, const double* R,
-                            int config, unsigned long long seed, int max_tries, double* x,
-                            double* y, double* phi);
+/* Gamma-distributed aspect ratios: set_shapes with an aspect width > 0
+ * (src/cuda/spherocyl_box.cpp:1266-1300).  A_p / R_p is drawn from a gamma
+ * distribution of mean aspect_mean and standard deviation aspect_rel_sigma *
+ * aspect_mean (Marsaglia-Tsang sampling of a seeded xoshiro256** stream, where
+ * the reference uses an unseeded std::default_random_engine); R_p = 0.5
+ * bidispersity for even p and 0.5 for odd p.  The reference then rescales the
+ * box so that the drawn shapes fill the target packing fraction: size the box
+ * from the drawn R and A (L = sqrt(sum R (pi R + 4 A) / phi)).  Host only. */
+int sc_gamma_shapes(int n, double aspect_mean, double aspect_rel_sigma, double bidispersity,
+                    unsigned long long seed, double* R, double* A);
 /* sc_random_configuration in RANDOM mode. */
 int sc_random_packing(int n, double L, double gamma, const double* A, const double* R,
                       unsigned long long seed, int max_tries, double* x, double* y, double* phi);

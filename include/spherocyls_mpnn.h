@@ -71,6 +71,14 @@ int sc_mpnn_export_graph(sc_mpnn* m, sc_system* s, float* node_feat, int* offset
  * give no displacement.  Contacts of crossed rods (d = 0) carry a zero
  * normal. */
 int sc_mpnn_warm_start(sc_mpnn* m, sc_system* s);
+/* Damped warm start, all on the device: the proposal of sc_mpnn_warm_start is
+ * applied scaled by each factor lambdas[0..nlam) in turn and every packing's
+ * energy evaluated; each packing keeps the factor (or none, factor 0) whose
+ * state has the lowest energy, so no packing starts above its unmoved energy.
+ * e_before / e_after / lam_out [npack] (host, may be NULL): energy of the
+ * unmoved state, energy kept, factor kept. */
+int sc_mpnn_warm_start_damped(sc_mpnn* m, sc_system* s, int nlam, const double* lambdas,
+                              double* e_before, double* e_after, double* lam_out);
 
 #ifdef __cplusplus
 }

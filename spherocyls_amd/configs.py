@@ -99,3 +99,14 @@ def ensemble(count=256, seed=1, n_min=64, n_max=4096):
         out.append(make_packing(R, A, phi, seed * 100003 + k,
                                 f"ensemble[{k}] N={n} phi={phi:.4f} aspect {aspect}"))
     return out
+
+
+def gamma_aspect(n, aspect_mean=2.0, rel_sigma=0.5, phi=0.84, seed=1, bidispersity=1.4):
+    """Packing of gamma-distributed aspect ratios A/R (mean aspect_mean, relative
+    width rel_sigma; the reference's set_shapes with an aspect width), radii
+    0.5 bidispersity / 0.5 by index parity, box sized for phi with the drawn
+    shapes."""
+    from . import native
+    R, A = native.gamma_shapes(n, aspect_mean, rel_sigma, bidispersity, seed)
+    return make_packing(R, A, phi, seed, f"gamma aspect N={n} mean {aspect_mean} "
+                                       f"sigma {rel_sigma} phi={phi}")

@@ -29,7 +29,7 @@ __global__ void k_update(UpdateArgs u);
 __global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot);
 __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart);
 __global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
-                             int ncells_total);
+                             int ncells_total, int cap);
 __global__ void k_gather(Bank src, Bank dst, const int* perm, int ntot, int with_velocity);
 __global__ void k_export(Bank b, const PackInfo* packs, const PackScal* scal, const int* cnt,
                          const int* slot, int pk, int contacts, int* out_q, int* out_n);
@@ -110,6 +110,7 @@ struct sc_system {
   std::vector<double> Rmax, Amax, filled;
   int path = SC_PATH_LIST;
   int tile_cols = 12, block = 128, list_block = 256, list_lanes = 1;
+  int cap = kCapMin;  // bucket capacity (slots per cell) of every packing
   int lds_max = 64 * 1024;
   int prec = SC_PREC_FP64;
   int contact_model = SC_CONTACT_CLOSEST;
@@ -292,6 +293,7 @@ void build_tiles(sc_system* s) {
     int tw = P.cols >= 3 ? std::min(s->tile_cols, P.cols - 2) : P.cols;
     tw = std::max(1, std::min(tw, kMaxTileCols));
     P.tile_cols = tw;
+    P.cap = s->cap;
     std::vector<int4> spatial;
     for (int r = 0; r < P.rows; ++r)
       for (int c0 = 0; c0 < P.cols; c0 += tw)
@@ -312,7 +314,9 @@ void build_pblocks(sc_system* s) {
   for (int k = 0; k < s->npack; ++k) {
     const PackInfo& P = s->packs[k];
     std::vector<int2> spatial;
-    const int per_block = s->list_block / s->list_lanes;
+    // a workgroup owns one particle per thread (its wavefronts run the pair
+    // phase in list_lanes rounds)
+    const int per_block = s->list_block;
     for (int st = P.poff; st < P.poff + P.n; st += per_block) spatial.push_back(make_int2(k, st));
     s->list_ranges[k] = make_int2((int)s->pblocks.size(), (int)spatial.size());
     auto launch = xcd_order(spatial, (int)s->pblocks.size());
@@ -334,7 +338,7 @@ void alloc_grid(sc_system* s) {
     s->dfree(s->d_cnt[i]);
     s->dfree(s->d_slot[i]);
     s->d_cnt[i] = s->dalloc<int>(s->ncells);
-    s->d_slot[i] = s->dalloc<int>((size_t)s->ncells * kCap);
+    s->d_slot[i] = s->dalloc<int>((size_t)s->ncells * s->cap);
     HIPCK(hipMemsetAsync(s->d_cnt[i], 0, sizeof(int) * s->ncells, s->stream));
   }
   s->dfree(s->d_cstart);
@@ -397,7 +401,8 @@ int check_errors(sc_system* s) {
   if (!e) return SC_OK;
   HIPCK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
   if (e & kErrCellOverflow)
-    return fail(s, SC_ERR_CELL_OVERFLOW, "a cell holds more particles than its bucket (16)");
+    return fail(s, SC_ERR_CELL_OVERFLOW,
+                "a cell holds more particles than its bucket (" + std::to_string(s->cap) + ")");
   if (e & kErrList)
     return fail(s, SC_ERR_LIST_OVERFLOW, "a particle has more near candidates than its list holds");
   return fail(s, SC_ERR_ARG, "non-finite particle coordinate");
@@ -423,6 +428,40 @@ void bin_all(sc_system* s, int buf, int wrap_angles) {
                      s->d_slot[buf], s->d_err, s->ntot, wrap_angles);
 }
 
+// Raise the bucket capacity when bucket buffer `buf` (just filled by a binning
+// of every particle) holds a cell with more than half the capacity: the new
+// capacity is the next power of two at or above twice the largest count (at
+// most kCapMax).  The slot arrays of all three buffers are reallocated (their
+// counts stay valid: the buffer in use is binned again, the others are
+// cleared before their next fill).  Returns whether the capacity grew.
+bool grow_capacity(sc_system* s, int buf) {
+  std::vector<int> h(s->ncells);
+  HIPCK(hipMemcpyAsync(h.data(), s->d_cnt[buf], sizeof(int) * s->ncells, hipMemcpyDeviceToHost,
+                       s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
+  int most = 0;
+  for (int v : h) most = std::max(most, v);
+  if (2 * most <= s->cap || s->cap >= kCapMax) return false;
+  int cap = s->cap;
+  while (cap < 2 * most && cap < kCapMax) cap *= 2;
+  s->cap = cap;
+  for (auto& P : s->packs) P.cap = cap;
+  for (int i = 0; i < 3; ++i) {
+    s->dfree(s->d_slot[i]);
+    s->d_slot[i] = s->dalloc<int>((size_t)s->ncells * cap);
+  }
+  HIPCK(hipMemcpyAsync(s->d_packs, s->packs.data(), sizeof(PackInfo) * s->npack,
+                       hipMemcpyHostToDevice, s->stream));
+  HIPCK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));  // the overflow it cures
+  return true;
+}
+
+// bin_all, then raise the capacity and bin again if the buckets were crowded.
+void bin_all_sized(sc_system* s, int buf, int wrap_angles) {
+  bin_all(s, buf, wrap_angles);
+  if (grow_capacity(s, buf)) bin_all(s, buf, 0);
+}
+
 // Counting sort of the current bank by the cells of bucket buffer cur_buf:
 // per-packing scan of the counts, per-cell ordering by id, coalesced gather.
 void resort(sc_system* s) {
@@ -431,7 +470,8 @@ void resort(sc_system* s) {
   hipLaunchKernelGGL(k_scan_cells, dim3(s->npack), dim3(256), 0, s->stream, s->d_packs,
                      s->d_cnt[buf], s->d_cstart);
   hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(s->ncells, 256)), dim3(256), 0, s->stream,
-                     s->d_cstart, s->d_cnt[buf], s->d_slot[buf], s->B().id, s->d_perm, s->ncells);
+                     s->d_cstart, s->d_cnt[buf], s->d_slot[buf], s->B().id, s->d_perm, s->ncells,
+                     s->cap);
   Bank& src = s->bank[s->cur_bank];
   Bank& dst = s->bank[1 - s->cur_bank];
   hipLaunchKernelGGL(k_gather, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, src, dst,
@@ -498,7 +538,7 @@ void launch_force(sc_system* s, const PassSpec& ps) {
     a.two_point = s->contact_model == SC_CONTACT_TWO_POINT;
     a.npack = s->npack;
     a.list_groups = s->block / kLanes;
-    a.stage_cap = 3 * (s->tile_cols + 2) * kCap;
+    a.stage_cap = 3 * (s->tile_cols + 2) * s->cap;
     size_t lds = tile_lds_bytes(a.stage_cap, a.list_groups, s->block);
     if (lds > (size_t)s->lds_max) throw StatusError{SC_ERR_ARG, "tile too wide for LDS"};
     using TFn = void (*)(TileArgs);
@@ -577,8 +617,12 @@ void launch_force(sc_system* s, const PassSpec& ps) {
 #undef SC_LIST_MODES
 #undef SC_LIST_ROW
     const int li = s->list_lanes == 1 ? 0 : (s->list_lanes == 2 ? 1 : (s->list_lanes == 4 ? 2 : 3));
-    const int xi = s->prec == SC_PREC_MIXED ? 1 : 0;
     const bool se = ps.se_always || ps.se_every > 0;
+    // Mixed precision runs the dynamics and the minimiser on FP32 pair forces;
+    // a pass that records stress and energy evaluates its pairs in FP64 (the
+    // FP64-only kernel), so reported stress and energy are FP64 evaluations of
+    // the state the mixed run reached.
+    const int xi = (s->prec == SC_PREC_MIXED && !(ps.mode == kModeForces && se)) ? 1 : 0;
     const bool box = ps.resize || ps.pure;
     const int mi = ps.mode == kModeForces ? (se ? 0 : 3) : (box ? ps.mode + 3 : ps.mode);
     Fn fn = table[xi][a.two_point ? 1 : 0][mi][li];
@@ -934,7 +978,7 @@ int sc_system_create(sc_system** out, int device, int npack, const int* offsets,
     s->cur_bank = 0;
     s->cur_buf = 0;
     segvec(s);
-    bin_all(s, 0, 0);
+    bin_all_sized(s, 0, 0);
     resort(s);
     HIPCK(hipStreamSynchronize(s->stream));
     int st = check_errors(s);
@@ -1073,7 +1117,7 @@ int sc_get_path(const sc_system* s, int* path) {
 int sc_set_tiling(sc_system* s, int tile_cols, int block_threads) {
   if (!s || tile_cols < 1 || tile_cols > kMaxTileCols) return SC_ERR_ARG;
   if (block_threads != 64 && block_threads != 128 && block_threads != 256) return SC_ERR_ARG;
-  if (tile_lds_bytes(3 * (tile_cols + 2) * kCap, block_threads / kLanes, block_threads) >
+  if (tile_lds_bytes(3 * (tile_cols + 2) * s->cap, block_threads / kLanes, block_threads) >
       (size_t)s->lds_max)
     return fail(s, SC_ERR_ARG, "tile too wide for the workgroup's LDS");
   API_BEGIN
@@ -1126,7 +1170,7 @@ int sc_set_positions(sc_system* s, const double* x, const double* y, const doubl
                      s->d_pack_of, s->d_packs, s->d_io, s->d_io + s->ntot, s->d_io + 2 * s->ntot,
                      s->ntot);
   s->cur_buf = 0;
-  bin_all(s, 0, 1);
+  bin_all_sized(s, 0, 1);
   segvec(s);
   resort(s);
   return check_errors(s);
@@ -1248,7 +1292,10 @@ int sc_strain_steps(sc_system* s, int nsteps, int se_every, double* se_out) {
   }
   for (; st < nsteps; ++st) {
     heun_substep(s, false, false, false, se_every, st);
-    if ((st + 1) % resort_every == 0 && st + 1 < nsteps) resort(s);
+    if ((st + 1) % resort_every == 0 && st + 1 < nsteps) {
+      if (grow_capacity(s, s->cur_buf)) bin_all(s, s->cur_buf, 0);
+      resort(s);
+    }
   }
   force_pass(s, false, se_every);
   copy_se_out(s, nrec, se_out);
@@ -1419,7 +1466,7 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
   // positions are final; leave forces, energy and stress of the final state
   // (the updates skip finished packings: fill every packing's cells first)
   s->cur_buf = 0;
-  bin_all(s, 0, 0);
+  bin_all_sized(s, 0, 0);
   force_pass(s, true, 0);
   return check_errors(s);
   API_END
@@ -1443,7 +1490,7 @@ int sc_affine_resize(sc_system* s, const double* scale) {
   int e = reconfigure_if_needed(s, nullptr);
   if (e != SC_OK) return e;
   s->cur_buf = 0;
-  bin_all(s, 0, 0);
+  bin_all_sized(s, 0, 0);
   resort(s);
   return check_errors(s);
   API_END
@@ -1527,7 +1574,7 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
         if ((it + 1) % resort_every == 0) {
           // finished packings are no longer binned by the updates: bin every
           // packing into the current buffer so the sort permutes all of them
-          bin_all(s, s->cur_buf, 0);
+          bin_all_sized(s, s->cur_buf, 0);
           resort(s);
         }
       }
@@ -1565,7 +1612,7 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
   set_prec_all(s, kFP64);
   upload_scal(s);
   s->cur_buf = 0;
-  bin_all(s, 0, 0);
+  bin_all_sized(s, 0, 0);
   force_pass(s, true, 0);
   int e = check_errors(s);
   download_scal(s);
@@ -1695,7 +1742,7 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
     for (size_t ci = 0; ci < cands.size(); ++ci) {
       const Cand& c = cands[ci];
       if (c.path == SC_PATH_TILE) {
-        if (tile_lds_bytes(3 * (c.tile + 2) * kCap, c.block / kLanes, c.block) > (size_t)s->lds_max)
+        if (tile_lds_bytes(3 * (c.tile + 2) * s->cap, c.block / kLanes, c.block) > (size_t)s->lds_max)
           continue;
         if (sc_set_tiling(s, c.tile, c.block) != SC_OK) continue;
       } else if (sc_set_list_block(s, c.block, c.lanes) != SC_OK) {
@@ -1766,14 +1813,26 @@ int system_view(sc_system* s, SystemView* v) {
   v->cnt = s->d_cnt[s->cur_buf];
   v->slot = s->d_slot[s->cur_buf];
   v->pack_of = s->d_pack_of;
+  v->pack_sum = s->d_pack_sum;
   return SC_OK;
+}
+
+int system_energy_pass(sc_system* s) {
+  API_BEGIN
+  s->cur_buf = 0;
+  bin_all(s, 0, 0);
+  segvec(s);
+  force_rebuild(s);
+  force_pass(s, true, 0);
+  return SC_OK;
+  API_END
 }
 
 int system_positions_changed(sc_system* s) {
   if (!s) return SC_ERR_ARG;
   API_BEGIN
   s->cur_buf = 0;
-  bin_all(s, 0, 1);
+  bin_all_sized(s, 0, 1);
   segvec(s);
   resort(s);
   return check_errors(s);
@@ -1801,7 +1860,7 @@ int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, doub
   download_scal(s);  // device L is authoritative
   if (!regrid(s)) return SC_OK;
   s->cur_buf = 0;
-  bin_all(s, 0, 0);
+  bin_all_sized(s, 0, 0);
   resort(s);
   return check_errors(s);
 }

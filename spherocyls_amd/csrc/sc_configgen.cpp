@@ -49,6 +49,37 @@ struct Rng {
   double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
 };
 
+// Standard normal variate (Box-Muller transform of two uniforms).
+double normal(Rng& rng) {
+  const double kTwoPi = 6.283185307179586476925287;
+  double u1 = rng.uniform();
+  while (u1 <= 0.0) u1 = rng.uniform();
+  const double u2 = rng.uniform();
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(kTwoPi * u2);
+}
+
+// Gamma(shape k, scale theta) variate by Marsaglia and Tsang's squeeze method
+// (G. Marsaglia, W. W. Tsang, "A simple method for generating gamma
+// variables", ACM Trans. Math. Softw. 26 (2000) 363-372); for k < 1 the
+// variate of shape k + 1 times u^(1/k).
+double gamma_variate(Rng& rng, double k, double theta) {
+  if (k < 1.0) {
+    double u = rng.uniform();
+    while (u <= 0.0) u = rng.uniform();
+    return gamma_variate(rng, k + 1.0, theta) * std::pow(u, 1.0 / k);
+  }
+  const double d = k - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+  for (;;) {
+    const double x = normal(rng);
+    double v = 1.0 + c * x;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    const double u = rng.uniform();
+    if (u < 1.0 - 0.0331 * x * x * x * x) return d * v * theta;
+    if (u > 0.0 && std::log(u) < 0.5 * x * x + d * (1.0 - v + std::log(v))) return d * v * theta;
+  }
+}
+
 // Squared closest distance between two segments (same closest-approach
 // parametrisation as the pair force) for positions in sheared coordinates at
 // strain gamma: minimum image in each coordinate, then dx += gamma dy, the pair
@@ -168,6 +199,30 @@ extern "C" int sc_random_configuration(int n, double L, double gamma, const doub
                                        int max_tries, double* x, double* y, double* phi) {
   try {
     return random_configuration(n, L, gamma, A, R, config, seed, max_tries, x, y, phi);
+  } catch (...) {
+    return sc::current_exception_status(nullptr);
+  }
+}
+
+extern "C" int sc_gamma_shapes(int n, double aspect_mean, double aspect_rel_sigma,
+                               double bidispersity, unsigned long long seed, double* R,
+                               double* A) {
+  if (n <= 0 || !(aspect_mean > 0) || !(aspect_rel_sigma > 0) || !(bidispersity >= 1) ||
+      !std::isfinite(aspect_mean) || !std::isfinite(aspect_rel_sigma) || !R || !A)
+    return SC_ERR_ARG;
+  try {
+    // set_shapes with m_dASig > 0 (src/cuda/spherocyl_box.cpp:1266-1300):
+    // A/R ~ Gamma(k = mu^2 / sigma^2, theta = sigma^2 / mu) with
+    // sigma = aspect_rel_sigma * mu, and R = 0.5 bidispersity for even
+    // indices, 0.5 for odd ones
+    const double sig = aspect_mean * aspect_rel_sigma;
+    const double k = aspect_mean * aspect_mean / (sig * sig), theta = sig * sig / aspect_mean;
+    Rng rng(seed);
+    for (int p = 0; p < n; ++p) {
+      R[p] = 0.5 + (1 - (p % 2)) * (bidispersity - 1.0) * 0.5;
+      A[p] = R[p] * gamma_variate(rng, k, theta);
+    }
+    return SC_OK;
   } catch (...) {
     return sc::current_exception_status(nullptr);
   }

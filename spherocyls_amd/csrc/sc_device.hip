@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
     int r = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
     int cc = full ? jj : (c0 - 1 + jj + cols) % cols;
     int cell = P.coff + r * cols + cc;
-    L.cs[k + 1] = min(a.cnt[cell], kCap);
+    L.cs[k + 1] = min(a.cnt[cell], P.cap);
   }
   __syncthreads();
   if (tid < 64) {  // exclusive prefix of <= 128 cell counts in wave 0
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
     int cell = P.coff + r * cols + cc;
     int m = L.cs[k + 1] - L.cs[k];
     int j = e - L.cs[k];
-    const int* sl = a.slot + (size_t)cell * kCap;
+    const int* sl = a.slot + (size_t)cell * P.cap;
     int g = sl[j];
     int myid = a.b.id[g];
     int rank = 0;
@@ -756,7 +756,7 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
         u.flag_out[pk] = 1;
     }
   }
-  bin_wave<1>(bin_c, g, u.cnt, u.slot, u.err);
+  bin_wave<1>(bin_c, g, P.cap, u.cnt, u.slot, u.err);
 }
 
 // Affine rescale (x, y) *= scale[pack] (quasi-static compression step).
@@ -782,7 +782,7 @@ __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart)
   __syncthreads();
   for (int base = 0; base < ncell; base += nthr) {
     int c = base + tid;
-    int v = c < ncell ? min(cnt[P.coff + c], kCap) : 0;
+    int v = c < ncell ? min(cnt[P.coff + c], P.cap) : 0;
     int incl = v;
     for (int off = 1; off < 64; off <<= 1) {
       int t = __shfl_up(incl, off);
@@ -803,30 +803,43 @@ __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart)
 // Per cell: order members by particle id, record the permutation and rewrite
 // the bucket with the new (sorted) storage indices.
 __global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
-                             int ncells_total) {
+                             int ncells_total, int cap) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells_total) return;
-  int m = min(cnt[c], kCap);
-  int mem[kCap], key[kCap];
-  for (int j = 0; j < m; ++j) {
-    mem[j] = slot[c * kCap + j];
-    key[j] = id[mem[j]];
-  }
-  for (int j = 1; j < m; ++j) {  // insertion sort by id
-    int km = key[j], mm = mem[j], i = j - 1;
-    while (i >= 0 && key[i] > km) {
-      key[i + 1] = key[i];
-      mem[i + 1] = mem[i];
-      --i;
+  int m = min(cnt[c], cap);
+  int* sl = slot + (size_t)c * cap;
+  const int base = cstart[c];
+  if (m <= kCapMin) {
+    int mem[kCapMin], key[kCapMin];
+    for (int j = 0; j < m; ++j) {
+      mem[j] = sl[j];
+      key[j] = id[mem[j]];
     }
-    key[i + 1] = km;
-    mem[i + 1] = mm;
+    for (int j = 1; j < m; ++j) {  // insertion sort by id
+      int km = key[j], mm = mem[j], i = j - 1;
+      while (i >= 0 && key[i] > km) {
+        key[i + 1] = key[i];
+        mem[i + 1] = mem[i];
+        --i;
+      }
+      key[i + 1] = km;
+      mem[i + 1] = mm;
+    }
+    for (int j = 0; j < m; ++j) perm[base + j] = mem[j];
+  } else {
+    // a crowded cell: insertion sort by id in the bucket itself
+    for (int j = 1; j < m; ++j) {
+      const int mm = sl[j], km = id[mm];
+      int i = j - 1;
+      while (i >= 0 && id[sl[i]] > km) {
+        sl[i + 1] = sl[i];
+        --i;
+      }
+      sl[i + 1] = mm;
+    }
+    for (int j = 0; j < m; ++j) perm[base + j] = sl[j];
   }
-  int base = cstart[c];
-  for (int j = 0; j < m; ++j) {
-    perm[base + j] = mem[j];
-    slot[c * kCap + j] = base + j;
-  }
+  for (int j = 0; j < m; ++j) sl[j] = base + j;
   cnt[c] = m;
 }
 
@@ -873,9 +886,8 @@ __global__ void k_export(Bank b, const PackInfo* packs, const PackScal* scal, co
   int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= P.n) return;
   int g = P.poff + li;
-  double w = S.Lx / P.cols, h = S.Ly / P.rows;
-  int col = min(max((int)(b.x[g] / w), 0), P.cols - 1);
-  int row = min(max((int)(b.y[g] / h), 0), P.rows - 1);
+  int col, row;
+  cell_col_row(b.x[g], b.y[g], S.Lx, S.Ly, P, col, row);
   int rs[3], cs[3], nr = 0, ncl = 0;
   for (int d = -1; d <= 1; ++d) {
     int r = (row + d + P.rows) % P.rows;
@@ -892,9 +904,9 @@ __global__ void k_export(Bank b, const PackInfo* packs, const PackScal* scal, co
   for (int i = 0; i < nr; ++i)
     for (int j = 0; j < ncl; ++j) {
       int cell = P.coff + rs[i] * P.cols + cs[j];
-      int m = min(cnt[cell], kCap);
+      int m = min(cnt[cell], P.cap);
       for (int k = 0; k < m; ++k) {
-        int q = slot[cell * kCap + k];
+        int q = slot[(size_t)cell * P.cap + k];
         int idq = b.id[q];
         if (idq <= idp) continue;
         bool hit;

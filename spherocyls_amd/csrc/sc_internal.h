@@ -8,9 +8,13 @@
 
 namespace sc {
 
-// Particles a cell bucket holds.  Cells are at least 2.24 (Rmax + Amax) + pad
-// wide, so a bucket holds several times the mean occupancy of a dense packing.
-constexpr int kCap = 16;
+// Particles a cell bucket holds: at least kCapMin (the width of the in-register
+// sorts of a bucket), raised by the host to twice the largest occupancy it
+// sees (polydisperse packings, whose cells are sized by the longest rod, put
+// many short rods in one cell), at most kCapMax.  The capacity of a system is
+// PackInfo::cap, the same for all its packings.
+constexpr int kCapMin = 16;
+constexpr int kCapMax = 256;
 // Columns of cells per tile (upper bound; the autotuner picks the actual width).
 constexpr int kMaxTileCols = 32;
 // Passing candidates kept per particle in the LDS contact list.
@@ -37,6 +41,7 @@ struct PackInfo {
   int rows, cols, coff;  // cell grid, first global cell index
   int toff, ntiles;      // tile range
   int tile_cols;
+  int cap;               // bucket capacity (slots per cell)
   double pad;            // cell padding
   double dt, strain_rate, resize_rate;
   double resize_scale;   // (1 - resize_rate)^dt, the box scale of one resize step
@@ -201,6 +206,7 @@ struct SystemView {
   const int* cnt;          // exact cell buckets of the current state
   const int* slot;
   const int* pack_of;      // storage index -> packing
+  const double* pack_sum;  // [npack][kNSum] reduced sums of the last force pass
 };
 
 }  // namespace sc
@@ -211,5 +217,10 @@ int system_view(sc_system* s, SystemView* v);
 // After device code changed x, y or phi of the current bank: wrap, re-bin,
 // refresh segment vectors, re-sort and schedule a list rebuild.
 int system_positions_changed(sc_system* s);
+// After device code changed x, y or phi of the current bank: bin, refresh
+// segment vectors and evaluate energy and stress of every packing (into the
+// view's pack_sum), without re-sorting the bank.  Errors (a crowded cell, a
+// non-finite coordinate) surface at the next synchronising call.
+int system_energy_pass(sc_system* s);
 }  // namespace sc
 

@@ -375,8 +375,8 @@ __device__ int contacts_of(const SystemView& v, int g, int* out, int cap) {
   double xw = v.b.x[g], yw = v.b.y[g];
   wrap_coord(xw, Lx);
   wrap_coord(yw, Ly);
-  const int col = min(max((int)(xw / (Lx / P.cols)), 0), P.cols - 1);
-  const int row = min(max((int)(yw / (Ly / P.rows)), 0), P.rows - 1);
+  int col, row;
+  cell_col_row(xw, yw, Lx, Ly, P, col, row);
   int rs[3], cs[3], nr = 0, nc = 0;
   for (int d = -1; d <= 1; ++d) {
     int r = (row + d + P.rows) % P.rows, c = (col + d + P.cols) % P.cols;
@@ -390,9 +390,9 @@ __device__ int contacts_of(const SystemView& v, int g, int* out, int cap) {
   for (int i = 0; i < nr; ++i)
     for (int j = 0; j < nc; ++j) {
       const int cell = P.coff + rs[i] * P.cols + cs[j];
-      const int m = min(v.cnt[cell], kCap);
+      const int m = min(v.cnt[cell], P.cap);
       for (int k = 0; k < m; ++k) {
-        const int q = v.slot[(size_t)cell * kCap + k];
+        const int q = v.slot[(size_t)cell * P.cap + k];
         if (q == g) continue;
         PairOut<double> r;
         if (!pair_canonical(v, g, q, Lx, Ly, gam, r)) continue;
@@ -479,6 +479,9 @@ using namespace sc::mp;
 
 struct sc_mpnn {
   int device = 0;
+  // damped warm start: stashed positions and per-packing best energy / factor
+  double *x0 = nullptr, *y0 = nullptr, *p0 = nullptr, *bestE = nullptr, *bestL = nullptr;
+  int wcap = 0, wpack = 0;
   hipStream_t stream = nullptr;
   Layout lay;
   float scale[3] = {1.0f, 1.0f, 1.0f};
@@ -1032,6 +1035,105 @@ int sc_mpnn_export_graph(sc_mpnn* m, sc_system* s, float* node_feat, int* offset
     }
   }
   return SC_OK;
+  MP_END
+}
+
+namespace {
+
+__global__ void k_stash(Bank b, int n, double* x0, double* y0, double* p0) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  x0[g] = b.x[g];
+  y0[g] = b.y[g];
+  p0[g] = b.phi[g];
+}
+
+// x = x0 + lambda * proposal, lambda per packing (lam_pack) or one for all.
+__global__ void k_apply_from(Bank b, const float* __restrict__ Y, float sx, float sy, float sp,
+                             double cap_xy, double cap_phi, int n, const double* x0,
+                             const double* y0, const double* p0, double lam,
+                             const double* lam_pack, const int* pack_of) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double l = lam_pack ? lam_pack[pack_of[g]] : lam;
+  b.x[g] = x0[g] + l * proposal(Y[(size_t)g * kNo + 0], sx, cap_xy);
+  b.y[g] = y0[g] + l * proposal(Y[(size_t)g * kNo + 1], sy, cap_xy);
+  b.phi[g] = p0[g] + l * proposal(Y[(size_t)g * kNo + 2], sp, cap_phi);
+}
+
+// Keep, per packing, the factor whose state has the lowest energy so far.
+__global__ void k_take_best(const double* pack_sum, int npack, double lam, double* bestE,
+                            double* bestL, int first) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npack) return;
+  const double e = pack_sum[(size_t)k * kNSum + kSeE];
+  if (first) {
+    bestE[k] = e;
+    bestL[k] = 0.0;
+  } else if (e < bestE[k]) {
+    bestE[k] = e;
+    bestL[k] = lam;
+  }
+}
+
+}  // namespace
+
+int sc_mpnn_warm_start_damped(sc_mpnn* m, sc_system* s, int nlam, const double* lambdas,
+                              double* e_before, double* e_after, double* lam_out) {
+  if (!s || nlam < 0 || (nlam > 0 && !lambdas)) return SC_ERR_ARG;
+  for (int i = 0; i < nlam; ++i)
+    if (!std::isfinite(lambdas[i])) return SC_ERR_ARG;
+  int st = sc_mpnn_graph_from_system(m, s, nullptr, nullptr);
+  if (st != SC_OK) return st;
+  MP_BEGIN
+  forward(m);
+  SystemView v;
+  system_view(s, &v);
+  if (v.ntot > m->wcap) {
+    realloc_dev(m, m->x0, v.ntot);
+    realloc_dev(m, m->y0, v.ntot);
+    realloc_dev(m, m->p0, v.ntot);
+    m->wcap = v.ntot;
+  }
+  if (v.npack > m->wpack) {
+    realloc_dev(m, m->bestE, v.npack);
+    realloc_dev(m, m->bestL, v.npack);
+    m->wpack = v.npack;
+  }
+  // the system's kernels run on its own stream: order the two streams
+  MPCK(hipStreamSynchronize(m->stream));
+  hipLaunchKernelGGL(k_stash, dim3(blocks(v.ntot)), dim3(256), 0, v.stream, v.b, v.ntot, m->x0,
+                     m->y0, m->p0);
+  // the unmoved state, then each factor; the bank keeps its storage order
+  // throughout (no re-sort until the end), so the stash stays aligned
+  for (int i = -1; i < nlam; ++i) {
+    if (i >= 0)
+      hipLaunchKernelGGL(k_apply_from, dim3(blocks(v.ntot)), dim3(256), 0, v.stream, v.b, m->Y,
+                         m->scale[0], m->scale[1], m->scale[2], (double)kCapXY, (double)kCapPhi,
+                         v.ntot, m->x0, m->y0, m->p0, lambdas[i], nullptr, v.pack_of);
+    st = system_energy_pass(s);
+    if (st != SC_OK) return st;
+    hipLaunchKernelGGL(k_take_best, dim3(blocks(v.npack)), dim3(256), 0, v.stream, v.pack_sum,
+                       v.npack, i >= 0 ? lambdas[i] : 0.0, m->bestE, m->bestL, i < 0 ? 1 : 0);
+    if (i < 0 && e_before) {
+      std::vector<double> ps((size_t)v.npack * kNSum);
+      MPCK(hipMemcpyAsync(ps.data(), v.pack_sum, sizeof(double) * ps.size(),
+                          hipMemcpyDeviceToHost, v.stream));
+      MPCK(hipStreamSynchronize(v.stream));
+      for (int k = 0; k < v.npack; ++k) e_before[k] = ps[(size_t)k * kNSum + kSeE];
+    }
+  }
+  hipLaunchKernelGGL(k_apply_from, dim3(blocks(v.ntot)), dim3(256), 0, v.stream, v.b, m->Y,
+                     m->scale[0], m->scale[1], m->scale[2], (double)kCapXY, (double)kCapPhi,
+                     v.ntot, m->x0, m->y0, m->p0, 0.0, m->bestL, v.pack_of);
+  if (e_after)
+    MPCK(hipMemcpyAsync(e_after, m->bestE, sizeof(double) * v.npack, hipMemcpyDeviceToHost,
+                        v.stream));
+  if (lam_out)
+    MPCK(hipMemcpyAsync(lam_out, m->bestL, sizeof(double) * v.npack, hipMemcpyDeviceToHost,
+                        v.stream));
+  MPCK(hipStreamSynchronize(v.stream));
+  return system_positions_changed(s);
   MP_END
 }
 

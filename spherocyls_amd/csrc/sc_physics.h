@@ -389,15 +389,21 @@ __device__ __forceinline__ void wrap_angle(double& p) {
   while (p < -kRefPi && guard < 16) { p += 2 * kRefPi; ++guard; }
 }
 
-// Cell of a wrapped position; the rule is find_cells'
-// (src/cpp/spherocyl_box.cpp:293-295), clamped so x == L stays in range.
+// Column and row of a wrapped position: find_cells' floor(x / w)
+// (src/cpp/spherocyl_box.cpp:293-295) as floor(x cols / Lx), one division
+// instead of two, clamped so x == L stays in range.  Every kernel that bins a
+// particle or looks up its own cell uses this one rule, so a particle's cell
+// is the same wherever it is computed.
+__device__ __forceinline__ void cell_col_row(double xw, double yw, double Lx, double Ly,
+                                             const PackInfo& P, int& col, int& row) {
+  col = min(max((int)(xw * ((double)P.cols / Lx)), 0), P.cols - 1);
+  row = min(max((int)(yw * ((double)P.rows / Ly)), 0), P.rows - 1);
+}
+
 __device__ __forceinline__ int cell_of(double xw, double yw, double Lx, double Ly,
                                       const PackInfo& P) {
-  double w = Lx / P.cols, h = Ly / P.rows;
-  int col = (int)(xw / w);
-  int row = (int)(yw / h);
-  col = min(max(col, 0), P.cols - 1);
-  row = min(max(row, 0), P.rows - 1);
+  int col, row;
+  cell_col_row(xw, yw, Lx, Ly, P, col, row);
   return P.coff + row * P.cols + col;
 }
 
@@ -406,7 +412,7 @@ __device__ __forceinline__ void bin_into(int g, double xw, double yw, double Lx,
                                          const PackInfo& P, int* cnt, int* slot, int* err) {
   const int c = cell_of(xw, yw, Lx, Ly, P);
   int k = atomicAdd(&cnt[c], 1);
-  if (k < kCap) slot[c * kCap + k] = g;
+  if (k < P.cap) slot[(size_t)c * P.cap + k] = g;
   else atomicOr(err, kErrCellOverflow);
 }
 
@@ -422,7 +428,7 @@ __device__ __forceinline__ void bin_into(int g, double xw, double yw, double Lx,
 // wavefronts do little besides the binning (about 2.5 particles per cell);
 // the Heun passes keep one atomic per particle, which measured no slower there.
 template <int STRIDE>
-__device__ __forceinline__ void bin_wave(int c, int g, int* cnt, int* slot, int* err) {
+__device__ __forceinline__ void bin_wave(int c, int g, int cap, int* cnt, int* slot, int* err) {
   const int lane = __lane_id();
   const bool act = c >= 0;
   const int cp = __shfl_up(c, STRIDE);  // lanes below STRIDE read their own value
@@ -443,7 +449,7 @@ __device__ __forceinline__ void bin_wave(int c, int g, int* cnt, int* slot, int*
   base = __shfl(base, h);
   if (act) {
     const int k = base + __popcll(actm & below & ~((1ull << h) - 1ull));
-    if (k < kCap) slot[c * kCap + k] = g;
+    if (k < cap) slot[(size_t)c * cap + k] = g;
     else atomicOr(err, kErrCellOverflow);
   }
 }

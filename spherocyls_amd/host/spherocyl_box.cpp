@@ -997,6 +997,166 @@ long unsigned int SpherocylBox::find_jam_2p(double dJamE, double dResizeStep) {
   return t;
 }
 
+// ------------------------------------------------ one-point quasi-static protocols
+
+FILE* SpherocylBox::qs_open_se() {
+  if (m_nQsPosEvery < 1) throw SpherocylError(SC_ERR_ARG, "position interval < 1");
+  std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
+  FILE* f = std::fopen(sePath.c_str(), "w");
+  if (!f) throw SpherocylError(SC_ERR_ARG, "quasi-static protocol: cannot open " + sePath);
+  return f;
+}
+
+// quasistatic_compress (src/cuda/energy_minimize.cu:1731-1783)
+void SpherocylBox::quasistatic_compress(double dMaxPack, double dResizeStep) {
+  if (dResizeStep == 0 || !std::isfinite(dResizeStep))
+    throw SpherocylError(SC_ERR_ARG, "quasistatic_compress: resize step must be nonzero");
+  dResizeStep = std::fabs(dResizeStep);
+  ensure_device();
+  m_dPacking = packing_from_arrays();
+  if (!(dMaxPack >= m_dPacking))
+    throw SpherocylError(SC_ERR_ARG, "quasistatic_compress: target below the packing");
+  FILE* f = qs_open_se();
+  m_vQsIters.clear();
+  unsigned long t = 0;
+  try {
+    qs_minimize();
+    save_positions_sd(t);
+    qs_line(f, "%d %f %g %g %g %g %g\n", t);
+    while (m_dEnergy > 0) {
+      ++t;
+      qs_step(dResizeStep);
+      if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
+      qs_line(f, "%d %f %g %g %g %g %g\n", t);
+    }
+    while (m_dPacking < dMaxPack) {
+      ++t;
+      qs_step(-dResizeStep);
+      if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
+      qs_line(f, "%d %f %g %g %g %g %g\n", t);
+    }
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+}
+
+// qs_one_cycle (src/cuda/energy_minimize.cu:1785-1836): the reference's six
+// damped conjugate-gradient passes per step are one FIRE minimisation here
+long unsigned int SpherocylBox::qs_one_cycle(FILE* f, long unsigned int nTime, double dMaxE,
+                                             double dResizeStep) {
+  if (dResizeStep == 0 || !std::isfinite(dResizeStep))
+    throw SpherocylError(SC_ERR_ARG, "qs_one_cycle: resize step must be nonzero");
+  dResizeStep = std::fabs(dResizeStep);
+  ensure_device();
+  qs_minimize();
+  save_positions_sd(nTime);
+  qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
+  while (m_dEnergy > 0) {
+    ++nTime;
+    qs_step(dResizeStep);
+    if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
+    qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
+  }
+  while (m_dEnergy < dMaxE) {
+    ++nTime;
+    qs_step(-dResizeStep);
+    if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
+    qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
+  }
+  return nTime;
+}
+
+// quasistatic_cycle (src/cuda/energy_minimize.cu:1838-1857)
+long unsigned int SpherocylBox::quasistatic_cycle(int nCycles, double dMaxE, double dResizeStep) {
+  FILE* f = qs_open_se();
+  m_vQsIters.clear();
+  unsigned long t = 0;
+  try {
+    for (int c = 0; c < nCycles; ++c) {
+      t = qs_one_cycle(f, t, dMaxE, dResizeStep);
+      t += 1;
+    }
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  return t;
+}
+
+// quasistatic_find_jam (src/cuda/energy_minimize.cu:1859-1880)
+long unsigned int SpherocylBox::quasistatic_find_jam(double dMaxE, double dMinE,
+                                                     double dResizeStep) {
+  if (!(dResizeStep != 0) || !std::isfinite(dResizeStep) || !(dMaxE > 0))
+    throw SpherocylError(SC_ERR_ARG, "quasistatic_find_jam: need dMaxE > 0 and a nonzero step");
+  ensure_device();
+  read_scalars();
+  FILE* f = qs_open_se();
+  m_vQsIters.clear();
+  unsigned long t = 0;
+  try {
+    const double dMinResizeStep = m_dLx * 1e-15;
+    dResizeStep = std::fabs(dResizeStep);
+    while (dMaxE > dMinE && dResizeStep > dMinResizeStep) {
+      t = qs_one_cycle(f, t, dMaxE, dResizeStep);
+      dMaxE *= 0.5;
+      dResizeStep *= 0.5;
+      t += 1;
+    }
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  return t;
+}
+
+// resize_to_energy_1p (src/cuda/energy_minimize.cu:1882-1935): the energy is
+// that of the current state (the reference's calc_fe pass)
+long unsigned int SpherocylBox::resize_to_energy_1p(FILE* f, long unsigned int nTime,
+                                                     double dEnergy, double dStep) {
+  dStep = std::fabs(dStep);
+  calc_se();
+  const bool compress = m_dEnergy < dEnergy;
+  while (compress ? m_dEnergy < dEnergy : m_dEnergy > dEnergy) {
+    ++nTime;
+    qs_step(compress ? -dStep : dStep);
+    if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
+    qs_line(f, "%d %.9f %g %g %g %g %g\n", nTime);
+    dStep *= 1.1;
+  }
+  return nTime;
+}
+
+// find_jam_1p (src/cuda/energy_minimize.cu:1937-1961)
+long unsigned int SpherocylBox::find_jam_1p(double dJamE, double dResizeStep) {
+  if (!(dResizeStep > 0) || !std::isfinite(dResizeStep) || !(dJamE >= 0))
+    throw SpherocylError(SC_ERR_ARG, "find_jam_1p: need dJamE >= 0 and a positive step");
+  ensure_device();
+  read_scalars();
+  m_dPacking = packing_from_arrays();
+  FILE* f = qs_open_se();
+  m_vQsIters.clear();
+  unsigned long t = 0, told = 0;
+  try {
+    save_positions_sd(0);
+    const double dMinResizeStep = m_dLx * 1e-13;
+    while (dResizeStep > dMinResizeStep) {
+      t = resize_to_energy_1p(f, t, dJamE, dResizeStep);
+      dResizeStep = dResizeStep * std::pow(1.1, (double)((long)t - (long)told)) / 2;
+      told = t;
+    }
+    save_positions_bin(t);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  return t;
+}
+
 // ------------------------------------------------------------ position files
 
 PositionFile read_position_file(const std::string& path) {

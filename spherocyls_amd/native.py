@@ -13,7 +13,9 @@ import re
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libspherocyls.so")
+# SPHEROCYLS_LIB names another build of the same library (for example one
+# built with different compile-time options); the default is the in-tree build.
+LIB_PATH = os.environ.get("SPHEROCYLS_LIB") or os.path.join(PKG, "libspherocyls.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "spherocyls.h")
 
 _dp = C.POINTER(C.c_double)
@@ -92,6 +94,8 @@ def load():
         "sc_synchronize": ([sysp], C.c_int),
         "sc_random_packing": ([C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_ulonglong, C.c_int,
                                _dp, _dp, _dp], C.c_int),
+        "sc_gamma_shapes": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_ulonglong, _dp,
+                             _dp], C.c_int),
         "sc_random_configuration": ([C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int,
                                      C.c_ulonglong, C.c_int, _dp, _dp, _dp], C.c_int),
         "sc_write_positions": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -119,6 +123,7 @@ def load():
         "sc_mpnn_graph_from_system": ([sysp, sysp, _ip, _ip], C.c_int),
         "sc_mpnn_export_graph": ([sysp, sysp, _fp, _ip, _ip, _fp], C.c_int),
         "sc_mpnn_warm_start": ([sysp, sysp], C.c_int),
+        "sc_mpnn_warm_start_damped": ([sysp, sysp, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -148,6 +153,16 @@ def random_packing(L, A, R, seed, gamma=0.0, max_tries=100000):
     if st != 0:
         raise SpherocylsError(st, "random placement")
     return x, y, phi
+
+
+def gamma_shapes(n, aspect_mean, aspect_rel_sigma, bidispersity=1.4, seed=1):
+    """Radii and half-lengths with gamma-distributed A/R (sc_gamma_shapes)."""
+    lib = load()
+    R, A = np.empty(n), np.empty(n)
+    st = lib.sc_gamma_shapes(n, aspect_mean, aspect_rel_sigma, bidispersity, seed, _p(R), _p(A))
+    if st != 0:
+        raise SpherocylsError(st, "sc_gamma_shapes")
+    return R, A
 
 
 def random_configuration(L, A, R, seed, config=0, gamma=0.0, max_tries=100000):
@@ -578,3 +593,12 @@ class Mpnn:
 
     def warm_start(self, system):
         self._ck(self.lib.sc_mpnn_warm_start(self.h, system.h))
+
+    def warm_start_damped(self, system, lambdas=(1.0, 0.5, 0.25)):
+        """Damped proposal chosen per packing on the device; returns (energy
+        before, energy after, factor kept) per packing."""
+        lam = _f64(lambdas)
+        e0, e1, lk = (np.zeros(system.npack) for _ in range(3))
+        self._ck(self.lib.sc_mpnn_warm_start_damped(self.h, system.h, len(lam), _p(lam), _p(e0),
+                                                    _p(e1), _p(lk)))
+        return e0, e1, lk

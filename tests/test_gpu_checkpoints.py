@@ -1,7 +1,7 @@
 """The long-shear protocol with minimisation checkpoints, on a small packing:
-the mixed-precision stress and energy of each sheared state against FP64 on
-the same state, and every checkpoint minimised to F_rms < 1e-10 (with and
-without the learned warm start)."""
+the stress and energy of the mixed-precision trajectory against the FP64
+trajectory from the same start, and every checkpoint minimised to
+F_rms < 1e-10 (with and without the learned warm start)."""
 import pytest
 
 from spherocyls_amd import checkpoints, configs
@@ -14,8 +14,10 @@ def test_shear_with_checkpoints_small():
     out = checkpoints.run(pk, checkpoints=2, every=200, precision="mixed")
     assert out["steps"] == 400 and out["n_particles"] == 4096
     assert out["particle_steps_per_s"] > 0
-    err = out["se_rel_error_vs_fp64_max"]
-    # FP32 pair forces, FP64 sums: the error against FP64 on the same state
+    err = out["se_rel_error_vs_fp64_trajectory_max"]
+    # FP32 pair forces in the dynamics: the trajectories part slowly at this
+    # strain rate; the records themselves are FP64 evaluations
+    print("mixed vs FP64 trajectory, relative:", err, flush=True)
     for k in ("E", "Pxx", "Pyy", "Pxy_vs_pressure"):
         assert err[k] < 1e-4, (k, err[k])
     assert out["minimisations_converged"] == 2

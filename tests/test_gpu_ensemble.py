@@ -65,10 +65,15 @@ def test_ensemble_minimised_states_pass_the_oracle(work, precision):
         b.calc_se()
         sc = b.scalars()
         bx = boxes[k]
-        mag = abs(sc["cPxx"]) + abs(sc["cPyy"])
+        # At an unjammed minimum the overlaps 1 - d/sigma are ~1e-10 while d is
+        # an O(1) difference: one rounding of d (1e-16) moves a contact force by
+        # ~1e-16 / sigma whatever the implementation.  The stress comparison
+        # therefore has an absolute floor of that size per contact (x 1000);
+        # jammed packings, with finite stress, meet the 1e-8 relative bar.
+        floor = 1e-13 * (sc["ncontacts"] + 1) / (pk.L * pk.L)
         assert bx["E"] == pytest.approx(sc["E"], rel=1e-8, abs=1e-22), (k, pk.name)
         for key in ("cPxx", "cPyy", "cPxy"):
-            assert bx[key] == pytest.approx(sc[key], rel=1e-8, abs=1e-8 * mag + 1e-22), (k, key)
+            assert bx[key] == pytest.approx(sc[key], rel=1e-8, abs=floor), (k, key)
         assert bx["frms"] == pytest.approx(f, rel=1e-6, abs=1e-14)
     print(f"ensemble {precision}: {int(conv.sum())} converged, oracle worst F_rms {worst:.3g}, "
           f"iterations mean {it.mean():.0f} max {it.max()}", flush=True)

@@ -268,8 +268,10 @@ def test_mixed_precision_error_report():
         f64 = s.forces()
         b64 = s.box()
         s.set_precision("mixed")
-        s.calc_se()
+        s.calc_forces()  # the dynamics' pass: FP32 pair forces
         f32 = s.forces()
+        s.calc_se()  # a recording pass: FP64 pair forces
+        f_se = s.forces()
         b32 = s.box()
     finally:
         s.close()
@@ -284,7 +286,12 @@ def test_mixed_precision_error_report():
     # FP32 pair math on FP64 separations: ~1e-7 absolute per pair force, which is
     # up to ~1e-4 of a particle's force scale for small overlaps
     assert max(report[k] for k in ("fx", "fy", "tau")) < 1e-3
-    assert max(report[k] for k in ("E", "Pxx", "Pyy", "Pxy")) < 1e-4
+    assert max(report[k] for k in ("fx", "fy", "tau")) > 0  # the FP32 path ran
+    # stress and energy records evaluate their pairs in FP64 in mixed mode too
+    for a, b in zip(f_se, f64):
+        np.testing.assert_array_equal(a, b)
+    for k in ("E", "Pxx", "Pyy", "Pxy"):
+        assert report[k] == 0.0, k
 
 
 def test_batch_equals_single_packings():

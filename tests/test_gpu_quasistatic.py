@@ -119,3 +119,110 @@ def test_find_jam_2p_ends_on_the_target_energy(tmp_path):
     print("find_jam_2p: steps", t_end, "phi_J", packings[-1], "E", e, flush=True)
     assert e == pytest.approx(energies[-1], rel=1e-5)
     assert abs(e - target) < 0.1 * target
+
+
+def _oracle_1p(path):
+    n, Lx, Ly, packing, rows = _sd(path)
+    b = oracle.OracleBox(Lx, rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], rows[:, 3], 0.1)
+    b.find_neighbors()
+    b.calc_se()
+    return b
+
+
+def test_find_jam_1p_ends_on_the_target_energy(tmp_path):
+    """find_jam_1p (src/cuda/energy_minimize.cu:1882-1961) with the
+    closest-point model: the bracketing search ends next to the target
+    energy, and the oracle's energy of the final state (binary file) and of
+    the saved states equals the SE lines'."""
+    target, n = 1e-5, 64
+    # a closest-point minimisation that stalls on the alignment kink stops at
+    # the iteration cap, as the reference's minimiser stops at its step limit
+    r = subprocess.run([os.path.join(BIN, "spherocyl_findjam_1p"), "r", str(tmp_path), str(n),
+                        "1e-3", "0.01", str(target), "0.01", "0.01", "0.1", "0.80", "1", "1.4"],
+                       capture_output=True, text=True, timeout=140,
+                       env=dict(os.environ, SPHEROCYL_SEED="4", SPHEROCYL_FIRE_MAX_ITER="2000"))
+    print("driver: spherocyl_findjam_1p", flush=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
+    bins = glob.glob(str(tmp_path / "sd*.bin"))
+    assert len(bins) == 1
+    h, a = native.read_positions_bin(bins[0])
+    t_end = int(os.path.basename(bins[0])[2:12])
+    assert int(lines[-1].split()[0]) == t_end
+    energies = np.array([float(l.split()[2]) for l in lines])
+    crossings = np.nonzero(np.diff(np.sign(energies - target)))[0]
+    assert len(crossings) >= 5
+    # every line prints the packing with %.9f (resize_to_energy_1p's format)
+    assert all(len(l.split()[1].split(".")[1]) == 9 for l in lines)
+    b = oracle.OracleBox(h["Lx"], a["x"], a["y"], a["phi"], a["A"], a["R"], 0.1)
+    b.find_neighbors()
+    b.calc_se()
+    e = b.scalars()["E"]
+    print("find_jam_1p: steps", t_end, "E", e, "F_rms", b.frms(), flush=True)
+    assert e == pytest.approx(energies[-1], rel=1e-5)
+    assert abs(e - target) < 0.3 * target
+    for path in sorted(glob.glob(str(tmp_path / "sd0*.dat")))[1::7]:
+        t = int(os.path.basename(path)[2:12])
+        ob = _oracle_1p(path)
+        line = next(l for l in lines if int(l.split()[0]) == t)
+        assert float(line.split()[2]) == pytest.approx(ob.scalars()["E"], rel=1e-5, abs=1e-14), t
+
+
+def test_quasistatic_find_jam_cycles_and_compress(tmp_path):
+    """quasistatic_find_jam and quasistatic_compress (src/cuda/energy_minimize.cu:
+    1731-1880) through SpherocylBox in a small C++ program
+    (tests/cpp/box_qs_1p.cpp), closest-point model.  Each cycle expands until
+    the energy vanishes, then compresses until it reaches that cycle's
+    maximum, which halves from cycle to cycle together with the step; the
+    compression then expands to zero energy and compresses to the target
+    packing.  Saved states agree with their SE lines (oracle energy)."""
+    pkg = os.path.join(ROOT, "spherocyls_amd")
+    exe = str(tmp_path / "box_qs_1p")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "box_qs_1p.cpp"), "-L", pkg,
+                    "-lspherocyls", f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
+    n, step, emax, emin, maxpack = 64, 2e-3, 1e-4, 1e-6, 0.86
+    print("box_qs_1p", n, flush=True)
+    (tmp_path / "c").mkdir()
+    r = subprocess.run([exe, str(tmp_path), str(n), "0.84", "5", str(step), str(emax), str(emin),
+                        str(maxpack), str(tmp_path / "c")], capture_output=True, text=True,
+                       timeout=140)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = open(tmp_path / "cycle_se.dat").read().splitlines()
+    t = np.array([int(l.split()[0]) for l in lines])
+    phi = np.array([float(l.split()[1]) for l in lines])
+    E = np.array([float(l.split()[2]) for l in lines])
+    # a new cycle starts where the time stamp skips one (t += 1 between cycles)
+    starts = list(np.nonzero(np.diff(t) == 2)[0] + 1)
+    bounds = [0] + starts + [len(t)]
+    ncyc = len(bounds) - 1
+    assert ncyc == int(np.ceil(np.log2(emax / emin)))  # dMaxE halves until <= dMinE
+    for c in range(ncyc):
+        cyc_E, cyc_phi = E[bounds[c]:bounds[c + 1]], phi[bounds[c]:bounds[c + 1]]
+        emax_c = emax * 0.5 ** c
+        # compression ends at the first state at or above the cycle's maximum
+        assert cyc_E[-1] >= emax_c * (1 - 1e-5), c
+        comp = np.nonzero(np.diff(cyc_phi) > 0)[0]
+        assert np.all(cyc_E[comp] < emax_c * (1 + 1e-5)), c
+        # compression starts from a state without overlaps
+        assert len(comp) and cyc_E[comp[0]] == 0.0, c
+        # steps of the cycle: phi changes by (1 +- step_c)^-2
+        ratio = cyc_phi[1:] / cyc_phi[:-1]
+        step_c = step * 0.5 ** c
+        want = np.where(np.diff(cyc_phi) > 0, (1 - step_c) ** -2, (1 + step_c) ** -2)
+        np.testing.assert_allclose(ratio, want, rtol=2e-6)
+    comp_lines = open(tmp_path / "c" / "compress_se.dat").read().splitlines()
+    phi_c = np.array([float(l.split()[1]) for l in comp_lines])
+    assert phi_c[-1] >= maxpack and phi_c[-2] < maxpack
+    t_end, phi1, phi2 = r.stdout.split()
+    assert float(phi2) == pytest.approx(phi_c[-1], abs=1e-6)
+    checked = 0
+    for path in sorted(glob.glob(str(tmp_path / "sd0*.dat"))):
+        ts = int(os.path.basename(path)[2:12])
+        if ts % 25:
+            continue
+        ob = _oracle_1p(path)
+        line = next(l for l in lines if int(l.split()[0]) == ts)
+        assert float(line.split()[2]) == pytest.approx(ob.scalars()["E"], rel=1e-5, abs=1e-14), ts
+        checked += 1
+    assert checked >= 2

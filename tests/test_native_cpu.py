@@ -146,3 +146,53 @@ def test_exceptions_map_to_status_codes(tmp_path):
         assert "catch (...)" in body and "current_exception_status" in body, macro
     assert "catch (...)" in gen
     assert native.load().sc_status_string(10) == b"host memory or internal error"
+
+
+def test_gamma_shapes_follow_the_gamma_distribution():
+    """sc_gamma_shapes (set_shapes with an aspect width, src/cuda/spherocyl_box.cpp:
+    1266-1300): A/R is gamma-distributed with mean mu and standard deviation
+    sigma_rel mu (shape 1/sigma_rel^2, scale mu sigma_rel^2), for shapes above
+    and below 1 (the two branches of the sampler); radii alternate
+    0.5 bidispersity / 0.5; a seed gives one sequence."""
+    import scipy.stats as st
+
+    for mu, rel, seed in ((2.0, 0.5, 7), (1.5, 1.5, 8)):
+        R, A = native.gamma_shapes(50000, mu, rel, 1.4, seed)
+        a = A / R
+        k, theta = 1.0 / rel ** 2, mu * rel ** 2
+        assert abs(a.mean() - mu) < 5 * mu * rel / np.sqrt(len(a))
+        assert st.kstest(a, st.gamma(a=k, scale=theta).cdf).pvalue > 1e-3
+        np.testing.assert_array_equal(R[0::2], 0.7)
+        np.testing.assert_array_equal(R[1::2], 0.5)
+    R1, A1 = native.gamma_shapes(100, 2.0, 0.3, 1.0, 3)
+    R2, A2 = native.gamma_shapes(100, 2.0, 0.3, 1.0, 3)
+    np.testing.assert_array_equal(A1, A2)
+    with pytest.raises(native.SpherocylsError):
+        native.gamma_shapes(10, 2.0, 0.0)
+
+
+REF_CPP = "/root/reference/src/cpp"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CPP), reason="reference sources not present")
+@pytest.mark.parametrize("driver", ["main.cpp", "main_resize.cpp", "main_relax.cpp"])
+def test_reference_drivers_build_against_the_box_header(tmp_path, driver):
+    """The drop-in claim: the reference's own drivers compile unchanged against
+    include/spherocyl_box.h and link against the library (the reference's box
+    sources left out).  The driver and its file-input sources are compiled
+    from where they lie; only the reference's own spherocyl_box.h is kept out
+    of the include path, so the quoted include resolves to this header."""
+    src = tmp_path / "src"
+    src.mkdir()
+    # a directory holding the driver's other includes but not the old box header
+    for f in ("file_input.h",):
+        os.symlink(os.path.join(REF_CPP, f), src / f)
+    os.symlink(os.path.join(REF_CPP, driver), src / driver)
+    exe = str(tmp_path / "drv")
+    pkg = os.path.join(ROOT, "spherocyls_amd")
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-w", "-I", os.path.join(ROOT, "include"),
+                        str(src / driver), os.path.join(REF_CPP, "dat_file_input.cpp"), "-L", pkg,
+                        "-lspherocyls", f"-Wl,-rpath,{pkg}", "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.getsize(exe) > 0
