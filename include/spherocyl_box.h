@@ -115,10 +115,12 @@ class SpherocylBox {
   void set_minimizer_params(const double* pdParams);
 
   // Quasi-static protocols of the reference's GPU code
-  // (src/cuda/2point_dynamics.cu:4300-4606).  A step multiplies x, y and L by
-  // 1 - eps (compress) or 1 + eps (expand), as resize_step does
-  // (src/cuda/strain_dynamics.cu:1111-1128), then minimises to F_rms < ftol
-  // with FIRE (the reference runs its conjugate-gradient minimiser).  After
+  // (src/cuda/2point_dynamics.cu:4300-4632).  A step multiplies x, y and L by
+  // 1 - eps (compress) or 1 / (1 - eps) (expand), as resize_step_2p does
+  // (src/cuda/2point_dynamics.cu:1187-1285), takes one unsheared Heun step of
+  // dt 0.01 (the box's step is 0.01 afterwards, as in the reference), then
+  // minimises to F_rms < ftol with FIRE (the reference runs its
+  // conjugate-gradient minimiser).  After
   // each step: a line "t packing E Pxx Pyy Pxy Pyx" in the SE file (the
   // reference's printf formats; the stress is the corrected virial with the
   // CPU code's sign) and a position file sd%010lu.dat in the GPU code's text
@@ -131,6 +133,9 @@ class SpherocylBox {
   //                  steps growing by 1.1 until E crosses, then continue with
   //                  step * 1.1^(steps - 1) / 2, until the step is below 1e-15;
   //                  the final state also goes to sd%010lu.bin.
+  //   y_compress_energy_2p: the search of find_jam_2p towards dTargetE with
+  //                  y-only steps (y and Ly scaled, x and Lx kept) down to a
+  //                  step of 1e-14.
   // run_pure_strain_2p (src/cuda/2point_dynamics.cu:2286-2397): pure shear
   // (x expands, y contracts at half the strain rate; sc_pure_strain_steps) at
   // |strain rate| until Lx / Ly reaches dStopAspect.  Every dSvStressGamma of
@@ -149,7 +154,9 @@ class SpherocylBox {
   // the sheared box vector is the new x axis; the box becomes non-square
   void rotate_by_gamma();
   // One-point quasi-static protocols of the reference's GPU code
-  // (src/cuda/energy_minimize.cu:1731-1961).  The reference runs them with the
+  // (src/cuda/energy_minimize.cu:1731-1961).  Steps as above, except that an
+  // expansion multiplies by 1 + eps (resize_step, src/cuda/strain_dynamics.cu:
+  // 1124-1196).  The reference runs them with the
   // closest-point contact model and its conjugate-gradient minimiser with
   // angle damping; here each minimisation is FIRE with the box's contact model
   // and minimiser parameters (set_minimizer_params: a closest-point packing
@@ -177,6 +184,7 @@ class SpherocylBox {
   long unsigned int compress_qs2p(double dMaxPacking, double dResizeStep);
   long unsigned int expand_qs2p(double dMinPacking, double dResizeStep);
   long unsigned int find_jam_2p(double dJamE, double dResizeStep);
+  long unsigned int y_compress_energy_2p(double dTargetE, double dResizeStep);
   void set_qs_position_interval(int nSteps) { m_nQsPosEvery = nSteps; }
   // iterations of every minimisation of the last quasi-static protocol
   const std::vector<int>& qs_iterations() const { return m_vQsIters; }
@@ -214,14 +222,16 @@ class SpherocylBox {
   void write_se_line(std::ostream& out, unsigned long t, bool bPacking);
   double packing_from_arrays() const;
   int qs_minimize();
-  void qs_step(double dEpsilon);
+  void qs_step(double dEpsilon, bool bTwoPoint, bool bYOnly = false);
   void qs_line(FILE* f, const char* fmt, unsigned long t);
   long unsigned int qs_run(double dLimit, double dResizeStep, bool bCompress);
   FILE* qs_open_se();
   long unsigned int resize_to_energy_1p(FILE* f, long unsigned int nTime, double dEnergy,
                                         double dStep);
   long unsigned int resize_to_energy_2p(FILE* f, long unsigned int nTime, double dEnergy,
-                                        double dStep);
+                                        double dStep, bool bYOnly);
+  long unsigned int jam_search_2p(double dJamE, double dResizeStep, double dMinResizeStep,
+                                  bool bYOnly, const char* name);
 
   int m_nParticles = 0;
   double m_dLx = 0, m_dLy = 0;

@@ -197,6 +197,9 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done);
 /* Affine rescale of packing k (k = -1: all): x, y, L multiplied by scale[k]
  * (the quasi-static compression step of src/cuda/strain_dynamics.cu:1111-1128). */
 int sc_affine_resize(sc_system* s, const double* scale);
+/* The y-only form: y and Ly multiplied by scale[k], x and Lx kept
+ * (resize_y_coords_2p and y_resize_step_2p, src/cuda/2point_dynamics.cu:1174-1295). */
+int sc_affine_resize_y(sc_system* s, const double* scale);
 
 /* FIRE minimisation of every packing to F_rms < ftol (no reference counterpart;
  * the reference's own minimisers are gradient descent, relaxation_dynamics.cpp,
@@ -240,6 +243,25 @@ int sc_timing_get(sc_system* s, int kernel, double* ms_total, long long* launche
 int sc_timing_reset(sc_system* s);
 int sc_synchronize(sc_system* s);
 
+/* Seeded random placement (counterpart of SpherocylBox::random_configuration,
+ * src/cpp/spherocyl_box.cpp:694-790, and its initialConfig modes,
+ * src/cpp/spherocyl_box.h:8 plus the grid modes 4/5 of :718-737): centres
+ * uniform in [0,L)^2 (sheared coordinates at strain gamma; grid modes: one per
+ * site of a rows x cols grid in the particle aspect ratio) and angles uniform in
+ * [0,pi) (ALIGNED modes: 0).  RANDOM modes redraw a particle whose segment comes
+ * within sqrt(1e-5)(R_p+R_q) of an earlier one under the Lees-Edwards pair
+ * separation; ZERO_E and GRID modes redraw any contact.  Host only; needs no
+ * GPU.  SC_ERR_PLACEMENT after max_tries draws for one particle, or when the
+ * grid sites are too small for the particles. */
+#define SC_CONFIG_RANDOM 0
+#define SC_CONFIG_RANDOM_ALIGNED 1
+#define SC_CONFIG_ZERO_E 2
+#define SC_CONFIG_ZERO_E_ALIGNED 3
+#define SC_CONFIG_GRID 4
+#define SC_CONFIG_GRID_ALIGNED 5
+int sc_random_configuration(int n, double L, double gamma, const double* A, const double* R,
+                            int config, unsigned long long seed, int max_tries, double* x,
+                            double* y, double* phi);
 /* Gamma-distributed aspect ratios: set_shapes with an aspect width > 0
  * (src/cuda/spherocyl_box.cpp:1266-1300).  A_p / R_p is drawn from a gamma
  * distribution of mean aspect_mean and standard deviation aspect_rel_sigma *

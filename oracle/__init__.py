@@ -63,6 +63,8 @@ def _load_oracle():
         lib.orc_set_contact_model.argtypes = [C.c_void_p, C.c_int]
         lib.orc_set_contact_model.restype = None
         lib.orc_set_positions.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.orc_set_box.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        lib.orc_set_box.restype = None
         lib.orc_run_steps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         lib.orc_run_steps.restype = C.c_int
         lib.orc_resize_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -182,6 +184,9 @@ class OracleBox(_BoxBase):
     def set_positions(self, x=None, y=None, phi=None):
         x, y, phi = (None if v is None else _f64(v) for v in (x, y, phi))
         self.lib.orc_set_positions(self.h, _ptr(x), _ptr(y), _ptr(phi))
+
+    def set_box(self, Lx, Ly):
+        self.lib.orc_set_box(self.h, float(Lx), float(Ly))
 
     def find_neighbors(self): self.lib.orc_find_neighbors(self.h)
     def calc_forces(self): self.lib.orc_calc_forces(self.h)

@@ -939,6 +939,13 @@ void orc_set_positions(orc_box* b, const double* x, const double* y, const doubl
   }
 }
 
+void orc_set_box(orc_box* b, double Lx, double Ly) {
+  b->Lx = Lx;
+  b->Ly = Ly;
+  configure_cells(b);
+  build_neighbors(b);
+}
+
 void orc_find_neighbors(orc_box* b) { build_neighbors(b); }
 void orc_calc_forces(orc_box* b) { forces_main(b); }
 void orc_calc_se(orc_box* b) { forces_se(b); }

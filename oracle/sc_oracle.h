@@ -22,6 +22,8 @@ void orc_set_params(orc_box* b, double gamma, double gamma_total, double strain_
                     double resize_rate, double step);
 // Overwrite positions without wrapping (tests that probe images / invariances).
 void orc_set_positions(orc_box* b, const double* x, const double* y, const double* phi);
+// Overwrite the box lengths (states read from non-square files); rebuilds cells.
+void orc_set_box(orc_box* b, double Lx, double Ly);
 
 void orc_find_neighbors(orc_box* b);   // spherocyl_box.cpp:320-355
 void orc_calc_forces(orc_box* b);      // strain_dynamics.cpp:137-153

@@ -42,6 +42,7 @@ DRIVERS = {
     "spherocyl_minimize": ("main_relax.cpp", []),
     "spherocyl_qscompress": ("main_qs2p.cpp", []),
     "spherocyl_findjam": ("main_qs2p.cpp", ["-DSPHEROCYL_FIND_JAM"]),
+    "spherocyl_ycompress": ("main_qs2p.cpp", ["-DSPHEROCYL_Y_COMPRESS"]),
     "spherocyl_findjam_1p": ("main_qs2p.cpp", ["-DSPHEROCYL_FIND_JAM_1P"]),
     "spherocyl_qscycle": ("main_qs2p.cpp", ["-DSPHEROCYL_QS_CYCLE"]),
 }

@@ -26,7 +26,7 @@ __global__ void k_segvec(Bank b, int ntot);
 __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
                           int* cnt, int* slot, int* err, int ntot, int wrap_angles);
 __global__ void k_update(UpdateArgs u);
-__global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot);
+__global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot, int y_only);
 __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart);
 __global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
                              int ncells_total, int cap);
@@ -1472,21 +1472,21 @@ int sc_gd_steps(sc_system* s, int nmax, double emin, int* steps_done) {
   API_END
 }
 
-int sc_affine_resize(sc_system* s, const double* scale) {
+static int affine_resize(sc_system* s, const double* scale, bool y_only) {
   if (!s || !scale) return SC_ERR_ARG;
   for (int k = 0; k < s->npack; ++k)
     if (!(scale[k] > 0) || !std::isfinite(scale[k])) return SC_ERR_ARG;
   API_BEGIN
   download_scal(s);
   for (int k = 0; k < s->npack; ++k) {
-    s->scal[k].Lx *= scale[k];
+    if (!y_only) s->scal[k].Lx *= scale[k];
     s->scal[k].Ly *= scale[k];
   }
   upload_scal(s);
   HIPCK(hipMemcpyAsync(s->d_scale, scale, sizeof(double) * s->npack, hipMemcpyHostToDevice,
                        s->stream));
   hipLaunchKernelGGL(k_affine, dim3(grid_for(s->ntot, 256)), dim3(256), 0, s->stream, s->B(),
-                     s->d_pack_of, s->d_scale, s->ntot);
+                     s->d_pack_of, s->d_scale, s->ntot, y_only ? 1 : 0);
   int e = reconfigure_if_needed(s, nullptr);
   if (e != SC_OK) return e;
   s->cur_buf = 0;
@@ -1495,6 +1495,10 @@ int sc_affine_resize(sc_system* s, const double* scale) {
   return check_errors(s);
   API_END
 }
+
+int sc_affine_resize(sc_system* s, const double* scale) { return affine_resize(s, scale, false); }
+
+int sc_affine_resize_y(sc_system* s, const double* scale) { return affine_resize(s, scale, true); }
 
 int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int* iters,
                      double* frms) {

@@ -760,11 +760,11 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
 }
 
 // Affine rescale (x, y) *= scale[pack] (quasi-static compression step).
-__global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot) {
+__global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot, int y_only) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntot) return;
   double s = scale[pack_of[g]];
-  b.x[g] *= s;
+  if (!y_only) b.x[g] *= s;
   b.y[g] *= s;
 }
 

@@ -89,7 +89,7 @@ __device__ __forceinline__ void row_from_bucket(const ListArgs& a, const int* sl
 // A bucket of more than kCapMin members (crowded cells of polydisperse
 // packings): the members in increasing id by repeated search for the next
 // larger id, each tested as row_from_bucket does.
-__device__ __noinline__ void row_from_crowded_bucket(const ListArgs& a, const int* sl, int m,
+__device__ __forceinline__ void row_from_crowded_bucket(const ListArgs& a, const int* sl, int m,
                                                      int g, double xi, double yi, double reach_i,
                                                      double Lx, double Ly, double s, double sx,
                                                      double glo, double ghi, int& n) {

@@ -1,11 +1,13 @@
-// spherocyl_qscompress / spherocyl_findjam / spherocyl_findjam_1p /
-// spherocyl_qscycle: the quasi-static protocols with the argument order of the
-// reference's GPU drivers.  qscompress, findjam and findjam_1p follow
-// src/cuda/main_2point_qscompress.cpp, main_2point_findjam.cpp and
-// main_1point_findjam.cpp (lines 60-251):
+// spherocyl_qscompress / spherocyl_findjam / spherocyl_ycompress /
+// spherocyl_findjam_1p / spherocyl_qscycle: the quasi-static protocols with the
+// argument order of the reference's GPU drivers.  qscompress, findjam,
+// ycompress and findjam_1p follow src/cuda/main_2point_qscompress.cpp,
+// main_2point_findjam.cpp, main_2point_ycompress.cpp and main_1point_findjam.cpp
+// (lines 60-251):
 //   1 data file ("r" for random)   2 output directory   3 N
 //   4 resize rate (epsilon)   5 integration step size
-//   6 final packing fraction (qscompress) | energy at jamming (findjam)
+//   6 final packing fraction (qscompress) | energy at jamming (findjam) |
+//     target energy (ycompress)
 //   7 position save rate   8 stress save rate   9 cell padding
 //   [random only: 10 packing fraction  11 aspect ratio A/R  12 bidispersity
 //    13 initial configuration (0 random, 2 random-aligned, 3 zero-energy,
@@ -21,7 +23,7 @@
 // bidispersity for odd ones (src/cuda/main_2point_qscompress.cpp:164-170).
 // A data file is either format: save_positions (sp*.dat, "N L ..." header)
 // or the GPU code's sd*.dat ("N Lx Ly ..." header; Lx must equal Ly here).
-// qscompress and findjam use the two-point contact model, findjam_1p and
+// qscompress, findjam and ycompress use the two-point contact model, findjam_1p and
 // qscycle the closest-point model (the reference's one-point protocols);
 // minimisation is FIRE to F_rms < 1e-10.  SPHEROCYL_PRECISION=mixed selects
 // FP32 pair forces; SPHEROCYL_SEED seeds random starts;
@@ -60,6 +62,8 @@ int main(int argc, char* argv[]) {
   const char* self = "spherocyl_qscycle";
 #elif defined(SPHEROCYL_FIND_JAM_1P)
   const char* self = "spherocyl_findjam_1p";
+#elif defined(SPHEROCYL_Y_COMPRESS)
+  const char* self = "spherocyl_ycompress";
 #elif defined(SPHEROCYL_FIND_JAM)
   const char* self = "spherocyl_findjam";
 #else
@@ -85,6 +89,8 @@ int main(int argc, char* argv[]) {
     double step = float_input(argc, argv, ++argn, "Integration step size");
 #if defined(SPHEROCYL_FIND_JAM) || defined(SPHEROCYL_FIND_JAM_1P)
     double target = float_input(argc, argv, ++argn, "Energy at jamming");
+#elif defined(SPHEROCYL_Y_COMPRESS)
+    double target = float_input(argc, argv, ++argn, "Target Energy");
 #else
     double target = float_input(argc, argv, ++argn, "Final packing fraction");
 #endif
@@ -159,6 +165,8 @@ int main(int argc, char* argv[]) {
     unsigned long t = box.find_jam_1p(target, rate);
 #elif defined(SPHEROCYL_FIND_JAM)
     unsigned long t = box.find_jam_2p(target, rate);
+#elif defined(SPHEROCYL_Y_COMPRESS)
+    unsigned long t = box.y_compress_energy_2p(target, rate);
 #else
     unsigned long t = box.compress_qs2p(target, rate);
 #endif

@@ -814,11 +814,26 @@ int SpherocylBox::qs_minimize() {
   return it;
 }
 
-// x, y, L *= 1 + eps (src/cuda/strain_dynamics.cu:1111-1128), then minimise.
-void SpherocylBox::qs_step(double dEpsilon) {
+// One quasi-static step as the protocols take it: m_dStep = 0.01, then
+// resize_step (src/cuda/strain_dynamics.cu:1124-1196), resize_step_2p
+// (src/cuda/2point_dynamics.cu:1187-1285) or its y-only form y_resize_step_2p
+// (:1287-1385) -- x, y, L *= 1 + eps (y and Ly only for the last), where the
+// two-point form expands by 1/(1 - eps) so that an expansion undoes a
+// compression by the same step, followed by one unsheared Heun step of
+// dt 0.01 -- then the minimiser.  The box keeps the step 0.01 afterwards, as
+// the reference's does.
+void SpherocylBox::qs_step(double dEpsilon, bool bTwoPoint, bool bYOnly) {
   ensure_device();
+  if (bTwoPoint && dEpsilon > 0) dEpsilon = 1.0 / (1.0 - dEpsilon) - 1.0;
   const double scale = 1.0 + dEpsilon;
-  check(sc_affine_resize(m_sys, &scale), "affine_resize");
+  if (bYOnly)
+    check(sc_affine_resize_y(m_sys, &scale), "affine_resize_y");
+  else
+    check(sc_affine_resize(m_sys, &scale), "affine_resize");
+  m_dStep = 0.01;
+  check(sc_set_dynamics(m_sys, 0, 0.0, 0.0, m_dStep), "set_dynamics");
+  check(sc_strain_steps(m_sys, 1, 0, nullptr), "strain_steps");
+  check(sc_set_dynamics(m_sys, 0, m_dStrainRate, m_dResizeRate, m_dStep), "set_dynamics");
   m_bHostStale = true;
   read_scalars();
   qs_minimize();
@@ -931,7 +946,7 @@ long unsigned int SpherocylBox::qs_run(double dLimit, double dResizeStep, bool b
     qs_line(f, "%d %f %g %g %g %g %g\n", t);
     while (bCompress ? m_dPacking < dLimit : m_dPacking > dLimit) {
       ++t;
-      qs_step(bCompress ? -dResizeStep : dResizeStep);
+      qs_step(bCompress ? -dResizeStep : dResizeStep, true);
       if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
       qs_line(f, "%d %.9f %g %g %g %g %g\n", t);
     }
@@ -953,14 +968,15 @@ long unsigned int SpherocylBox::expand_qs2p(double dMinPacking, double dResizeSt
 
 // resize_to_energy_2p (src/cuda/2point_dynamics.cu:4393-4450): the energy
 // is that of the current state (the reference's calc_fe pass).
+// y_resize_to_energy_2p (:4458-4520) is the same search with y-only steps.
 long unsigned int SpherocylBox::resize_to_energy_2p(FILE* f, long unsigned int nTime,
-                                                     double dEnergy, double dStep) {
+                                                     double dEnergy, double dStep, bool bYOnly) {
   dStep = std::fabs(dStep);
   calc_se();
   const bool compress = m_dEnergy < dEnergy;
   while (compress ? m_dEnergy < dEnergy : m_dEnergy > dEnergy) {
     ++nTime;
-    qs_step(compress ? -dStep : dStep);
+    qs_step(compress ? -dStep : dStep, true, bYOnly);
     if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
     qs_line(f, "%d %.10f %.9g %.9g %.9g %.9g %.9g\n", nTime);
     dStep *= 1.1;
@@ -970,21 +986,32 @@ long unsigned int SpherocylBox::resize_to_energy_2p(FILE* f, long unsigned int n
 
 // find_jam_2p (src/cuda/2point_dynamics.cu:4582-4606)
 long unsigned int SpherocylBox::find_jam_2p(double dJamE, double dResizeStep) {
+  return jam_search_2p(dJamE, dResizeStep, 1e-15, false, "find_jam_2p");
+}
+
+// y_compress_energy_2p (src/cuda/2point_dynamics.cu:4608-4632): the search of
+// find_jam_2p with y-only steps and a minimum step of 1e-14
+long unsigned int SpherocylBox::y_compress_energy_2p(double dTargetE, double dResizeStep) {
+  return jam_search_2p(dTargetE, dResizeStep, 1e-14, true, "y_compress_energy_2p");
+}
+
+long unsigned int SpherocylBox::jam_search_2p(double dJamE, double dResizeStep,
+                                              double dMinResizeStep, bool bYOnly,
+                                              const char* name) {
   if (!(dResizeStep > 0) || !std::isfinite(dResizeStep) || !(dJamE >= 0))
-    throw SpherocylError(SC_ERR_ARG, "find_jam_2p: need dJamE >= 0 and a positive step");
+    throw SpherocylError(SC_ERR_ARG, std::string(name) + ": need a target energy >= 0 and a positive step");
   if (m_nQsPosEvery < 1) throw SpherocylError(SC_ERR_ARG, "position interval < 1");
   ensure_device();
   m_dPacking = packing_from_arrays();
   std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
   FILE* f = std::fopen(sePath.c_str(), "w");
-  if (!f) throw SpherocylError(SC_ERR_ARG, "find_jam_2p: cannot open " + sePath);
+  if (!f) throw SpherocylError(SC_ERR_ARG, std::string(name) + ": cannot open " + sePath);
   m_vQsIters.clear();
   unsigned long t = 0, told = 0;
   try {
     save_positions_sd(0);
-    const double dMinResizeStep = 1e-15;
     while (dResizeStep > dMinResizeStep) {
-      t = resize_to_energy_2p(f, t, dJamE, dResizeStep);
+      t = resize_to_energy_2p(f, t, dJamE, dResizeStep, bYOnly);
       dResizeStep = dResizeStep * std::pow(1.1, (double)((long)t - (long)told - 1)) / 2;
       told = t;
     }
@@ -1014,8 +1041,6 @@ void SpherocylBox::quasistatic_compress(double dMaxPack, double dResizeStep) {
   dResizeStep = std::fabs(dResizeStep);
   ensure_device();
   m_dPacking = packing_from_arrays();
-  if (!(dMaxPack >= m_dPacking))
-    throw SpherocylError(SC_ERR_ARG, "quasistatic_compress: target below the packing");
   FILE* f = qs_open_se();
   m_vQsIters.clear();
   unsigned long t = 0;
@@ -1025,13 +1050,13 @@ void SpherocylBox::quasistatic_compress(double dMaxPack, double dResizeStep) {
     qs_line(f, "%d %f %g %g %g %g %g\n", t);
     while (m_dEnergy > 0) {
       ++t;
-      qs_step(dResizeStep);
+      qs_step(dResizeStep, false);
       if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
       qs_line(f, "%d %f %g %g %g %g %g\n", t);
     }
     while (m_dPacking < dMaxPack) {
       ++t;
-      qs_step(-dResizeStep);
+      qs_step(-dResizeStep, false);
       if (t % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(t);
       qs_line(f, "%d %f %g %g %g %g %g\n", t);
     }
@@ -1055,13 +1080,13 @@ long unsigned int SpherocylBox::qs_one_cycle(FILE* f, long unsigned int nTime, d
   qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
   while (m_dEnergy > 0) {
     ++nTime;
-    qs_step(dResizeStep);
+    qs_step(dResizeStep, false);
     if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
     qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
   }
   while (m_dEnergy < dMaxE) {
     ++nTime;
-    qs_step(-dResizeStep);
+    qs_step(-dResizeStep, false);
     if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
     qs_line(f, "%d %f %g %g %g %g %g\n", nTime);
   }
@@ -1122,7 +1147,7 @@ long unsigned int SpherocylBox::resize_to_energy_1p(FILE* f, long unsigned int n
   const bool compress = m_dEnergy < dEnergy;
   while (compress ? m_dEnergy < dEnergy : m_dEnergy > dEnergy) {
     ++nTime;
-    qs_step(compress ? -dStep : dStep);
+    qs_step(compress ? -dStep : dStep, false);
     if (nTime % (unsigned long)m_nQsPosEvery == 0) save_positions_sd(nTime);
     qs_line(f, "%d %.9f %g %g %g %g %g\n", nTime);
     dStep *= 1.1;

@@ -82,6 +82,7 @@ def load():
         "sc_resize_steps": ([sysp, C.c_int, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
         "sc_gd_steps": ([sysp, C.c_int, C.c_double, _ip], C.c_int),
         "sc_affine_resize": ([sysp, _dp], C.c_int),
+        "sc_affine_resize_y": ([sysp, _dp], C.c_int),
         "sc_pure_strain_steps": ([sysp, C.c_int, C.c_int, _dp], C.c_int),
         "sc_flip_shear": ([sysp, C.c_int], C.c_int),
         "sc_rotate_by_gamma": ([sysp, C.c_int], C.c_int),
@@ -423,9 +424,10 @@ class System:
         self._ck(self.lib.sc_gd_steps(self.h, nmax, emin, C.byref(done)))
         return done.value
 
-    def affine_resize(self, scale):
+    def affine_resize(self, scale, y_only=False):
         s = _f64(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.npack,)))
-        self._ck(self.lib.sc_affine_resize(self.h, _p(s)))
+        fn = self.lib.sc_affine_resize_y if y_only else self.lib.sc_affine_resize
+        self._ck(fn(self.h, _p(s)))
 
     def fire(self, params=None, switch_frms=1e-6, check=True):
         p = fire_params() if params is None else _f64(params)

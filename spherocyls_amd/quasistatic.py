@@ -1,6 +1,7 @@
 """Batched quasi-static compression: every packing of one system takes the
 compress_qs2p step of the reference's GPU code (src/cuda/2point_dynamics.cu:
-4300-4340) -- x, y and L multiplied by 1 - eps -- and is then minimised to
+4300-4340, resize_step_2p :1187-1285) -- x, y and L multiplied by 1 - eps,
+then one unsheared Heun step of dt 0.01 -- and is then minimised to
 F_rms < ftol, all packings in one batched FIRE.  A packing that has reached
 its final packing fraction takes scale 1 and, already minimised, costs its
 FIRE nothing.  The per-packing loop of SpherocylBox::compress_qs2p is the same
@@ -43,14 +44,16 @@ def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm
     its packing fraction reaches phi_final (a scalar or one per packing),
     minimising after each step (mixed precision: FP32 pair math down to
     switch_frms).  warm: a native.Mpnn whose damped proposal precedes FIRE.
-    on_step(step, s) runs after each affine step, before the warm start and
-    FIRE, and after_step(step, s) after FIRE (used to record pre- and
-    post-minimisation states).  Returns a Curve."""
+    on_step(step, s) runs after each affine and Heun step, before the warm
+    start and FIRE, and after_step(step, s) after FIRE (used to record pre-
+    and post-minimisation states).  Leaves the system's dynamics at strain
+    rate 0 and step 0.01.  Returns a Curve."""
     from . import warmstart
 
     fin = np.broadcast_to(np.asarray(phi_final, dtype=np.float64), (s.npack,))
     sw = switch_frms if s.precision == "mixed" else 0.0
     rows = []
+    s.set_dynamics(strain_rate=0.0, resize_rate=0.0, step=0.01)
     s.synchronize()
     t0 = time.perf_counter()
     it, fr, _ = s.fire(params, switch_frms=sw, check=False)
@@ -61,6 +64,7 @@ def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm
         if not np.any(active):
             break
         s.affine_resize(np.where(active, 1.0 - eps, 1.0))
+        s.strain_steps(1)
         if on_step is not None:
             on_step(step, s)
         if warm is not None:

@@ -121,6 +121,48 @@ def test_find_jam_2p_ends_on_the_target_energy(tmp_path):
     assert abs(e - target) < 0.1 * target
 
 
+def test_y_compress_energy_2p_moves_only_y(tmp_path):
+    """y_compress_energy_2p (src/cuda/2point_dynamics.cu:4458-4632) through
+    spherocyl_ycompress (argument order of main_2point_ycompress.cpp): the
+    bracketing search of find_jam_2p with steps that scale y and Ly only.  Lx
+    stays; the first search compresses with steps s0 1.1^k (packing / (1 - s));
+    the second expands from s0 1.1^(steps - 1) / 2 by 1 / (1 - s) (packing
+    times 1 - s); the final state is a minimum on the target energy."""
+    target, n, s0 = 1e-6, 64, 1e-3
+    _run("spherocyl_ycompress", ["r", str(tmp_path), str(n), str(s0), "0.01", str(target), "0.01",
+                                 "0.01", "0.1", "0.80", "1", "1.4"])
+    n0, Lx0, Ly0, phi0, _ = _sd(str(tmp_path / "sd0000000000.dat"))
+    assert Lx0 == Ly0
+    lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
+    t = np.array([int(l.split()[0]) for l in lines])
+    phi = np.array([float(l.split()[1]) for l in lines])
+    E = np.array([float(l.split()[2]) for l in lines])
+    assert np.array_equal(t, np.arange(1, len(t) + 1))
+    first = int(np.argmax(E >= target)) + 1  # steps of the first search
+    assert first >= 2 and np.all(E[:first - 1] < target)
+    want = phi0 / np.cumprod(1 - s0 * 1.1 ** np.arange(first))
+    np.testing.assert_allclose(phi[:first], want, rtol=0, atol=3e-10)  # printed with %.10f
+    s1 = s0 * 1.1 ** (first - 1) / 2
+    assert phi[first] == pytest.approx(phi[first - 1] * (1 - s1), abs=3e-10)
+    crossings = np.nonzero(np.diff(np.sign(E - target)))[0]
+    assert len(crossings) >= 5
+    bins = glob.glob(str(tmp_path / "sd*.bin"))
+    assert len(bins) == 1
+    h, a = native.read_positions_bin(bins[0])
+    assert h["Lx"] == pytest.approx(Lx0, rel=1e-13) and h["Ly"] < 0.99 * Ly0
+    assert h["Ly"] == pytest.approx(Ly0 * phi0 / phi[-1], rel=1e-9)
+    b = oracle.OracleBox(h["Lx"], a["x"], a["y"], a["phi"], a["A"], a["R"], 0.1)
+    b.set_box(h["Lx"], h["Ly"])
+    b.set_contact_model(1)
+    b.find_neighbors()
+    b.calc_se()
+    e = b.scalars()["E"]
+    print("y_compress_energy_2p: steps", t[-1], "Ly/Lx", h["Ly"] / h["Lx"], "E", e, flush=True)
+    assert b.frms() < 1e-10
+    assert e == pytest.approx(E[-1], rel=1e-5)
+    assert abs(e - target) < 0.1 * target
+
+
 def _oracle_1p(path):
     n, Lx, Ly, packing, rows = _sd(path)
     b = oracle.OracleBox(Lx, rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], rows[:, 3], 0.1)
@@ -130,17 +172,18 @@ def _oracle_1p(path):
 
 
 def test_find_jam_1p_ends_on_the_target_energy(tmp_path):
-    """find_jam_1p (src/cuda/energy_minimize.cu:1882-1961) with the
-    closest-point model: the bracketing search ends next to the target
-    energy, and the oracle's energy of the final state (binary file) and of
-    the saved states equals the SE lines'."""
-    target, n = 1e-5, 64
-    # a closest-point minimisation that stalls on the alignment kink stops at
-    # the iteration cap, as the reference's minimiser stops at its step limit
+    """find_jam_1p (src/cuda/energy_minimize.cu:1882-1961): the bracketing
+    search ends next to the target energy, and the oracle's energy of the
+    final state (binary file) and of the saved states equals the SE lines'.
+    The one-point step rule differs from the two-point one: an expansion
+    multiplies x, y, L by 1 + s (packing / (1 + s)^2), and the next search
+    starts from s 1.1^(steps) / 2.  Bidisperse discs (aspect 0), so that no
+    minimisation stalls on the closest-point model's alignment kink."""
+    target, n, s0 = 1e-5, 64, 1e-3
     r = subprocess.run([os.path.join(BIN, "spherocyl_findjam_1p"), "r", str(tmp_path), str(n),
-                        "1e-3", "0.01", str(target), "0.01", "0.01", "0.1", "0.80", "1", "1.4"],
+                        str(s0), "0.01", str(target), "0.01", "0.01", "0.1", "0.80", "0", "1.4"],
                        capture_output=True, text=True, timeout=140,
-                       env=dict(os.environ, SPHEROCYL_SEED="4", SPHEROCYL_FIRE_MAX_ITER="2000"))
+                       env=dict(os.environ, SPHEROCYL_SEED="4"))
     print("driver: spherocyl_findjam_1p", flush=True)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
@@ -150,6 +193,14 @@ def test_find_jam_1p_ends_on_the_target_energy(tmp_path):
     t_end = int(os.path.basename(bins[0])[2:12])
     assert int(lines[-1].split()[0]) == t_end
     energies = np.array([float(l.split()[2]) for l in lines])
+    phi = np.array([float(l.split()[1]) for l in lines])
+    phi0 = _sd(str(tmp_path / "sd0000000000.dat"))[3]
+    # the random start overlaps (E > target): the first search expands
+    first = int(np.argmax(energies <= target)) + 1
+    want = phi0 / np.cumprod((1 + s0 * 1.1 ** np.arange(first)) ** 2)
+    np.testing.assert_allclose(phi[:first], want, rtol=0, atol=2e-9)  # printed with %.9f
+    s1 = s0 * 1.1 ** first / 2
+    assert phi[first] == pytest.approx(phi[first - 1] / (1 - s1) ** 2, abs=2e-9)
     crossings = np.nonzero(np.diff(np.sign(energies - target)))[0]
     assert len(crossings) >= 5
     # every line prints the packing with %.9f (resize_to_energy_1p's format)
@@ -181,7 +232,7 @@ def test_quasistatic_find_jam_cycles_and_compress(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "box_qs_1p.cpp"), "-L", pkg,
                     "-lspherocyls", f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
-    n, step, emax, emin, maxpack = 64, 2e-3, 1e-4, 1e-6, 0.86
+    n, step, emax, emin, maxpack = 64, 2e-3, 1e-4, 1e-6, 0.90
     print("box_qs_1p", n, flush=True)
     (tmp_path / "c").mkdir()
     r = subprocess.run([exe, str(tmp_path), str(n), "0.84", "5", str(step), str(emax), str(emin),
