@@ -207,7 +207,7 @@ def quasistatic_compression(eps=1e-4, phi0=0.80, phi1=0.86, n=16384, samples=13)
     out["curve_fp64"] = [[float(f64.packing[i, 0]), float(f64.pressure[i, 0]),
                           int(f64.iterations[i, 0])] for i in idx]
     p64, pmx = f64.pressure[:k, 0], mix.pressure[:k, 0]
-    jam = p64 > 1e-6 * np.max(np.abs(p64))
+    jam = (p64 > 1e-8) & (pmx > 1e-8)  # finite pressure; unjammed states read ~1e-11
     out["mixed_pressure_rel_error"] = {
         "max": float(np.max(np.abs(pmx[jam] - p64[jam]) / p64[jam])) if np.any(jam) else None,
         "median": float(np.median(np.abs(pmx[jam] - p64[jam]) / p64[jam])) if np.any(jam) else None,
@@ -243,15 +243,15 @@ def jammed_compression(n=1024, phi0=0.86, phi1=0.90, eps=1e-3, seed=3):
     f64, mix = curves["fp64"], curves["mixed"]
     k = min(len(f64.packing), len(mix.packing))
     p64, pmx = f64.pressure[:k, 0], mix.pressure[:k, 0]
-    jam = (p64 > 1e-6) & (pmx > 1e-6)
+    jam = (p64 > 1e-8) & (pmx > 1e-8)
     (x, y, phi), bx = final["mixed"]
     ob = oracle.OracleBox(bx["Lx"], x, y, phi, pk.A, pk.R, 0.1)
     ob.set_contact_model(1)
     ob.find_neighbors()
     ob.calc_se()
     o = ob.scalars()
-    p_orc = 0.5 * (o["cPxx"] + o["cPyy"])
-    p_rep = 0.5 * (bx["cPxx"] + bx["cPyy"])
+    p_orc = -0.5 * (o["cPxx"] + o["cPyy"])
+    p_rep = -0.5 * (bx["cPxx"] + bx["cPyy"])
     rel = np.abs(pmx[jam] - p64[jam]) / p64[jam]
     return {"workload": f"N={n} bidisperse aspect 2, phi {phi0} -> {phi1}, eps {eps}, "
                         "two-point FIRE to 1e-10",

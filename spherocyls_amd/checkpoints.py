@@ -14,7 +14,11 @@ state it reached, so the error that FP32 pair forces bring into the reported
 stress is that of the trajectory: an FP64 run from the same start steps beside
 the mixed one, and at each checkpoint the relative difference of their energy
 and stress is recorded (for the shear stress also relative to the pressure,
-since Pxy is small beside Pxx and Pyy).  The final minimised state of each
+since Pxy is small beside Pxx and Pyy).  Near jamming the sheared dynamics is
+chaotic, so two trajectories that start a rounding error apart separate
+anyway: a second FP64 run, from the start with every x displaced by a seeded
+1e-12, gives that intrinsic spread at each checkpoint, beside which the
+mixed-precision difference is read.  The final minimised state of each
 checkpoint is evaluated in FP64 by both paths alike.
 
     python -m spherocyls_amd.checkpoints [--checkpoints 10] [--every 1000]
@@ -45,17 +49,23 @@ def run(pk=None, checkpoints=10, every=1000, strain_rate=1e-6, dt=0.01, precisio
     params = native.fire_params(max_iter=200000) if params is None else params
     net = warmstart.load() if warm_start and os.path.exists(warmstart.DEFAULT_WEIGHTS) else None
     s = native.System(pk, pads=0.1)
-    # the FP64 trajectory from the same start, stepped beside the run
+    # the FP64 trajectory from the same start, stepped beside the run, and an
+    # FP64 trajectory from a start displaced by 1e-12 (the intrinsic spread)
     ref = native.System(pk, pads=0.1) if precision != "fp64" else None
+    kick = np.random.default_rng(12).standard_normal(pk.n) * 1e-12
+    pk_kicked = configs.Packing(pk.L, pk.x + kick, pk.y, pk.phi, pk.A, pk.R,
+                                pk.packing_fraction, pk.name)
+    ref2 = native.System(pk_kicked, pads=0.1) if precision != "fp64" else None
     recs = []
     heun_s = 0.0
     try:
         s.set_dynamics(strain_rate=strain_rate, step=dt)
         s.set_precision(precision)
         s.autotune(reps=5)
-        if ref is not None:
-            ref.set_dynamics(strain_rate=strain_rate, step=dt)
-            ref.autotune(reps=5)
+        for r in (ref, ref2):
+            if r is not None:
+                r.set_dynamics(strain_rate=strain_rate, step=dt)
+                r.autotune(reps=5)
         for c in range(checkpoints):
             s.synchronize()
             t0 = time.perf_counter()
@@ -69,11 +79,19 @@ def run(pk=None, checkpoints=10, every=1000, strain_rate=1e-6, dt=0.01, precisio
                 ref.strain_steps(every)
                 ref.calc_se()
                 b64 = ref.box()
+                ref2.strain_steps(every)
+                ref2.calc_se()
+                b_kick = ref2.box()
             else:
-                b64 = b_run
-            err = {k: abs(b_run[k] - b64[k]) / max(abs(b64[k]), 1e-300) for k in SE_KEYS}
-            pressure = 0.5 * abs(b64["Pxx"] + b64["Pyy"])
-            err["Pxy_vs_pressure"] = abs(b_run["Pxy"] - b64["Pxy"]) / max(pressure, 1e-300)
+                b64 = b_kick = b_run
+
+            def rel(b):
+                e = {k: abs(b[k] - b64[k]) / max(abs(b64[k]), 1e-300) for k in SE_KEYS}
+                pressure = 0.5 * abs(b64["Pxx"] + b64["Pyy"])
+                e["Pxy_vs_pressure"] = abs(b["Pxy"] - b64["Pxy"]) / max(pressure, 1e-300)
+                return e
+
+            err, spread = rel(b_run), rel(b_kick)
             b64 = b_run  # the state minimised below is the run's
             # minimisation checkpoint of a copy of the state
             x, y, phi = s.positions()
@@ -87,7 +105,8 @@ def run(pk=None, checkpoints=10, every=1000, strain_rate=1e-6, dt=0.01, precisio
             else:
                 sec, it, fr, fb = psec, pit, pfr, []
             rec = {"step": (c + 1) * every, "gamma_total": b64["gamma_total"],
-                   "se_rel_error_vs_fp64_trajectory": err, "E": b64["E"], "Pxx": b64["Pxx"],
+                   "se_rel_error_vs_fp64_trajectory": err,
+                   "se_rel_spread_fp64_from_kicked_start": spread, "E": b64["E"], "Pxx": b64["Pxx"],
                    "Pyy": b64["Pyy"], "Pxy": b64["Pxy"], "min_seconds": sec,
                    "min_iterations": int(it[0]), "min_frms": float(fr[0]),
                    "min_converged": bool(fr[0] < params[8]), "fallback": bool(fb),
@@ -98,17 +117,21 @@ def run(pk=None, checkpoints=10, every=1000, strain_rate=1e-6, dt=0.01, precisio
                 log(json.dumps(rec))
     finally:
         s.close()
-        if ref is not None:
-            ref.close()
+        for r in (ref, ref2):
+            if r is not None:
+                r.close()
         if net is not None:
             net.close()
     steps = checkpoints * every
     worst = {k: max(r["se_rel_error_vs_fp64_trajectory"][k] for r in recs)
              for k in (*SE_KEYS, "Pxy_vs_pressure")}
+    worst_spread = {k: max(r["se_rel_spread_fp64_from_kicked_start"][k] for r in recs)
+                    for k in (*SE_KEYS, "Pxy_vs_pressure")}
     return {"workload": pk.name, "n_particles": pk.n, "precision": precision,
             "steps": steps, "checkpoints": checkpoints, "heun_seconds": heun_s,
             "particle_steps_per_s": pk.n * steps / heun_s,
             "se_rel_error_vs_fp64_trajectory_max": worst,
+            "se_rel_spread_fp64_from_kicked_start_max": worst_spread,
             "minimisations_converged": sum(r["min_converged"] for r in recs),
             "min_seconds_mean": float(np.mean([r["min_seconds"] for r in recs])),
             "min_iterations_mean": float(np.mean([r["min_iterations"] for r in recs])),

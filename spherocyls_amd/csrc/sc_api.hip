@@ -314,9 +314,8 @@ void build_pblocks(sc_system* s) {
   for (int k = 0; k < s->npack; ++k) {
     const PackInfo& P = s->packs[k];
     std::vector<int2> spatial;
-    // a workgroup owns one particle per thread (its wavefronts run the pair
-    // phase in list_lanes rounds)
-    const int per_block = s->list_block;
+    // list_lanes threads share a particle
+    const int per_block = s->list_block / s->list_lanes;
     for (int st = P.poff; st < P.poff + P.n; st += per_block) spatial.push_back(make_int2(k, st));
     s->list_ranges[k] = make_int2((int)s->pblocks.size(), (int)spatial.size());
     auto launch = xcd_order(spatial, (int)s->pblocks.size());

@@ -216,49 +216,26 @@ __device__ __forceinline__ void pair_into(double xi, double yi, double nxi, doub
   }
 }
 
-// LANES threads share a particle in its pair phase: lane l takes list entries
-// l, l + LANES, ... in batches of kGather whose loads are all issued before
-// any is used, so a lane has several independent gathers in flight.  With two
-// lanes the partners that pass the prefilter are then dealt out evenly between
-// the two lanes (each evaluates half of the pair's hits, in row order).  The
-// LANES partial sums are combined with xor shuffles (fixed order for a given
-// LANES), after which every lane of the group holds the particle's sums.
-//
-// Every wavefront owns 64 particles and runs LANES rounds of the pair phase:
-// in round r, lane group s (lanes s LANES .. s LANES + LANES - 1) evaluates
-// particle r (64 / LANES) + s of the wavefront, and lane s LANES + r keeps its
-// sums.  The per-particle epilogue (integration, segment vector, binning,
-// rebuild test) then runs on all 64 lanes at once, one particle each, instead
-// of on one lane in LANES while the others idle.
+// LANES threads share a particle: lane l takes list entries l, l + LANES, ...
+// in batches of kGather whose loads are all issued before any is used, so a
+// lane has several independent gathers in flight.  The LANES partial sums are
+// combined with xor shuffles (fixed order for a given LANES).
 constexpr int kGather = 4;
-
-// Cosine of twice the angle from the segment vector (nx, ny) = A (cos phi,
-// sin phi) the state carries: (nx^2 - ny^2) / A^2.  A disc (A = 0) has RC = 0,
-// so its isolated rotation does not depend on the angle.
-__device__ __forceinline__ double cos2_from_segment(double nx, double ny, double A) {
-#pragma clang fp contract(off)
-  return A > 0.0 ? (nx * nx - ny * ny) / (A * A) : 1.0;
-}
-
-// isolated_rotation, src/cpp/strain_dynamics.cpp:174-177, with cos(2 phi) from
-// the segment vector
-__device__ __forceinline__ double iso_rot_seg(double RC, double nx, double ny, double A) {
-#pragma clang fp contract(off)
-  return 0.5 * (1 - RC * cos2_from_segment(nx, ny, A));
-}
 
 // MIXED: the kernel carries the FP32-force pair path besides the FP64 one and
 // picks per packing (a mixed-mode FIRE moves packings to FP64 one by one).
 // FP64 systems launch the MIXED = false kernel, whose single pair path needs
 // fewer registers.  Its force pass and corrector are held to 128 VGPRs (4
-// wavefronts per SIMD instead of 3).
+// wavefronts per SIMD instead of 3, at the cost of a few spilled dwords
+// outside the pair loop): 1.2 us off the corrector at N = 262144 and 5% off
+// the minimiser's force pass.  The predictor fits in 118 without the bound.
 // SE: the plain force pass accumulates stress and energy over its contacts
 // (calc_se, recorded steps).  The minimiser's passes need only the residual
 // and FIRE sums, which are formed after the pair loop; their SE = false
 // kernel keeps no accumulators live across the loop.
 // BOX: the Heun passes change the box (resize, pure shear).  Sheared Heun
 // steps run the BOX = false instance, whose epilogue carries only the
-// Lees-Edwards path.
+// Lees-Edwards path (corrector 114 VGPRs instead of 128 plus spills).
 template <int MODE, int LANES, bool TWO_POINT, bool MIXED, bool SE, bool BOX>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(MODE != kModePred && !MIXED ? 4 : 1))) k_list(ListArgs a) {
@@ -278,11 +255,9 @@ __global__ void __launch_bounds__(256)
   const PackInfo P = a.packs[pk];
   const PackScal S = a.scal[pk];
   if (MODE == kModeForces && (a.do_fire || a.do_gd) && S.converged) return;
-  constexpr int kGroups = 64 / LANES;     // particles per round in a wavefront
-  const int lane = tid % LANES;           // lane within the particle's group
-  const int grp = (tid & 63) / LANES;     // group within the wavefront
-  const int wbase = pb.y + (tid & ~63);   // first particle of the wavefront
-  const int pend = P.poff + P.n;
+  const int lane = tid % LANES;
+  const int g = pb.y + tid / LANES;
+  const bool active = g < P.poff + P.n;
 
   // Only the plain force pass reduces (stress/energy records, FIRE/GD sums,
   // residual); the Heun passes carry no accumulators, which keeps their
@@ -325,22 +300,12 @@ __global__ void __launch_bounds__(256)
     }
   }
   const bool fp64 = !MIXED || S.prec == kFP64;
-  // the rows are rebuilt in this pass (whole packing): every row's build
-  // position is then its evaluated position and the window centre the strain
-  const bool rebuild = a.flag_in[pk] != 0;
 
   double acc[kSums ? kNSum : 1];
 #pragma unroll
   for (int i = 0; i < (kSums ? kNSum : 1); ++i) acc[i] = 0.0;
 
-  // ---- pair phase: LANES rounds, one particle per lane group and round.
-  // The sums a lane keeps for its epilogue particle wait in LDS (s_keep), so
-  // no registers stay live across the rounds.
-  __shared__ double s_keep[3][256];
-#pragma nounroll
-  for (int r = 0; r < LANES; ++r) {
-    const int g = wbase + r * kGroups + grp;
-    if (g >= pend) continue;  // the group's lanes skip the round together
+  if (active) {
     const double xi = a.ex[g], yi = a.ey[g];
     const double nxi = a.enx[g], nyi = a.eny[g];
     const double Ri = a.b.R[g], Ai = a.b.A[g];
@@ -350,8 +315,11 @@ __global__ void __launch_bounds__(256)
     // ---- list (rebuilt here when the previous pass asked for it)
     // Every lane of the particle builds the same row (identical stores), so a
     // lane only ever reads back what it wrote itself; rebuilds are rare.
+    const bool rebuild = a.flag_in[pk] != 0;
+    double center;
     int n;
     if (rebuild) {
+      center = gam;
       n = build_row(a, P, Lx, Ly, g, xi, yi, reach_i, gam - a.gamma_window, gam + a.gamma_window);
       if (lane == 0) {
         a.xb[g] = xi;
@@ -360,16 +328,17 @@ __global__ void __launch_bounds__(256)
       }
     } else {
       n = a.nbr_n[g];
+      center = a.list_center[pk];
     }
 
     // ---- forces over the row, in two phases.  Phase 1 walks the row with
     // batched gathers and keeps the partners that pass the bounding-circle
     // test in this thread's LDS column; phase 2 runs the closest-approach math
-    // over the compacted hits.  A wavefront then executes the expensive pair
+    // over that compacted list.  A wavefront then executes the expensive pair
     // math as often as its busiest lane has real candidates, not as often as
-    // its longest row.  Hits are visited in row order within each lane's
-    // share, so the sums have a fixed order.
-    double fx = 0.0, fy = 0.0, tq = 0.0;
+    // its longest row; partners are visited in row order either way, so the
+    // sums are the same.
+    double Fx = 0.0, Fy = 0.0, Tq = 0.0;
     int k0 = lane;  // this lane's next row entry
     while (true) {
       int nh = 0;
@@ -399,61 +368,34 @@ __global__ void __launch_bounds__(256)
           ++nh;
         }
       }
-      // this lane's share of the piece's hits: indices [lo, hi) of the
-      // concatenation (lane 0's column: n0 hits, then lane 1's column)
-      int c0 = tid, n0 = nh, lo = 0, hi = nh;
-      if (LANES == 2) {
-        // the pair of lanes deals its hits out evenly: lane 0 takes the
-        // first ceil(T/2), lane 1 the rest
-        const int other = __shfl_xor(nh, 1);
-        const int total = nh + other, half = (total + 1) >> 1;
-        n0 = lane == 0 ? nh : other;
-        c0 = tid - lane;
-        lo = lane == 0 ? 0 : half;
-        hi = lane == 0 ? half : total;
-        // the other lane's column entries were written by this wavefront
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      for (int h = lo; h < hi; ++h) {
-        const bool first = h < n0;
-        const int q = s_hit[(first ? h : h - n0) * nthr + (first ? c0 : c0 + 1)];
+      for (int h = 0; h < nh; ++h) {
+        const int q = s_hit[h * nthr + tid];
         pair_into<kSums && SE, TWO_POINT>(xi, yi, nxi, nyi, Ri, Ai, idi, a.ex[q], a.ey[q], a.enx[q],
-                                          a.eny[q], a.b.R[q], a.b.A[q], a.b.id[q], Lx, Ly, gam,
-                                          fp64, do_se, fx, fy, tq, acc);
+                                    a.eny[q], a.b.R[q], a.b.A[q], a.b.id[q], Lx, Ly, gam, fp64,
+                                    do_se, Fx, Fy, Tq, acc);
       }
-      if (LANES == 2) {
-        // both lanes of the pair leave the piece loop together
-        const int more = k0 < n ? 1 : 0;
-        if (!(more | __shfl_xor(more, 1))) break;
-        __builtin_amdgcn_wave_barrier();
-      } else if (k0 >= n) {
-        break;
-      }
+      if (k0 >= n) break;
     }
     if (LANES > 1) {
 #pragma unroll
       for (int off = LANES >> 1; off > 0; off >>= 1) {
-        fx += __shfl_xor(fx, off);
-        fy += __shfl_xor(fy, off);
-        tq += __shfl_xor(tq, off);
+        Fx += __shfl_xor(Fx, off);
+        Fy += __shfl_xor(Fy, off);
+        Tq += __shfl_xor(Tq, off);
       }
     }
-    if (lane == r) {
-      s_keep[0][tid] = fx;
-      s_keep[1][tid] = fy;
-      s_keep[2][tid] = tq;
-    }
-  }
-
-  // ---- per-particle epilogue: all lanes, particle lane (64 / LANES) + group
-  const int g = wbase + lane * kGroups + grp;
-  if (g < pend) {
-    const double Fx = s_keep[0][tid], Fy = s_keep[1][tid], Tq = s_keep[2][tid];
+    // ---- per-particle epilogue (lane 0)
     const double half_skin = 0.5 * P.pad;
+    // build position of the row (written above by this lane when it rebuilt
+    // the row), loaded only here so it is not live across the pair loop
+    double xbi = 0.0, ybi = 0.0;
+    if (MODE != kModeForces && lane == 0) {
+      xbi = a.xb[g];
+      ybi = a.yb[g];
+    }
     bool raise = false;
-    if (MODE == kModeForces) {
+    if (lane != 0) {
+    } else if (MODE == kModeForces) {
       a.b.fx[g] = Fx;
       a.b.fy[g] = Fy;
       a.b.tau[g] = Tq;
@@ -466,145 +408,135 @@ __global__ void __launch_bounds__(256)
         acc[kSumG % (kSums ? kNSum : 1)] += Fx * Fx + Fy * Fy + gp * gp;
         acc[kSumV % (kSums ? kNSum : 1)] += vx * vx + vy * vy + vp * vp;
       }
-    } else {
-      // the row's build position and the window centre (this pass's evaluated
-      // state when it rebuilt the rows)
-      const double xi = a.ex[g], yi = a.ey[g];
-      const double xbi = rebuild ? xi : a.xb[g], ybi = rebuild ? yi : a.yb[g];
-      const double center = rebuild ? gam : a.list_center[pk];
-      const double Ai = a.b.A[g];
-      if (MODE == kModePred) {
-        const double p0 = a.b.phi[g];
-        double v0x, v0y, v0t, qx, qy, qp;
-        if (box_pure) {
-          // pure_euler_est_2p (src/cuda/2point_dynamics.cu:1885-1890), mobility 1:
-          // the affine field (+x/2, -y/2) times the rate, and the rod's rotation
-          // by the strain, -RC sin(2 phi) / 2 times the rate; v0 keeps the force
-          // part for the corrector
-          v0x = Fx;
-          v0y = Fy;
-          v0t = Tq / a.b.I[g];
-          qx = xi + dt * (Fx + 0.5 * gd * xi);
-          qy = yi + dt * (Fy - 0.5 * gd * yi);
-          qp = p0 + dt * (v0t + gd * (-0.5 * a.b.RC[g] * sin(2.0 * p0)));
-        } else if (!box_resize) {
-          // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
-          v0x = Fx - S.gamma * Fy;
-          v0y = Fy;
-          v0t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], a.enx[g], a.eny[g], Ai);
-          qx = xi + dt * v0x;
-          qy = yi + dt * v0y;
-          qp = p0 + dt * v0t;
+    } else if (MODE == kModePred) {
+      const double p0 = a.b.phi[g];
+      double v0x, v0y, v0t, qx, qy, qp;
+      if (box_pure) {
+        // pure_euler_est_2p (src/cuda/2point_dynamics.cu:1885-1890), mobility 1:
+        // the affine field (+x/2, -y/2) times the rate, and the rod's rotation
+        // by the strain, -RC sin(2 phi) / 2 times the rate; v0 keeps the force
+        // part for the corrector
+        v0x = Fx;
+        v0y = Fy;
+        v0t = Tq / a.b.I[g];
+        qx = xi + dt * (Fx + 0.5 * gd * xi);
+        qy = yi + dt * (Fy - 0.5 * gd * yi);
+        qp = p0 + dt * (v0t + gd * (-0.5 * a.b.RC[g] * sin(2.0 * p0)));
+      } else if (!box_resize) {
+        // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
+        v0x = Fx - S.gamma * Fy;
+        v0y = Fy;
+        v0t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], p0);
+        qx = xi + dt * v0x;
+        qy = yi + dt * v0y;
+        qp = p0 + dt * v0t;
+      } else {
+        // resize_step predictor, src/cpp/resize_dynamics.cpp:174-190
+        v0x = Fx;
+        v0y = Fy;
+        v0t = Tq;
+        qx = xi + dt * v0x;
+        qy = yi + dt * v0y;
+        qp = p0 + dt * v0t / a.b.I[g];
+        if (box_shrink) {
+          qx *= P.resize_scale;
+          qy *= P.resize_scale;
         } else {
-          // resize_step predictor, src/cpp/resize_dynamics.cpp:174-190
-          v0x = Fx;
-          v0y = Fy;
-          v0t = Tq;
-          qx = xi + dt * v0x;
-          qy = yi + dt * v0y;
-          qp = p0 + dt * v0t / a.b.I[g];
-          if (box_shrink) {
-            qx *= P.resize_scale;
-            qy *= P.resize_scale;
-          } else {
-            qx /= P.resize_scale;
-            qy /= P.resize_scale;
-          }
+          qx /= P.resize_scale;
+          qy /= P.resize_scale;
         }
-        a.px[g] = qx;
-        a.py[g] = qy;
-        a.pphi[g] = qp;
-        double sn, cs;
-        sincos(qp, &sn, &cs);
-        a.pnx[g] = Ai * cs;
-        a.pny[g] = Ai * sn;
-        a.v0x[g] = v0x;
-        a.v0y[g] = v0y;
-        a.v0t[g] = v0t;
-        const double Lbx = (box_resize || box_pure) ? L_new_x : Lx;
-        const double Lby = (box_resize || box_pure) ? L_new_y : Ly;
-        double wx = qx, wy = qy;
-        bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
-        if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
-        bin_into(g, wx, wy, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
-        // the corrector evaluates x~ at gamma + dt gdot, with the rescaled box
-        // length in resize runs
-        const double gnext = S.gamma + dt * gd;
-        raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
-                fabs(min_image(qy - ybi, Lby)) >= half_skin ||
-                fabs(gnext - center) >= a.gamma_window || box_resize || box_pure;
-      } else {  // kModeCorr
-        double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
-        double Lbx = Lx, Lby = Ly;
-        bool remap = false;
-        if (box_pure) {
-          // pure_heun_corr_2p (src/cuda/2point_dynamics.cu:2141-2166): the
-          // predicted state plus half the change of the force part
-          x1 = a.px[g] + 0.5 * dt * (Fx - a.v0x[g]);
-          y1 = a.py[g] + 0.5 * dt * (Fy - a.v0y[g]);
-          p1 = a.pphi[g] + 0.5 * dt * (Tq / a.b.I[g] - a.v0t[g]);
-          Lbx = L_new_x;
-          Lby = L_new_y;
-        } else if (!box_resize) {
-          // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215; the
-          // isolated rotation at the predicted angle, from its segment vector
-          double v1x = Fx - gam * Fy;
-          double v1y = Fy;
-          double v1t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], a.enx[g], a.eny[g], Ai);
-          x1 += 0.5 * dt * (a.v0x[g] + v1x);
-          y1 += 0.5 * dt * (a.v0y[g] + v1y);
-          p1 += 0.5 * dt * (a.v0t[g] + v1t);
-          double gnew = S.gamma + gd * dt;
-          if (gnew >= 0.5 * le_ratio) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
-            remap = true;
-            if (y1 < 0) y1 += S.Ly;
-            else if (y1 > S.Ly) y1 -= S.Ly;
-            x1 += le_ratio * y1;
-            if (x1 < 0) x1 += S.Lx;
-            else {
-              int guard = 0;
-              while (x1 > S.Lx && guard++ < 8) x1 -= S.Lx;
-            }
-          }
-          Lbx = S.Lx;
-          Lby = S.Ly;
-        } else {
-          // resize_step corrector, src/cpp/resize_dynamics.cpp:196-215
-          x1 += 0.5 * dt * (a.v0x[g] + Fx);
-          y1 += 0.5 * dt * (a.v0y[g] + Fy);
-          p1 += 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
-          if (box_shrink) {
-            x1 *= P.resize_scale;
-            y1 *= P.resize_scale;
-          } else {
-            x1 /= P.resize_scale;
-            y1 /= P.resize_scale;
-          }
-          Lbx = L_new_x;
-          Lby = L_new_y;
-        }
-        bool okx = wrap_coord(x1, Lbx), oky = wrap_coord(y1, Lby);
-        if (!okx || !oky || !isfinite(p1)) atomicOr(a.err, kErrNonFinite);
-        a.b.x[g] = x1;
-        a.b.y[g] = y1;
-        a.b.phi[g] = p1;
-        double sn, cs;
-        sincos(p1, &sn, &cs);
-        a.b.nx[g] = Ai * cs;
-        a.b.ny[g] = Ai * sn;
-        bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
-        double gnext = box_pure ? S.gamma : S.gamma + gd * dt;
-        if (remap) gnext -= le_ratio;
-        raise = remap || box_resize || box_pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
-                fabs(min_image(y1 - ybi, Lby)) >= half_skin ||
-                fabs(gnext - center) >= a.gamma_window;
       }
+      a.px[g] = qx;
+      a.py[g] = qy;
+      a.pphi[g] = qp;
+      double sn, cs;
+      sincos(qp, &sn, &cs);
+      a.pnx[g] = Ai * cs;
+      a.pny[g] = Ai * sn;
+      a.v0x[g] = v0x;
+      a.v0y[g] = v0y;
+      a.v0t[g] = v0t;
+      const double Lbx = (box_resize || box_pure) ? L_new_x : Lx;
+      const double Lby = (box_resize || box_pure) ? L_new_y : Ly;
+      double wx = qx, wy = qy;
+      bool okx = wrap_coord(wx, Lbx), oky = wrap_coord(wy, Lby);
+      if (!okx || !oky || !isfinite(qp)) atomicOr(a.err, kErrNonFinite);
+      bin_into(g, wx, wy, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
+      // the corrector evaluates x~ at gamma + dt gdot, with the rescaled box
+      // length in resize runs
+      const double gnext = S.gamma + dt * gd;
+      raise = fabs(min_image(qx - xbi, Lbx)) >= half_skin ||
+              fabs(min_image(qy - ybi, Lby)) >= half_skin ||
+              fabs(gnext - center) >= a.gamma_window || box_resize || box_pure;
+    } else {  // kModeCorr
+      double x1 = a.b.x[g], y1 = a.b.y[g], p1 = a.b.phi[g];
+      double Lbx = Lx, Lby = Ly;
+      bool remap = false;
+      if (box_pure) {
+        // pure_heun_corr_2p (src/cuda/2point_dynamics.cu:2141-2166): the
+        // predicted state plus half the change of the force part
+        x1 = a.px[g] + 0.5 * dt * (Fx - a.v0x[g]);
+        y1 = a.py[g] + 0.5 * dt * (Fy - a.v0y[g]);
+        p1 = a.pphi[g] + 0.5 * dt * (Tq / a.b.I[g] - a.v0t[g]);
+        Lbx = L_new_x;
+        Lby = L_new_y;
+      } else if (!box_resize) {
+        // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
+        double v1x = Fx - gam * Fy;
+        double v1y = Fy;
+        double v1t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], a.pphi[g]);
+        x1 += 0.5 * dt * (a.v0x[g] + v1x);
+        y1 += 0.5 * dt * (a.v0y[g] + v1y);
+        p1 += 0.5 * dt * (a.v0t[g] + v1t);
+        double gnew = S.gamma + gd * dt;
+        if (gnew >= 0.5 * le_ratio) {  // set_back_gamma, src/cpp/spherocyl_box.cpp:529-540
+          remap = true;
+          if (y1 < 0) y1 += S.Ly;
+          else if (y1 > S.Ly) y1 -= S.Ly;
+          x1 += le_ratio * y1;
+          if (x1 < 0) x1 += S.Lx;
+          else {
+            int guard = 0;
+            while (x1 > S.Lx && guard++ < 8) x1 -= S.Lx;
+          }
+        }
+        Lbx = S.Lx;
+        Lby = S.Ly;
+      } else {
+        // resize_step corrector, src/cpp/resize_dynamics.cpp:196-215
+        x1 += 0.5 * dt * (a.v0x[g] + Fx);
+        y1 += 0.5 * dt * (a.v0y[g] + Fy);
+        p1 += 0.5 * dt * (a.v0t[g] + Tq) / a.b.I[g];
+        if (box_shrink) {
+          x1 *= P.resize_scale;
+          y1 *= P.resize_scale;
+        } else {
+          x1 /= P.resize_scale;
+          y1 /= P.resize_scale;
+        }
+        Lbx = L_new_x;
+        Lby = L_new_y;
+      }
+      bool okx = wrap_coord(x1, Lbx), oky = wrap_coord(y1, Lby);
+      if (!okx || !oky || !isfinite(p1)) atomicOr(a.err, kErrNonFinite);
+      a.b.x[g] = x1;
+      a.b.y[g] = y1;
+      a.b.phi[g] = p1;
+      double sn, cs;
+      sincos(p1, &sn, &cs);
+      a.b.nx[g] = Ai * cs;
+      a.b.ny[g] = Ai * sn;
+      bin_into(g, x1, y1, Lbx, Lby, P, a.cnt_fill, a.slot_fill, a.err);
+      double gnext = box_pure ? S.gamma : S.gamma + gd * dt;
+      if (remap) gnext -= le_ratio;
+      raise = remap || box_resize || box_pure || fabs(min_image(x1 - xbi, Lbx)) >= half_skin ||
+              fabs(min_image(y1 - ybi, Lby)) >= half_skin || fabs(gnext - center) >= a.gamma_window;
     }
     if (raise && a.flag_out) a.flag_out[pk] = 1;
   }
 
   // ---- scalar advance (corrector: one thread per packing)
-  if (MODE == kModeCorr && g == P.poff) {
+  if (MODE == kModeCorr && g == P.poff && lane == 0) {
     // copied word by word and updated in place: the struct never lives in
     // registers across the kernel (it would spill)
     const int* src = reinterpret_cast<const int*>(&a.scal[pk]);

@@ -25,7 +25,7 @@ from . import native
 class Curve:
     """One row per step (row 0: the minimised start) and packing."""
     packing: np.ndarray     # [steps+1, npack]
-    pressure: np.ndarray    # (cPxx + cPyy) / 2, corrected virial
+    pressure: np.ndarray    # -(cPxx + cPyy) / 2, corrected virial: positive under compression
     energy: np.ndarray
     iterations: np.ndarray  # FIRE iterations of each minimisation
     frms: np.ndarray
@@ -34,7 +34,9 @@ class Curve:
 
 def _record(s):
     b = [s.box(k) for k in range(s.npack)]
-    return (np.array([x["packing"] for x in b]), np.array([0.5 * (x["cPxx"] + x["cPyy"]) for x in b]),
+    # the CPU code's stress is the virial sum of f (x_A - x_B), negative for
+    # repulsive contacts: the pressure is minus its mean diagonal
+    return (np.array([x["packing"] for x in b]), np.array([-0.5 * (x["cPxx"] + x["cPyy"]) for x in b]),
             np.array([x["E"] for x in b]))
 
 

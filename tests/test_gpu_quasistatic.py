@@ -92,11 +92,61 @@ def test_compress_qs2p_files(tmp_path):
     assert len(lines[1].split()[1].split(".")[1]) == 9
 
 
+def _check_search(lines, phi0, target, s0, factor, grow_after):
+    """Replay the bracketing search of resize_to_energy_{1p,2p} on the SE
+    lines and check every line's packing.  A search compresses while
+    E < target (expands while E > target) with steps s, 1.1 s, ...; the next
+    search starts from s 1.1^(steps + grow_after) / 2 in the other direction.
+    factor(compress, s) is one step's packing ratio; each line is checked
+    against the line before it, so the tolerance is the printed precision.
+    Searches are told apart by the direction of the packing's change, so the
+    replay stops where a step no longer shows in the printed digits; the
+    energies (printed with 6 digits) must lie on the search's side of the
+    target until its last line, which crosses it."""
+    digits = len(lines[0].split()[1].split(".")[1])
+    res = 10.0 ** -digits
+    t = [int(l.split()[0]) for l in lines]
+    phi = [float(l.split()[1]) for l in lines]
+    E = [float(l.split()[2]) for l in lines]
+    assert t == list(range(1, len(lines) + 1))
+    compress = phi[0] > phi0  # the random start overlaps: the first search expands
+    i, start, searches = 0, s0, 0
+    while i < len(lines):
+        prev = phi0 if i == 0 else phi[i - 1]
+        if 2 * start * prev < 40 * res:
+            break  # the steps no longer show in the printed packing
+        s, steps = start, 0
+        while i < len(lines):
+            prev = phi0 if i == 0 else phi[i - 1]
+            if steps and (phi[i] > prev) != compress:
+                break  # the next search
+            assert phi[i] == pytest.approx(prev * factor(compress, s), abs=2.5 * res), (i, s)
+            i += 1
+            steps += 1
+            s *= 1.1
+        # every line but the last stays on the search's side of the target
+        side = E[i - steps:i - 1]
+        if compress:
+            assert all(e < target * (1 + 1e-5) for e in side) and E[i - 1] >= target * (1 - 1e-5)
+        else:
+            assert all(e > target * (1 - 1e-5) for e in side) and E[i - 1] <= target * (1 + 1e-5)
+        searches += 1
+        start = start * 1.1 ** (steps + grow_after) / 2
+        compress = not compress
+    return searches
+
+
 def test_find_jam_2p_ends_on_the_target_energy(tmp_path):
     target, n = 1e-6, 64
     _run("spherocyl_findjam", ["r", str(tmp_path), str(n), "1e-3", "0.01", str(target), "0.01",
                                "0.01", "0.1", "0.80", "1", "1.4"])
     lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
+    # two-point steps: x, y, L times 1 - s, or 1 / (1 - s) when expanding
+    # (resize_step_2p), so the packing changes by (1 - s)^-2 or (1 - s)^2;
+    # the next search starts from s 1.1^(steps - 1) / 2
+    phi0 = _sd(str(tmp_path / "sd0000000000.dat"))[3]
+    assert _check_search(lines, phi0, target, 1e-3,
+                         lambda c, s: (1 - s) ** -2 if c else (1 - s) ** 2, -1) >= 5
     bins = glob.glob(str(tmp_path / "sd*.bin"))
     assert len(bins) == 1
     h, a = native.read_positions_bin(bins[0])
@@ -124,32 +174,26 @@ def test_find_jam_2p_ends_on_the_target_energy(tmp_path):
 def test_y_compress_energy_2p_moves_only_y(tmp_path):
     """y_compress_energy_2p (src/cuda/2point_dynamics.cu:4458-4632) through
     spherocyl_ycompress (argument order of main_2point_ycompress.cpp): the
-    bracketing search of find_jam_2p with steps that scale y and Ly only.  Lx
-    stays; the first search compresses with steps s0 1.1^k (packing / (1 - s));
-    the second expands from s0 1.1^(steps - 1) / 2 by 1 / (1 - s) (packing
-    times 1 - s); the final state is a minimum on the target energy."""
+    bracketing search of find_jam_2p with steps that scale y and Ly only: Lx
+    stays, every step changes the packing by the factor of its Ly change, and
+    the final state is a minimum on the target energy."""
     target, n, s0 = 1e-6, 64, 1e-3
     _run("spherocyl_ycompress", ["r", str(tmp_path), str(n), str(s0), "0.01", str(target), "0.01",
                                  "0.01", "0.1", "0.80", "1", "1.4"])
     n0, Lx0, Ly0, phi0, _ = _sd(str(tmp_path / "sd0000000000.dat"))
     assert Lx0 == Ly0
     lines = open(tmp_path / "sd_stress_energy.dat").read().splitlines()
-    t = np.array([int(l.split()[0]) for l in lines])
     phi = np.array([float(l.split()[1]) for l in lines])
     E = np.array([float(l.split()[2]) for l in lines])
-    assert np.array_equal(t, np.arange(1, len(t) + 1))
-    first = int(np.argmax(E >= target)) + 1  # steps of the first search
-    assert first >= 2 and np.all(E[:first - 1] < target)
-    want = phi0 / np.cumprod(1 - s0 * 1.1 ** np.arange(first))
-    np.testing.assert_allclose(phi[:first], want, rtol=0, atol=3e-10)  # printed with %.10f
-    s1 = s0 * 1.1 ** (first - 1) / 2
-    assert phi[first] == pytest.approx(phi[first - 1] * (1 - s1), abs=3e-10)
+    # y-only steps: the packing changes by (1 - s)^-1 or (1 - s)
+    assert _check_search(lines, phi0, target, s0,
+                         lambda c, s: 1 / (1 - s) if c else 1 - s, -1) >= 5
     crossings = np.nonzero(np.diff(np.sign(E - target)))[0]
     assert len(crossings) >= 5
     bins = glob.glob(str(tmp_path / "sd*.bin"))
     assert len(bins) == 1
     h, a = native.read_positions_bin(bins[0])
-    assert h["Lx"] == pytest.approx(Lx0, rel=1e-13) and h["Ly"] < 0.99 * Ly0
+    assert h["Lx"] == pytest.approx(Lx0, rel=1e-13) and abs(h["Ly"] / Ly0 - 1) > 1e-3
     assert h["Ly"] == pytest.approx(Ly0 * phi0 / phi[-1], rel=1e-9)
     b = oracle.OracleBox(h["Lx"], a["x"], a["y"], a["phi"], a["A"], a["R"], 0.1)
     b.set_box(h["Lx"], h["Ly"])
@@ -157,7 +201,7 @@ def test_y_compress_energy_2p_moves_only_y(tmp_path):
     b.find_neighbors()
     b.calc_se()
     e = b.scalars()["E"]
-    print("y_compress_energy_2p: steps", t[-1], "Ly/Lx", h["Ly"] / h["Lx"], "E", e, flush=True)
+    print("y_compress_energy_2p: steps", len(lines), "Ly/Lx", h["Ly"] / h["Lx"], "E", e, flush=True)
     assert b.frms() < 1e-10
     assert e == pytest.approx(E[-1], rel=1e-5)
     assert abs(e - target) < 0.1 * target
@@ -195,12 +239,8 @@ def test_find_jam_1p_ends_on_the_target_energy(tmp_path):
     energies = np.array([float(l.split()[2]) for l in lines])
     phi = np.array([float(l.split()[1]) for l in lines])
     phi0 = _sd(str(tmp_path / "sd0000000000.dat"))[3]
-    # the random start overlaps (E > target): the first search expands
-    first = int(np.argmax(energies <= target)) + 1
-    want = phi0 / np.cumprod((1 + s0 * 1.1 ** np.arange(first)) ** 2)
-    np.testing.assert_allclose(phi[:first], want, rtol=0, atol=2e-9)  # printed with %.9f
-    s1 = s0 * 1.1 ** first / 2
-    assert phi[first] == pytest.approx(phi[first - 1] / (1 - s1) ** 2, abs=2e-9)
+    assert _check_search(lines, phi0, target, s0,
+                         lambda c, s: (1 - s) ** -2 if c else (1 + s) ** -2, 0) >= 5
     crossings = np.nonzero(np.diff(np.sign(energies - target)))[0]
     assert len(crossings) >= 5
     # every line prints the packing with %.9f (resize_to_energy_1p's format)
@@ -243,8 +283,10 @@ def test_quasistatic_find_jam_cycles_and_compress(tmp_path):
     t = np.array([int(l.split()[0]) for l in lines])
     phi = np.array([float(l.split()[1]) for l in lines])
     E = np.array([float(l.split()[2]) for l in lines])
-    # a new cycle starts where the time stamp skips one (t += 1 between cycles)
-    starts = list(np.nonzero(np.diff(t) == 2)[0] + 1)
+    assert np.all(np.diff(t) == 1)
+    # a cycle starts by minimising the state it is given, without a resize
+    # step: its first line repeats the packing of the line before
+    starts = list(np.nonzero(np.diff(phi) == 0)[0] + 1)
     bounds = [0] + starts + [len(t)]
     ncyc = len(bounds) - 1
     assert ncyc == int(np.ceil(np.log2(emax / emin)))  # dMaxE halves until <= dMinE
