@@ -243,6 +243,32 @@ int sc_timing_get(sc_system* s, int kernel, double* ms_total, long long* launche
 int sc_timing_reset(sc_system* s);
 int sc_synchronize(sc_system* s);
 
+/* Allocation-free calc_se.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The force / energy / stress pass
+ * of sc_calc_se (list path, one lane per particle, 256-thread workgroups) on
+ * one square packing whose arrays are already in device memory, in the caller's
+ * particle order: x, y, phi, A, R (n each, device) at strain gamma (|gamma| <=
+ * 1/2) in a box of side L with cell padding pad, contact model SC_CONTACT_*.
+ * Writes fx, fy, tau (n each, device) and, when not NULL, se[10] (device):
+ * E Pxx Pyy Pxy cPxx cPyy cPxy cPyx ncontacts F_rms as sc_get_box reports
+ * them, and *status (device int): SC_OK, or SC_ERR_CELL_OVERFLOW /
+ * SC_ERR_LIST_OVERFLOW / SC_ERR_ARG (non-finite coordinate) found on the
+ * device.  All working memory is one static device array of
+ * SC_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends): n above
+ * SC_NOALLOC_MAX_PARTICLES is SC_ERR_ARG, and a cell grid above
+ * SC_NOALLOC_MAX_CELLS cells is coarsened.  Launches on `stream` (a
+ * hipStream_t; NULL: the null stream), returns without synchronising, and
+ * calls no allocator.  Every call on one GPU shares that GPU's scratch array:
+ * calls that overlap in time on one GPU race (serialise them on one stream). */
+#define SC_NOALLOC_MAX_PARTICLES 32768
+#define SC_NOALLOC_MAX_CELLS 32768
+#define SC_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+int sc_calc_se_noalloc(int n, double L, double gamma, double pad, int contact_model,
+                       const double* x, const double* y, const double* phi, const double* A,
+                       const double* R, double* fx, double* fy, double* tau, double* se,
+                       int* status, void* stream);
+
 /* Seeded random placement (counterpart of SpherocylBox::random_configuration,
  * src/cpp/spherocyl_box.cpp:694-790, and its initialConfig modes,
  * src/cpp/spherocyl_box.h:8 plus the grid modes 4/5 of :718-737): centres

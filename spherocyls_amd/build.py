@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["sc_device.hip", "sc_list.hip", "sc_api.hip", "sc_mpnn.hip"]
+HIP_SOURCES = ["sc_device.hip", "sc_list.hip", "sc_api.hip", "sc_mpnn.hip", "sc_noalloc.hip"]
 CXX_SOURCES = [os.path.join(CSRC, "sc_configgen.cpp")]
 HOST_LIB_SOURCES = ["spherocyl_box.cpp", "dat_file.cpp"]
 DRIVERS = {
