@@ -138,6 +138,19 @@ int sc_get_tiling(const sc_system* s, int* tile_cols, int* block_threads);
 /* List path shape: workgroup size (64, 128 or 256) and lanes per particle
  * (1, 2, 4 or 8; more lanes give small systems more parallelism). */
 int sc_set_list_block(sc_system* s, int block_threads, int lanes);
+/* Cell padding (the list skin) of packing k (k = -1: all): the cell grid is
+ * rebuilt for it (pad 0 takes the grid's slack, as at creation) and so is every
+ * list row.  Results do not depend on the padding beyond the summation order
+ * of the rows; the time per step does.  (The reference fixes the padding when
+ * the box is built, SpherocylBox(..., dPadding), src/cpp/spherocyl_box.cpp:152-173.) */
+int sc_set_pad(sc_system* s, int k, double pad);
+/* Cell-size autotuner: for each of ncand candidate paddings, `steps` Heun steps
+ * of the system's own dynamics from the current state (restored after every
+ * trial), timed with their row rebuilds; keeps the fastest padding for every
+ * packing.  ms_out[ncand] (may be NULL) gets each trial's milliseconds and
+ * *chosen (may be NULL) the padding kept. */
+int sc_autotune_pad(sc_system* s, int ncand, const double* pads, int steps, double* ms_out,
+                    double* chosen);
 /* Autotuner: times reps force passes of this system for each candidate path and
  * launch shape and keeps the fastest.  *shape gets the tile width (tile path)
  * or the lanes per particle (list path).  Outputs may be NULL. */

@@ -1720,6 +1720,74 @@ int sc_synchronize(sc_system* s) {
 // Project autotuner: times the force pass of the system as it is (no state
 // change beyond the forces) for each candidate path and launch shape, keeps the
 // fastest.  reps passes per candidate after one warm-up pass.
+int sc_set_pad(sc_system* s, int k, double pad) {
+  if (!s || k < -1 || k >= s->npack || !(pad >= 0) || !std::isfinite(pad)) return SC_ERR_ARG;
+  API_BEGIN
+  download_scal(s);
+  for (int j = 0; j < s->npack; ++j) {
+    if (k >= 0 && j != k) continue;
+    PackInfo& P = s->packs[j];
+    double p = pad;
+    cell_grid(s->scal[j].Lx, s->scal[j].Ly, s->Rmax[j], s->Amax[j], p, P.rows, P.cols);
+    P.pad = p;
+  }
+  build_tiles(s);
+  alloc_grid(s);
+  s->cur_buf = 0;
+  bin_all_sized(s, 0, 0);
+  resort(s);  // also rebuilds every row with the new skin
+  return check_errors(s);
+  API_END
+}
+
+// Cell-size autotuner: each candidate padding (all packings alike) runs
+// `steps` Heun steps of the system's own dynamics from the same saved state,
+// timed on the host around a synchronised stream (row rebuilds included, which
+// is where the padding pays or costs); the fastest is kept and the state is
+// put back.  Positions, angles and strain are restored exactly; forces are
+// those of the restored state's next pass.
+int sc_autotune_pad(sc_system* s, int ncand, const double* pads, int steps, double* ms_out,
+                    double* chosen) {
+  if (!s || ncand < 1 || !pads || steps < 1) return SC_ERR_ARG;
+  for (int c = 0; c < ncand; ++c)
+    if (!(pads[c] >= 0) || !std::isfinite(pads[c])) return SC_ERR_ARG;
+  API_BEGIN
+  std::vector<double> x(s->ntot), y(s->ntot), phi(s->ntot);
+  int e = sc_get_positions(s, x.data(), y.data(), phi.data());
+  if (e != SC_OK) return e;
+  download_scal(s);
+  std::vector<double> g0(s->npack), gt0(s->npack);
+  for (int k = 0; k < s->npack; ++k) {
+    g0[k] = s->scal[k].gamma;
+    gt0[k] = s->scal[k].gamma_total;
+  }
+  auto restore = [&]() -> int {
+    int st = sc_set_positions(s, x.data(), y.data(), phi.data());
+    for (int k = 0; k < s->npack && st == SC_OK; ++k) st = sc_set_gamma(s, k, g0[k], gt0[k]);
+    return st;
+  };
+  int best = 0;
+  double best_ms = 1e300;
+  for (int c = 0; c < ncand; ++c) {
+    if ((e = sc_set_pad(s, -1, pads[c])) != SC_OK || (e = restore()) != SC_OK) return e;
+    HIPCK(hipStreamSynchronize(s->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((e = sc_strain_steps(s, steps, 0, nullptr)) != SC_OK) return e;
+    HIPCK(hipStreamSynchronize(s->stream));
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms_out) ms_out[c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  if ((e = sc_set_pad(s, -1, pads[best])) != SC_OK || (e = restore()) != SC_OK) return e;
+  if (chosen) *chosen = pads[best];
+  return SC_OK;
+  API_END
+}
+
 int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* block_out,
                 double* us_out) {
   if (!s || reps < 1) return SC_ERR_ARG;

@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
         // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
         v0x = Fx - S.gamma * Fy;
         v0y = Fy;
-        v0t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], p0);
+        v0t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], L.nx[o], L.ny[o], L.A[o]);
         qx = x0 + dt * v0x;
         qy = y0 + dt * v0y;
         qp = p0 + dt * v0t;
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(256) k_tile(TileArgs a) {
         // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
         double v1x = Fx - gam * Fy;
         double v1y = Fy;
-        double v1t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], a.pphi[g]);
+        double v1t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], L.nx[o], L.ny[o], L.A[o]);
         double ddx = 0.5 * dt * (a.v0x[g] + v1x);
         double ddy = 0.5 * dt * (a.v0y[g] + v1y);
         double ddp = 0.5 * dt * (a.v0t[g] + v1t);

@@ -426,7 +426,7 @@ __global__ void __launch_bounds__(256)
         // strain_step predictor, src/cpp/strain_dynamics.cpp:189-198
         v0x = Fx - S.gamma * Fy;
         v0y = Fy;
-        v0t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], p0);
+        v0t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], nxi, nyi, Ai);
         qx = xi + dt * v0x;
         qy = yi + dt * v0y;
         qp = p0 + dt * v0t;
@@ -484,7 +484,9 @@ __global__ void __launch_bounds__(256)
         // strain_step corrector, src/cpp/strain_dynamics.cpp:205-215
         double v1x = Fx - gam * Fy;
         double v1y = Fy;
-        double v1t = Tq / a.b.I[g] - gd * iso_rot(a.b.RC[g], a.pphi[g]);
+        // the isolated rotation at the predicted angle: (nxi, nyi) is the
+        // predicted segment vector, the state this pass evaluates
+        double v1t = Tq / a.b.I[g] - gd * iso_rot_seg(a.b.RC[g], nxi, nyi, Ai);
         x1 += 0.5 * dt * (a.v0x[g] + v1x);
         y1 += 0.5 * dt * (a.v0y[g] + v1y);
         p1 += 0.5 * dt * (a.v0t[g] + v1t);

@@ -454,10 +454,14 @@ __device__ __forceinline__ void bin_wave(int c, int g, int cap, int* cnt, int* s
   }
 }
 
-// isolated_rotation, src/cpp/strain_dynamics.cpp:174-177
-__device__ __forceinline__ double iso_rot(double RC, double phi) {
+// isolated_rotation, src/cpp/strain_dynamics.cpp:174-177: 0.5 (1 - RC cos 2 phi),
+// with cos 2 phi taken from the segment vector (nx, ny) = A (cos phi, sin phi)
+// the state carries, (nx^2 - ny^2) / A^2, instead of an FP64 cosine (a disc,
+// A = 0, has RC = 0).  It differs from cos(2 phi) by rounding only.
+__device__ __forceinline__ double iso_rot_seg(double RC, double nx, double ny, double A) {
 #pragma clang fp contract(off)
-  return 0.5 * (1 - RC * cos(2 * phi));
+  const double c2 = A > 0.0 ? (nx * nx - ny * ny) / (A * A) : 1.0;
+  return 0.5 * (1 - RC * c2);
 }
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,8 @@ from . import configs, native
 class Workload:
     packings: list
     gammas: list
+    pad: float = 0.1  # cell padding (chosen by the cell-size autotuner in prepare)
+    pad_trials: dict = dataclasses.field(default_factory=dict)  # padding -> ms
 
     @property
     def sizes(self):
@@ -31,13 +33,20 @@ class Workload:
 
 
 def prepare(packings, heun_steps=2000, strain_rate=1e-4, dt=0.01, contact_model="two_point",
-            pad=0.1) -> Workload:
+            pad=0.1, tune_pad=None, tune_steps=200) -> Workload:
     """Advance every packing by heun_steps sheared Heun steps (one batched
-    system) and return the resulting states."""
+    system) and return the resulting states.  tune_pad: candidate cell
+    paddings for the cell-size autotuner (native.System.autotune_pad), run
+    with the launch-shape autotuner before the steps; the chosen padding is
+    the workload's, for its minimisation too."""
     s = native.System(packings, pads=pad)
+    trials = {}
     try:
         s.set_contact_model(contact_model)
         s.set_dynamics(strain_rate=strain_rate, step=dt)
+        if tune_pad:
+            s.autotune(reps=5)
+            pad, trials = s.autotune_pad(tune_pad, steps=tune_steps)
         if heun_steps:
             s.strain_steps(heun_steps)
         x, y, phi = s.positions()
@@ -50,13 +59,13 @@ def prepare(packings, heun_steps=2000, strain_rate=1e-4, dt=0.01, contact_model=
         out.append(configs.Packing(p.L, x[sl].copy(), y[sl].copy(), phi[sl].copy(), p.A, p.R,
                                    p.packing_fraction, p.name))
         o += p.n
-    return Workload(out, gammas)
+    return Workload(out, gammas, pad, trials)
 
 
 def system_for(work: Workload, indices=None, precision="fp64", contact_model="two_point",
-               pad=0.1, autotune=True):
+               pad=None, autotune=True):
     idx = list(range(len(work.packings))) if indices is None else list(indices)
-    s = native.System([work.packings[i] for i in idx], pads=pad)
+    s = native.System([work.packings[i] for i in idx], pads=work.pad if pad is None else pad)
     s.set_contact_model(contact_model)
     for k, i in enumerate(idx):
         s.set_gamma(work.gammas[i], 0.0, k=k)

@@ -71,6 +71,8 @@ def load():
         "sc_get_path": ([sysp, _ip], C.c_int),
         "sc_set_list_block": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_autotune": ([sysp, C.c_int, _ip, _ip, _ip, _dp], C.c_int),
+        "sc_set_pad": ([sysp, C.c_int, C.c_double], C.c_int),
+        "sc_autotune_pad": ([sysp, C.c_int, _dp, C.c_int, _dp, _dp], C.c_int),
         "sc_get_tiling": ([sysp, _ip, _ip], C.c_int),
         "sc_set_positions": ([sysp, _dp, _dp, _dp], C.c_int),
         "sc_get_positions": ([sysp, _dp, _dp, _dp], C.c_int),
@@ -363,6 +365,20 @@ class System:
                                       C.byref(us)))
         path = ("tile", "list")[p.value]
         return dict(path=path, shape=t.value, block=b.value, us_per_pass=us.value)
+
+    def set_pad(self, pad, k=-1):
+        self._ck(self.lib.sc_set_pad(self.h, k, float(pad)))
+
+    def autotune_pad(self, pads=(0.05, 0.1, 0.2, 0.3), steps=200):
+        """Cell-size autotuner: times `steps` Heun steps (the system's own
+        dynamics, state restored after each trial) per candidate padding and
+        keeps the fastest.  Returns (chosen padding, {padding: ms})."""
+        c = _f64(pads)
+        ms = np.zeros(len(c))
+        chosen = C.c_double()
+        self._ck(self.lib.sc_autotune_pad(self.h, len(c), _p(c), int(steps), _p(ms),
+                                          C.byref(chosen)))
+        return chosen.value, dict(zip(c.tolist(), ms.tolist()))
 
     def tiling(self):
         a, b = C.c_int(), C.c_int()

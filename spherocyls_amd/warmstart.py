@@ -296,7 +296,8 @@ def minimize_with_warm_start(work, params, mpnn, precision="mixed", switch_frms=
             fr = np.full(len(work.packings), np.inf)
         bad = [k for k in range(len(work.packings)) if not fr[k] < params[8]]
         if bad:
-            sub = ensemble.Workload([work.packings[k] for k in bad], [work.gammas[k] for k in bad])
+            sub = ensemble.Workload([work.packings[k] for k in bad], [work.gammas[k] for k in bad],
+                                    work.pad)
             s2 = ensemble.system_for(sub, precision=precision, autotune=False)
             try:
                 it2, fr2, _ = s2.fire(params, switch_frms=switch_frms if precision == "mixed" else 0.0,
@@ -325,14 +326,16 @@ def main():
         t.add_argument("--batch", type=int, default=8)
         t.add_argument("--lr", type=float, default=1e-3)
         t.add_argument("--seed", type=int, default=1)
+        t.add_argument("--every", type=int, default=10,
+                       help="train-qs: record every k-th compression step")
     args = ap.parse_args()
     t0 = time.perf_counter()
     if args.cmd == "train-qs":
         # (pre-step, minimised) pairs of quasi-static compression, phi 0.80 -> 0.86
-        # in steps of 1e-4, every 10th step (the committed weights)
+        # in steps of 1e-4, every k-th step
         params = native.fire_params(max_iter=200000)
         pairs, _ = qs_pairs(qs_packings(args.packings, args.n, seed=10 + args.seed), params,
-                            log=lambda m: print(m, flush=True))
+                            every=args.every, log=lambda m: print(m, flush=True))
     else:
         # (sheared, minimised) pairs of the ragged-ensemble physics
         params = native.fire_params(max_iter=20000)
