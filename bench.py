@@ -44,7 +44,7 @@ BYTES_CORR = 144
 # event-timed launches after 1000 steps), scaled by a plain copy's readings
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; profiles/k_list_heun_shear_N262144.md):
 # (workload, path, precision) -> bytes
-MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (23777.2 * 2.0 + 17984.2) * 1024}
+MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (22705.47 * 2.0 + 18303.91) * 1024}
 
 WORKLOADS = {
     "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
