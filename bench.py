@@ -44,7 +44,7 @@ BYTES_CORR = 144
 # event-timed launches after 1000 steps), scaled by a plain copy's readings
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; profiles/k_list_heun_shear_N262144.md):
 # (workload, path, precision) -> bytes
-MEASURED_TRAFFIC = {("shear_large", "list", "fp64"): (22705.47 * 2.0 + 18303.91) * 1024}
+MEASURED_TRAFFIC = {("shear_large", "list", "fp64", 0.1): (22705.47 * 2.0 + 18303.91) * 1024}
 
 WORKLOADS = {
     "shear_large": (lambda: configs.shear_large(1), 1e-6, 0.01),
@@ -346,6 +346,9 @@ def main():
     ap.add_argument("--tile", type=int, default=12, help="tile path: columns per tile")
     ap.add_argument("--lanes", type=int, default=1, help="list path: lanes per particle")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--pad", type=float, default=0.1,
+                    help="cell padding (list skin) of the Heun leg; 0: the cell-size autotuner "
+                         "picks among 0.05, 0.1 and 0.2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200,
                     help="CPU baseline: Heun steps per run (from the relaxed state)")
@@ -376,7 +379,7 @@ def main():
 
     make, strain_rate, dt = WORKLOADS[args.workload]
     pk0 = configs.shear_large(1, n=args.n) if args.n and args.workload == "shear_large" else make()
-    s = native.System(pk0, pads=0.1)
+    s = native.System(pk0, pads=args.pad or 0.1)
     s.set_dynamics(strain_rate=strain_rate, step=dt)
     if args.precision == "mixed":
         s.set_precision("mixed")
@@ -401,6 +404,9 @@ def main():
         else:
             s.set_list_block(args.block or 256, args.lanes)
         tuned = None
+    pad, pad_trials = args.pad, None
+    if not pad:  # the cell-size autotuner, from the relaxed state
+        pad, pad_trials = s.autotune_pad((0.05, 0.1, 0.2), steps=200)
     path = s.path()
     shape, block = s.tiling()
 
@@ -424,7 +430,7 @@ def main():
     # the same steps from the caller's host arrays to host results: system
     # creation (upload, cell sort, lists) + steps + download of the positions
     t0 = time.perf_counter()
-    s2 = native.System(pk, pads=0.1)
+    s2 = native.System(pk, pads=pad)
     try:
         s2.set_gamma(gamma_r, 0.0)
         s2.set_dynamics(strain_rate=strain_rate, step=dt)
@@ -459,14 +465,15 @@ def main():
         "data": f"synthetic (seeded non-crossing random packing, relaxed by {args.relax_steps} "
                 "untimed Heun steps in setup)",
         "config": {"workload": pk.name, "n_particles": pk.n, "strain_rate": strain_rate, "dt": dt,
-                   "cell_pad": 0.1, "force_path": path, "relax_steps": args.relax_steps,
+                   "cell_pad": pad, "cell_pad_trials_ms_200_steps": pad_trials,
+                   "force_path": path, "relax_steps": args.relax_steps,
                    "relax_s": relax_s,
                    ("tile_cols" if path == "tile" else "lanes_per_particle"): shape,
                    "block_threads": block, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if args.n else MEASURED_TRAFFIC.get((args.workload, path,
-                                                                             args.precision)),
+                                                                             args.precision, pad)),
                      "traffic_source": "steady-state constant: rocprofv3 FETCH_SIZE / WRITE_SIZE "
                                        "per corrector launch after the relaxation, scaled by a "
                                        "plain copy's readings (profiles/), not this run",

@@ -97,3 +97,21 @@ def test_quasistatic_warm_start_gain(net):
     assert res["plain"].iterations.shape == res["warm start"].iterations.shape
     # the damped warm start never makes FIRE do materially more work
     assert gain > 0.9
+
+
+def test_exact_proposal_ceiling(held_out):
+    """What a warm start could buy at best on the held-out ensemble: each
+    packing's exact displacement to the minimum plain FIRE reaches, applied
+    at 90%, must cut the mean FIRE iterations by at least 20% (so the
+    measurement above would show a gain where one exists); the printed table
+    is the ceiling the learned proposal is read against."""
+    params = native.fire_params(max_iter=20000)
+    rows = warmstart.ceiling(held_out, params, qualities=(0.0, 0.5, 0.9))
+    for r in rows:
+        print(f"q = {r['quality']}: {r['mean_iterations']:.0f} mean / {r['max_iterations']} max "
+              f"iterations, {r['seconds'] * 1e3:.1f} ms, {r['converged']} converged", flush=True)
+    plain, q9 = rows[0], rows[2]
+    n = len(held_out.packings)
+    assert plain["converged"] == n and q9["converged"] == n
+    assert q9["mean_iterations"] < 0.8 * plain["mean_iterations"]
+
