@@ -19,7 +19,8 @@
 //   weight grad dW      = A^T . dC                   (k_gemm_tn, per-chunk
 //                                                     partials summed in order)
 // GEMM operands are BF16 and accumulate in FP32 on the MFMA units
-// (v_mfma_f32_16x16x16bf16_1k: a wavefront computes 16-row strips, one
+// (v_mfma_f32_16x16x32_bf16 of gfx950 over 32-wide K steps, the
+// 16x16x16 form for a remaining 16: a wavefront computes 16-row strips, one
 // 16 x 16 tile per MFMA); the residual stream h, pre-activations, gradients,
 // FP32 master weights and Adam moments stay FP32.  Edges are stored grouped by
 // receiver (CSR), with the index of each edge's reverse, so every sum over
@@ -43,6 +44,8 @@ namespace sc {
 namespace mp {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;
 
@@ -95,9 +98,18 @@ enum Epi : int {
   kEpiResid = 2,     // C32 = acc; H32 = Hin32 + silu(acc)
 };
 
+// One 16 x 16 x 32 MFMA step: lane group g = lane / 16 holds the eight K
+// indices 8 g .. 8 g + 7 of its row of A and of its column of B.  The unit
+// pairs A's and B's slots by lane group and position, so any assignment of K
+// to slots that A and B share gives the same sum.
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
 // C[M x NT*16] = A[M x K] . Bt^T, A row-major (lda), Bt row-major [N][ldb].
 // One wavefront per 16-row strip, four per workgroup.  K % 16 == 0,
-// lda % 4 == 0, ldb % 4 == 0 (8-byte operand loads).
+// lda % 8 == 0, ldb % 8 == 0 (16-byte operand loads in the 32-wide steps).
 template <int NT, int EPI>
 __global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int lda,
                                                   const bf16* __restrict__ Bt, int ldb, int M, int K,
@@ -113,7 +125,19 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int
   f32x4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  int k0 = 0;
+  const int k8 = 8 * (lane >> 4);
+  for (; k0 + 32 <= K; k0 += 32) {
+    s16x8 a = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (ar < M) a = *reinterpret_cast<const s16x8*>(A + (size_t)ar * lda + k0 + k8);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const s16x8 b =
+          *reinterpret_cast<const s16x8*>(Bt + (size_t)(t * 16 + (lane & 15)) * ldb + k0 + k8);
+      acc[t] = mfma32(a, b, acc[t]);
+    }
+  }
+  for (; k0 < K; k0 += 16) {
     bf16x4 a = bf16x4{0, 0, 0, 0};
     if (ar < M) a = *reinterpret_cast<const bf16x4*>(A + (size_t)ar * lda + k0 + kq);
 #pragma unroll
@@ -159,18 +183,20 @@ __global__ void __launch_bounds__(256) k_gemm_tn(const bf16* __restrict__ A, int
   for (int tile = wave; tile < ntiles; tile += 4) {
     const int kt = tile / NTN, nt = tile % NTN;
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int m0 = m_begin; m0 < m_end; m0 += 16) {
-      bf16x4 a, b;
+    // the reduction runs over rows m, 32 per MFMA: lane group g takes rows
+    // m0 + 8 g .. m0 + 8 g + 7 (rows past the chunk contribute zeros)
+    for (int m0 = m_begin; m0 < m_end; m0 += 32) {
+      s16x8 a, b;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = m0 + kq + i;
+      for (int i = 0; i < 8; ++i) {
+        const int m = m0 + 2 * kq + i;
         const bool ok = m < m_end;
         // A operand = A^T tile: row (lane % 16) of the K block, column m
         a[i] = ok ? (short)A[(size_t)m * lda + kt * 16 + (lane & 15)] : (short)0;
         // B operand = D tile: row m, column (lane % 16) of the N block
         b[i] = ok ? (short)D[(size_t)m * ldd + nt * 16 + (lane & 15)] : (short)0;
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      acc = mfma32(a, b, acc);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
