@@ -17,7 +17,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# ORACLE_SO may name another build of the restatement (the planted-mistake
+# check in oracle/mutants.py builds its mutants elsewhere and points here)
+ORACLE_SO = os.environ.get("SPHEROCYLS_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libspherocyl_ref.so")
 REF_SRC = "/root/reference/src/cpp"
 
