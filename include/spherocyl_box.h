@@ -147,6 +147,12 @@ class SpherocylBox {
   // state is saved as it is.  Returns the last time stamp.
   long unsigned int run_pure_strain(long unsigned int nStart, double dStopAspect,
                                     double dSvStressGamma, double dSvPosGamma);
+  // run_strain of the reference's GPU class (src/cuda/strain_dynamics.cu:
+  // 984-1083): sheared Heun steps from time nStart for dRunGamma of strain,
+  // with the GPU class's files: SE lines and sd%010lu.dat positions as for
+  // run_pure_strain above.  Returns the last time stamp.
+  long unsigned int run_strain_sd(long unsigned int nStart, double dRunGamma,
+                                  double dSvStressGamma, double dSvPosGamma);
   // flip_shear_direction (src/cuda/find_neighbors.cu:565-579): x -> L - x,
   // phi -> -phi, gamma -> -gamma
   void flip_shear_direction();
@@ -230,6 +236,9 @@ class SpherocylBox {
                                         double dStep);
   long unsigned int resize_to_energy_2p(FILE* f, long unsigned int nTime, double dEnergy,
                                         double dStep, bool bYOnly);
+  long unsigned int run_sd(long unsigned int nStart, bool bPure, unsigned long nStop,
+                           double dStopAspect, double dSvStressGamma, double dSvPosGamma,
+                           const char* name);
   long unsigned int jam_search_2p(double dJamE, double dResizeStep, double dMinResizeStep,
                                   bool bYOnly, const char* name);
 

@@ -38,6 +38,7 @@ CXX_SOURCES = [os.path.join(CSRC, "sc_configgen.cpp")]
 HOST_LIB_SOURCES = ["spherocyl_box.cpp", "dat_file.cpp"]
 DRIVERS = {
     "spherocyl_strain": ("main_strain.cpp", []),
+    "spherocyl_box": ("main_box.cpp", []),
     "spherocyl_resize": ("main_resize.cpp", []),
     "spherocyl_minimize": ("main_relax.cpp", []),
     "spherocyl_qscompress": ("main_qs2p.cpp", []),

@@ -81,6 +81,23 @@ inline double box_side(int n, double A, double R, double phi) {
   return std::sqrt(area / phi);
 }
 
+// The GPU drivers' initial-configuration codes (src/cuda/main.cpp:120) mapped
+// to sc_random_configuration's; the "uniform" angle variants (1, 4, 7) and the
+// custom configuration (9) are not built.
+inline int config_from_gpu_code(int c) {
+  switch (c) {
+    case 0: return SC_CONFIG_RANDOM;
+    case 2: return SC_CONFIG_RANDOM_ALIGNED;
+    case 3: return SC_CONFIG_ZERO_E;
+    case 5: return SC_CONFIG_ZERO_E_ALIGNED;
+    case 6: return SC_CONFIG_GRID;
+    case 8: return SC_CONFIG_GRID_ALIGNED;
+    default:
+      throw SpherocylError(SC_ERR_ARG, "initial configuration " + std::to_string(c) +
+                                           " is not supported (0, 2, 3, 5, 6, 8 are)");
+  }
+}
+
 }  // namespace driver
 
 #endif

@@ -38,23 +38,6 @@
 #include "driver_common.h"
 #include "spherocyl_files.h"
 
-namespace {
-
-int config_from_gpu_code(int c) {
-  switch (c) {
-    case 0: return SC_CONFIG_RANDOM;
-    case 2: return SC_CONFIG_RANDOM_ALIGNED;
-    case 3: return SC_CONFIG_ZERO_E;
-    case 5: return SC_CONFIG_ZERO_E_ALIGNED;
-    case 6: return SC_CONFIG_GRID;
-    case 8: return SC_CONFIG_GRID_ALIGNED;
-    default:
-      throw SpherocylError(SC_ERR_ARG, "initial configuration " + std::to_string(c) +
-                                           " is not supported (0, 2, 3, 5, 6, 8 are)");
-  }
-}
-
-}  // namespace
 
 int main(int argc, char* argv[]) {
   using namespace driver;

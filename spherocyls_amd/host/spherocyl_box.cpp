@@ -865,18 +865,42 @@ long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double
                                                 double dSvStressGamma, double dSvPosGamma) {
   if (m_dStrainRate == 0.0) throw SpherocylError(SC_ERR_ARG, "run_pure_strain: zero strain rate");
   m_dStrainRate = std::fabs(m_dStrainRate);  // as the reference does (2point_dynamics.cu:2292-2295)
-  const double rate = m_dStrainRate;
   ensure_device();
   read_scalars();
   if (m_dLx / m_dLy > dStopAspect)
     throw SpherocylError(SC_ERR_ARG, "run_pure_strain: box aspect already beyond the stop");
+  return run_sd(nStart, true, 0, dStopAspect, dSvStressGamma, dSvPosGamma, "run_pure_strain");
+}
+
+// run_strain of the GPU class (src/cuda/strain_dynamics.cu:984-1083)
+long unsigned int SpherocylBox::run_strain_sd(long unsigned int nStart, double dRunGamma,
+                                              double dSvStressGamma, double dSvPosGamma) {
+  if (!(m_dStrainRate > 0)) throw SpherocylError(SC_ERR_ARG, "run_strain_sd: strain rate must be positive");
+  if (!(dRunGamma >= 0) || !std::isfinite(dRunGamma))
+    throw SpherocylError(SC_ERR_ARG, "run_strain_sd: run length must be >= 0");
+  ensure_device();
+  read_scalars();
+  const unsigned long nStop = nStart + (unsigned long)(dRunGamma / m_dStrainRate + 0.5);
+  return run_sd(nStart, false, nStop, 0.0, dSvStressGamma, dSvPosGamma, "run_strain_sd");
+}
+
+// The time loop the GPU class's run_strain and run_pure_strain_2p share: per
+// time unit 1/step sub-steps; every nSvStress sub-steps a line "t E Pxx Pyy P
+// Pxy Pyx" (printf %.7g; the corrected stress with the CPU code's sign) of the
+// state the sub-step starts from, every nSvPos time units also a position file
+// sd%010lu.dat; a restart seeks the SE file to nStart.  Runs until nStop
+// (sheared) or until Lx / Ly reaches dStopAspect (pure shear).
+long unsigned int SpherocylBox::run_sd(long unsigned int nStart, bool bPure, unsigned long nStop,
+                                       double dStopAspect, double dSvStressGamma,
+                                       double dSvPosGamma, const char* name) {
+  const double rate = m_dStrainRate;
   if (dSvStressGamma < rate * m_dStep) dSvStressGamma = rate * m_dStep;
   if (dSvPosGamma < rate) dSvPosGamma = rate;
   const unsigned long nIntStep = (unsigned long)(1.0 / m_dStep + 0.5);
   const unsigned long nSvStress = (unsigned long)(dSvStressGamma / (rate * m_dStep) + 0.5);
   const unsigned long nSvPos = (unsigned long)(dSvPosGamma / rate + 0.5);
   if (nIntStep < 1 || nSvStress < 1 || nSvPos < 1)
-    throw SpherocylError(SC_ERR_ARG, "run_pure_strain: step or save cadence rounds to zero");
+    throw SpherocylError(SC_ERR_ARG, std::string(name) + ": step or save cadence rounds to zero");
   std::string sePath = m_strOutputDir + "/" + m_strSEOutput;
   std::fstream outfSE;
   if (nStart == 0) {
@@ -884,10 +908,10 @@ long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double
   } else {
     outfSE.open(sePath.c_str(), std::ios::in | std::ios::out);
     if (outfSE && !seek_after_time(outfSE, nStart))
-      throw SpherocylError(SC_ERR_ARG, "run_pure_strain: start time not found in " + sePath);
+      throw SpherocylError(SC_ERR_ARG, std::string(name) + ": start time not found in " + sePath);
   }
-  if (!outfSE) throw SpherocylError(SC_ERR_ARG, "run_pure_strain: cannot open " + sePath);
-  check(sc_set_path(m_sys, SC_PATH_LIST), "set_path");
+  if (!outfSE) throw SpherocylError(SC_ERR_ARG, std::string(name) + ": cannot open " + sePath);
+  if (bPure) check(sc_set_path(m_sys, SC_PATH_LIST), "set_path");
   check(sc_set_dynamics(m_sys, 0, rate, 0.0, m_dStep), "set_dynamics");
   auto line = [&](unsigned long t) {
     calc_se();
@@ -896,8 +920,9 @@ long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double
                   m_dCorr[0], m_dCorr[1], 0.5 * (m_dCorr[0] + m_dCorr[1]), m_dCorr[2], m_dCorr[3]);
     outfSE << buf;
   };
+  auto more = [&](unsigned long t) { return bPure ? m_dLx / m_dLy < dStopAspect : t < nStop; };
   unsigned long nTime = nStart, nTotal = nTime * nIntStep;
-  while (m_dLx / m_dLy < dStopAspect) {
+  while (more(nTime)) {
     unsigned long i = 0;
     while (i < nIntStep) {
       const bool svPos = i == 0 && nTime % nSvPos == 0;
@@ -910,7 +935,10 @@ long unsigned int SpherocylBox::run_pure_strain(long unsigned int nStart, double
       }
       // steps up to the next stress record or the end of this time unit
       unsigned long k = std::min(nIntStep - i, nSvStress - nTotal % nSvStress);
-      check(sc_pure_strain_steps(m_sys, (int)k, 0, nullptr), "pure_strain_steps");
+      if (bPure)
+        check(sc_pure_strain_steps(m_sys, (int)k, 0, nullptr), "pure_strain_steps");
+      else
+        check(sc_strain_steps(m_sys, (int)k, 0, nullptr), "strain_steps");
       m_bHostStale = true;
       i += k;
       nTotal += k;

@@ -227,3 +227,59 @@ def test_minimize_driver_fire(tmp_path):
     b.calc_forces()
     # 14 printed digits perturb a relaxed state by ~1e-13 relative
     assert b.frms() < 1e-10 * 10
+
+
+def _sd_file(path):
+    with open(path) as f:
+        head = f.readline().split()
+    rows = np.loadtxt(path, skiprows=1, ndmin=2)
+    return [float(v) for v in head], rows
+
+
+@pytest.mark.parametrize("change", [1, 2])
+def test_gpu_order_strain_driver(tmp_path, change):
+    """spherocyl_box (argument order of the GPU code's src/cuda/main.cpp): a
+    random start with gamma-distributed aspect ratios, the shear direction
+    reversed (1) or rotated by gamma (2), then the GPU class's run_strain:
+    SE lines every 10 sub-steps, sd position files every 10 time units, the
+    recorded energy the oracle's on each saved state.  A restart from the
+    file at t = 30 keeps the SE file up to its first line of time 30 (the
+    GPU class's seek, src/cuda/strain_dynamics.cu:1018-1047) and continues
+    from time 30 again with the values of the first run: to the printed
+    digits at first (the restarted lists and storage order sum in another
+    order), the same time stamps throughout."""
+    d1, d2 = tmp_path / "a", tmp_path / "b"
+    d1.mkdir()
+    d2.mkdir()
+    common = ["1e-3", "0.01"]
+    _run("spherocyl_box", ["r", str(d1), "256", *common, "0.05", "0.01", "1e-4", "0.1",
+                           str(change), "0.80", "1.0", "1.4", "0", "0.3"])
+    lines = open(d1 / "sd_stress_energy.dat").read().splitlines()
+    t = [int(l.split()[0]) for l in lines]
+    assert t[0] == 0 and t[-1] == 50 and len(lines) == 50 * 10 + 1
+    files = sorted(p.name for p in d1.glob("sd*.dat") if p.name != "sd_stress_energy.dat")
+    assert files == ["sd%010d.dat" % k for k in range(0, 51, 10)]
+    head, rows = _sd_file(d1 / "sd0000000030.dat")
+    assert len(set(np.round(rows[:, 4] / rows[:, 3], 9))) > 100  # gamma-distributed aspects
+    for k in (0, 30, 50):
+        head, rows = _sd_file(d1 / ("sd%010d.dat" % k))
+        b = oracle.OracleBox(head[1], rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], rows[:, 3], 0.1)
+        b.set_params(head[4], 0.0, 0.0, 0.0, 0.01)
+        b.find_neighbors()
+        b.calc_se()
+        line = next(l for l in lines if int(l.split()[0]) == k)
+        assert float(line.split()[1]) == pytest.approx(b.scalars()["E"], rel=2e-6), k
+    # restart at t = 30 from the saved file into a copy of the SE file
+    (d2 / "sd_stress_energy.dat").write_text("\n".join(lines) + "\n")
+    _run("spherocyl_box", [str(d1 / "sd0000000030.dat"), str(d2), "256", *common, "0.02", "0.01",
+                           "1e-4", "0.1", "0"])
+    lines2 = open(d2 / "sd_stress_energy.dat").read().splitlines()
+    i30 = next(i for i, l in enumerate(lines) if l.startswith("30 "))
+    assert lines2[:i30 + 1] == lines[:i30 + 1]
+    rest, rest2 = _num("\n".join(lines[i30:])), _num("\n".join(lines2[i30 + 1:]))
+    assert len(rest2) == len(rest)
+    assert [a[0] for a in rest] == [b[0] for b in rest2]
+    # the first two time units agree to the printed digits; later the sheared
+    # dynamics (strain rate 1e-3) amplifies the rounding of the other order
+    for a, b in zip(rest[:20], rest2[:20]):
+        np.testing.assert_allclose(b[1:], a[1:], rtol=0, atol=1e-6 * max(abs(v) for v in a[1:]))
