@@ -39,6 +39,7 @@ HOST_LIB_SOURCES = ["spherocyl_box.cpp", "dat_file.cpp"]
 DRIVERS = {
     "spherocyl_strain": ("main_strain.cpp", []),
     "spherocyl_box": ("main_box.cpp", []),
+    "spherocyl_pureshear": ("main_box.cpp", ["-DSPHEROCYL_PURE_SHEAR"]),
     "spherocyl_resize": ("main_resize.cpp", []),
     "spherocyl_minimize": ("main_relax.cpp", []),
     "spherocyl_qscompress": ("main_qs2p.cpp", []),

@@ -21,6 +21,12 @@
 // "t E Pxx Pyy P Pxy Pyx" and sd%010lu.dat positions (run_strain_sd).
 // SPHEROCYL_SEED seeds random starts; SPHEROCYL_PRECISION=mixed selects FP32
 // pair forces; SPHEROCYL_CONTACT=two_point the two-point contact model.
+//
+// Built with SPHEROCYL_PURE_SHEAR it is spherocyl_pureshear, the pure-shear
+// driver (src/cuda/main_pure_shear.cpp:60-296): argument 6 is the box aspect
+// Lx / Ly at which pure shear stops, argument 10 a rotation by gamma (1)
+// before the run, and the run is run_pure_strain with the two-point contact
+// model, as run_pure_strain_2p there.
 #include <cmath>
 #include <ctime>
 #include <iostream>
@@ -28,6 +34,14 @@
 
 #include "driver_common.h"
 #include "spherocyl_files.h"
+
+static const char* self_name() {
+#ifdef SPHEROCYL_PURE_SHEAR
+  return "spherocyl_pureshear";
+#else
+  return "spherocyl_box";
+#endif
+}
 
 int main(int argc, char* argv[]) {
   using namespace driver;
@@ -38,10 +52,19 @@ int main(int argc, char* argv[]) {
     int n = int_input(argc, argv, ++argn, "Number of particles");
     double rate = float_input(argc, argv, ++argn, "Strain rate");
     double step = float_input(argc, argv, ++argn, "Integration step size");
+#ifdef SPHEROCYL_PURE_SHEAR
+    double run = float_input(argc, argv, ++argn, "Strain stop (Box Aspect Lx/Ly)");
+#else
     double run = float_input(argc, argv, ++argn, "Strain run length (gamma)");
+#endif
     double pos_rate = float_input(argc, argv, ++argn, "Position data save rate");
     double se_rate = float_input(argc, argv, ++argn, "Stress data save rate");
     double pad = float_input(argc, argv, ++argn, "Cell padding");
+#ifdef SPHEROCYL_PURE_SHEAR
+    int change = int_input(argc, argv, ++argn, "Change shear direction (1 to reverse");
+    if (change == 1) change = 2;  // main_pure_shear.cpp:261-265 rotates by gamma
+    else if (change != 0) throw SpherocylError(SC_ERR_ARG, "change of shear direction must be 0 or 1");
+#else
     int change = int_input(argc, argv, ++argn,
                            "Change shear direction (1 to reverse, 2 to rotate by gamma, 3 to "
                            "increase gama and rotate)");
@@ -51,6 +74,7 @@ int main(int argc, char* argv[]) {
                            "supported; 0, 1 and 2 are");
     if (change < 0 || change > 3)
       throw SpherocylError(SC_ERR_ARG, "change of shear direction must be 0, 1 or 2");
+#endif
     if (n <= 0) throw SpherocylError(SC_ERR_ARG, "number of particles must be positive");
     if (se_rate < rate * step) se_rate = rate * step;
     if (pos_rate < rate) pos_rate = rate;
@@ -109,7 +133,11 @@ int main(int argc, char* argv[]) {
       t0 = position_file_time(file);
     }
     SpherocylBox box(L, n, x.data(), y.data(), phi.data(), A.data(), R.data(), pad);
+#ifdef SPHEROCYL_PURE_SHEAR
+    box.set_contact_model(SC_CONTACT_TWO_POINT);
+#else
     if (env_flag("SPHEROCYL_CONTACT", "two_point")) box.set_contact_model(SC_CONTACT_TWO_POINT);
+#endif
     if (env_flag("SPHEROCYL_PRECISION", "mixed")) box.set_precision(true);
     box.set_gamma(gamma);
     box.set_total_gamma(gamma_total);
@@ -129,11 +157,15 @@ int main(int argc, char* argv[]) {
     std::time_t start = std::time(nullptr);
     box.calc_se();
     box.display(0, 0);
+#ifdef SPHEROCYL_PURE_SHEAR
+    unsigned long t = box.run_pure_strain(t0, run, se_rate, pos_rate);
+#else
     unsigned long t = box.run_strain_sd(t0, run, se_rate, pos_rate);
+#endif
     box.display(0, 0);
     std::cout << "Steps: " << t << "\nRun Time: " << std::time(nullptr) - start << std::endl;
   } catch (const SpherocylError& e) {
-    std::cerr << "spherocyl_box: " << e.what() << std::endl;
+    std::cerr << self_name() << ": " << e.what() << std::endl;
     return 1;
   }
   return 0;

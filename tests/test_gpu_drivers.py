@@ -283,3 +283,34 @@ def test_gpu_order_strain_driver(tmp_path, change):
     # dynamics (strain rate 1e-3) amplifies the rounding of the other order
     for a, b in zip(rest[:20], rest2[:20]):
         np.testing.assert_allclose(b[1:], a[1:], rtol=0, atol=1e-6 * max(abs(v) for v in a[1:]))
+
+
+def test_pure_shear_driver(tmp_path):
+    """spherocyl_pureshear (argument order of src/cuda/main_pure_shear.cpp):
+    pure shear with the two-point model from a random start until Lx / Ly
+    reaches 1.05; the box aspect of every saved state grows, the last one is
+    at the stop, and each recorded energy is the oracle's on the saved
+    (non-square) state."""
+    d = tmp_path / "p"
+    d.mkdir()
+    _run("spherocyl_pureshear", ["r", str(d), "128", "1e-3", "0.01", "1.05", "0.01", "1e-4", "0.1",
+                                 "0", "0.80", "1.0", "1.4"])
+    lines = open(d / "sd_stress_energy.dat").read().splitlines()
+    files = sorted(p for p in d.glob("sd0*.dat"))
+    assert len(files) >= 3
+    aspects = []
+    for path in files:
+        head, rows = _sd_file(path)
+        aspects.append(head[1] / head[2])
+        k = int(path.name[2:12])
+        b = oracle.OracleBox(head[1], rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], rows[:, 3], 0.1)
+        b.set_box(head[1], head[2])
+        b.set_contact_model(1)
+        b.set_params(head[4], 0.0, 0.0, 0.0, 0.01)
+        b.find_neighbors()
+        b.calc_se()
+        line = next(l for l in lines if int(l.split()[0]) == k)
+        e = b.scalars()["E"]
+        assert float(line.split()[1]) == pytest.approx(e, rel=2e-6, abs=1e-12), k
+    assert all(a2 > a1 for a1, a2 in zip(aspects, aspects[1:]))
+    assert aspects[-1] >= 1.05 and aspects[-2] < 1.05
