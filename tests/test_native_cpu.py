@@ -113,8 +113,35 @@ def test_workload_generators():
 
 
 def test_drivers_built():
-    for exe in ("spherocyl_strain", "spherocyl_resize", "spherocyl_minimize"):
-        assert os.access(os.path.join(ROOT, "spherocyls_amd", "bin", exe), os.X_OK)
+    from spherocyls_amd import build
+    for exe in build.DRIVERS:
+        assert os.access(os.path.join(ROOT, "spherocyls_amd", "bin", exe), os.X_OK), exe
+
+
+def test_gpu_order_drivers_reject_unbuilt_options(tmp_path):
+    """spherocyl_box stops with an error, before any GPU call, for the shear
+    change it does not build (3: shear past the remap, then rotate) and
+    spherocyl_pureshear for a change other than 0 or 1."""
+    import subprocess
+    common = ["r", str(tmp_path), "16", "1e-3", "0.01", "0.05", "0.01", "1e-4", "0.1"]
+    for exe, change in (("spherocyl_box", "3"), ("spherocyl_box", "4"),
+                        ("spherocyl_pureshear", "2")):
+        r = subprocess.run([os.path.join(ROOT, "spherocyls_amd", "bin", exe), *common, change],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, (exe, change, r.stderr)
+        assert "not supported" in r.stderr or "must be" in r.stderr, (exe, change, r.stderr)
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Argument checks of the padding setter, the cell-size autotuner and the
+    y-only affine step come before any device work."""
+    import ctypes as C
+    lib = native.load()
+    assert lib.sc_set_pad(None, -1, C.c_double(0.1)) == native.SC_ERR_ARG
+    pads = (C.c_double * 2)(0.1, 0.2)
+    assert lib.sc_autotune_pad(None, 2, pads, 10, None, None) == native.SC_ERR_ARG
+    scale = (C.c_double * 1)(0.99)
+    assert lib.sc_affine_resize_y(None, scale) == native.SC_ERR_ARG
 
 
 def test_driver_reports_missing_gpu_or_bad_input(tmp_path):
