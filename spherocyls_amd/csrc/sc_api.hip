@@ -745,6 +745,9 @@ void set_compact(sc_system* s, const std::vector<int>& active) {
 // packing) the update kernel reduces; beyond it k_reduce reduces each packing
 // once, in a launch of its own.
 constexpr double kFuseMaxRereadBytes = 32.0 * 1024 * 1024;
+// Lanes of the chip (256 CUs x 4 SIMDs x 64): the minimiser's tail widens the
+// lanes per particle until its launch has about this many.
+constexpr long kTailLanes = 65536;
 constexpr int kFuseMaxBlocks = 4096;
 
 // Gradient-descent (mode 0) / FIRE (mode 1) step after a force pass: the
@@ -1566,6 +1569,14 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     ~CompactOff() { s->compact = false; }
   } compact_off{s};
   int last_active = s->npack;
+  // the system's own list shape, restored before the final evaluation
+  const int lanes0 = s->list_lanes;
+  auto shape_back = [&]() {
+    if (s->list_lanes != lanes0) {
+      s->list_lanes = lanes0;
+      alloc_pblocks(s);
+    }
+  };
   while (true) {
     if (graph) {
       if (!g.exec) g.capture([&]() { iterations(chunk); });
@@ -1586,6 +1597,7 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
     if (e != SC_OK) {
       s->velocity_live = false;
       s->compact = false;
+      shape_back();
       return e;
     }
     download_scal(s);
@@ -1598,12 +1610,25 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
         HIPCK(hipGraphExecDestroy(g.exec));
         g.exec = nullptr;
       }
+      // A tail of few particles leaves most of the GPU idle and each pass is
+      // as long as one particle's chain of dependent gathers: give every
+      // particle more lanes (up to 8) until the launch has about one lane
+      // per SIMD lane of the chip, which shortens that chain.
+      long nact = 0;
+      for (int k : active) nact += s->packs[k].n;
+      int lanes = s->list_lanes;
+      while (lanes < 8 && nact * lanes < kTailLanes) lanes *= 2;
+      if (lanes != s->list_lanes) {
+        s->list_lanes = lanes;
+        alloc_pblocks(s);
+      }
       set_compact(s, active);
       last_active = (int)active.size();
     }
   }
   s->compact = false;
   s->velocity_live = false;
+  shape_back();
   int status = SC_OK;
   for (int k = 0; k < s->npack; ++k) {
     if (iters) iters[k] = s->scal[k].iters;
