@@ -26,6 +26,8 @@ __global__ void k_segvec(Bank b, int ntot);
 __global__ void k_bin_all(Bank b, const int* pack_of, const PackInfo* packs, const PackScal* scal,
                           int* cnt, int* slot, int* err, int ntot, int wrap_angles);
 __global__ void k_update(UpdateArgs u);
+template <bool TWO_POINT, bool MIXED>
+__global__ void k_fire_small(SmallFireArgs a);
 __global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot, int y_only);
 __global__ void k_scan_cells(const PackInfo* packs, const int* cnt, int* cstart);
 __global__ void k_sort_cells(const int* cstart, int* cnt, int* slot, const int* id, int* perm,
@@ -115,6 +117,8 @@ struct sc_system {
   int prec = SC_PREC_FP64;
   int contact_model = SC_CONTACT_CLOSEST;
   bool use_graphs = true;
+  int small_max = kSmallMax;  // FIRE of packings up to this size in one launch (0: never)
+  int* d_small_idx = nullptr;
   int ncells = 0, ntiles = 0, npblocks = 0;
   std::vector<int4> tiles;
   std::vector<int2> pblocks, tile_ranges, list_ranges;
@@ -809,6 +813,34 @@ void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int m
   s->cur_scal = 1 - s->cur_scal;
 }
 
+// FIRE of the packings in `active` to convergence in one launch of
+// k_fire_small (one workgroup per packing; every packing holds at most
+// s->small_max <= kSmallMax particles).  Positions, velocities, forces and the
+// per-packing scalars are written back in place; the rows of the list path no
+// longer match the moved particles, so every row is rebuilt by the next pass.
+void run_small_fire(sc_system* s, const std::vector<int>& active, const FireParams& fp) {
+  if (!s->d_small_idx) s->d_small_idx = s->dalloc<int>(s->npack);
+  HIPCK(hipMemcpyAsync(s->d_small_idx, active.data(), sizeof(int) * active.size(),
+                       hipMemcpyHostToDevice, s->stream));
+  SmallFireArgs a{};
+  a.b = s->B();
+  a.packs = s->d_packs;
+  a.scal = s->d_scal[s->cur_scal];
+  a.pack_idx = s->d_small_idx;
+  a.err = s->d_err;
+  a.fire = fp;
+  const bool tp = s->contact_model == SC_CONTACT_TWO_POINT;
+  const bool mixed = s->prec == SC_PREC_MIXED;
+  auto kern = tp ? (mixed ? &k_fire_small<true, true> : &k_fire_small<true, false>)
+                 : (mixed ? &k_fire_small<false, true> : &k_fire_small<false, false>);
+  {
+    KernelTimer t(s, SC_KERNEL_FIRE_FORCE);
+    hipLaunchKernelGGL(kern, dim3((unsigned)active.size()), dim3(kSmallThreads), 0, s->stream,
+                       a);
+  }
+  force_rebuild(s);
+}
+
 int reconfigure_if_needed(sc_system* s, const std::vector<std::pair<double, double>>* Lhint);
 bool regrid(sc_system* s);
 // systems up to this many particles step through captured graphs
@@ -1094,6 +1126,12 @@ int sc_get_contact_model(const sc_system* s, int* model) {
 int sc_set_graphs(sc_system* s, int enable) {
   if (!s) return SC_ERR_ARG;
   s->use_graphs = enable != 0;
+  return SC_OK;
+}
+
+int sc_set_small_fire(sc_system* s, int max_particles) {
+  if (!s || max_particles < 0 || max_particles > kSmallMax) return SC_ERR_ARG;
+  s->small_max = max_particles;
   return SC_OK;
 }
 
@@ -1577,8 +1615,21 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       alloc_pblocks(s);
     }
   };
+  // Once every packing still running is small, one launch runs them to the
+  // end (k_fire_small): the tail of a batch then costs no launches per
+  // iteration, and a small system never launches per iteration at all.
+  std::vector<int> active;
+  for (int k = 0; k < s->npack; ++k) active.push_back(k);
+  auto all_small = [&]() {
+    if (s->small_max <= 0 || active.empty()) return false;
+    for (int k : active)
+      if (s->packs[k].n > s->small_max) return false;
+    return true;
+  };
   while (true) {
-    if (graph) {
+    if (all_small()) {
+      run_small_fire(s, active, fp);
+    } else if (graph) {
       if (!g.exec) g.capture([&]() { iterations(chunk); });
       g.launch();
       it += chunk;
@@ -1601,7 +1652,7 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       return e;
     }
     download_scal(s);
-    std::vector<int> active;
+    active.clear();
     for (int k = 0; k < s->npack; ++k)
       if (!s->scal[k].converged) active.push_back(k);
     if (active.empty()) break;

@@ -759,6 +759,272 @@ __global__ void __launch_bounds__(256) k_update(UpdateArgs u) {
   bin_wave<1>(bin_c, g, P.cap, u.cnt, u.slot, u.err);
 }
 
+// FIRE of one small packing per workgroup, from its current state to
+// convergence (or max_iter), in one launch.  An iteration is the force pass,
+// the reduction and decision, and the update of the two-launch path (k_list
+// force pass, k_update), with the same formulas and the same FIRE state
+// machine (decide); only the order of the sums differs.  Nothing of an
+// iteration touches global memory: the packing's coordinates, segment vectors,
+// radii, half lengths, ids, rows and forces live in LDS, and each particle's
+// angle, velocity and build position in the registers of the thread that
+// updates it.
+//
+// Two mappings of threads to particles:
+//   * force pass and row builds: L lanes per particle (the largest power of
+//     two up to 8 with n L <= kSmallThreads), lane l taking row entries
+//     l, l + L, ...; the lanes' sums are combined with xor shuffles and lane 0
+//     stores the particle's force and torque in LDS;
+//   * update: thread t moves particle t, so the FIRE sums, the velocity mixing
+//     and the move (square roots, divisions, sincos) run once per particle, on
+//     ceil(n / 64) wavefronts, not once per lane.
+// A row (the partners within the skin: the selection rule of the list kernels
+// at the fixed strain of a minimisation) is rebuilt from all n particles when
+// a particle has moved half the skin from its build position, as the update
+// kernel decides for the global rows; a ballot places the hits in increasing
+// partner index.
+template <bool TWO_POINT, bool MIXED>
+__global__ void __launch_bounds__(kSmallThreads) k_fire_small(SmallFireArgs a) {
+#pragma clang fp contract(off)
+  constexpr int kRowStride = kSmallRow + 2;  // odd word stride: rows start in different banks
+  constexpr int kWaves = kSmallThreads / 64;
+  constexpr int kFire = 4;                   // FIRE sums: |F|^2 + T^2, F.v, |g|^2, |v|^2
+  __shared__ double s_x[kSmallMax], s_y[kSmallMax], s_nx[kSmallMax], s_ny[kSmallMax];
+  __shared__ double s_R[kSmallMax], s_A[kSmallMax];
+  __shared__ double s_fx[kSmallMax], s_fy[kSmallMax], s_tq[kSmallMax];
+  __shared__ int s_id[kSmallMax];
+  __shared__ unsigned short s_row[kSmallMax * kRowStride];
+  __shared__ double s_part[kWaves * kFire];
+  __shared__ PackScal s_S;
+  // Flags, each written and read in phases separated by barriers so that
+  // every thread sees the same value and all leave the loop together:
+  // s_rebuild (raised by the update, read at the top of the next iteration,
+  // cleared during the decision), s_stop (non-finite coordinate: raised by the
+  // update, read at the top), s_over (row overflow: raised by a rebuild, read
+  // after its barrier).
+  __shared__ int s_rebuild, s_stop, s_over;
+  const Bank& b = a.b;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int pk = a.pack_idx[blockIdx.x];
+  const PackInfo P = a.packs[pk];
+  const int n = P.n;
+  int L = 1, lg = 0;
+  while (L < 8 && n * 2 * L <= kSmallThreads) {
+    L *= 2;
+    ++lg;
+  }
+  // force mapping
+  const int lane = tid & (L - 1);
+  const int p = tid >> lg;
+  const bool own = p < n;
+  const int glane = (tid & 63) & ~(L - 1);  // first lane of the particle's group in the wave
+  // update mapping
+  const bool upd = tid < n;
+  const int nuw = (n + 63) >> 6;  // wavefronts holding update threads
+  const int gu = P.poff + (upd ? tid : 0);
+  double x = b.x[gu], y = b.y[gu], phi = b.phi[gu];
+  double vx = b.vx[gu], vy = b.vy[gu], vp = b.vpsi[gu];
+  const double sq = sqrt(b.I[gu]), Au = b.A[gu];
+  if (upd) {
+    s_x[tid] = x;
+    s_y[tid] = y;
+    s_nx[tid] = b.nx[gu];
+    s_ny[tid] = b.ny[gu];
+    s_R[tid] = b.R[gu];
+    s_A[tid] = Au;
+    s_id[tid] = b.id[gu];
+  }
+  if (tid == 0) {
+    s_S = a.scal[pk];
+    s_rebuild = 1;
+    s_stop = 0;
+    s_over = 0;
+  }
+  __syncthreads();
+  PackScal S = s_S;
+  const double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
+  const double skin = P.pad, sx = 1.5 * P.pad, hs = 0.5 * P.pad;
+  const FireParams& fp = a.fire;
+  double xb = x, yb = y;
+  int cnt = 0;  // row length of the force mapping's particle (same in its L lanes)
+  while (true) {
+    if (s_stop) break;
+    if (s_rebuild) {
+      xb = x;
+      yb = y;
+      cnt = 0;
+      const int pp = own ? p : 0;
+      const double xp = s_x[pp], yp = s_y[pp], reach_p = s_R[pp] + s_A[pp];
+      for (int base = 0; base < n; base += L) {
+        const int q = base + lane;
+        bool hit = false;
+        if (own && q < n && q != p) {
+          const double reach = reach_p + s_R[q] + s_A[q];
+          const double dy = min_image(yp - s_y[q], Ly);
+          const double dx = min_image(xp - s_x[q], Lx);
+          const double f = dx + gam * dy;
+          hit = fabs(dy) < reach + skin && fabs(f) < reach + sx;
+        }
+        const unsigned long long bits = (__ballot(hit) >> glane) & ((1ull << L) - 1ull);
+        const int pos = cnt + __popcll(bits & ((1ull << lane) - 1ull));
+        if (hit && pos < kSmallRow) s_row[p * kRowStride + pos] = (unsigned short)q;
+        cnt += __popcll(bits);
+      }
+      if (cnt > kSmallRow) {
+        if (lane == 0) atomicOr(a.err, kErrList);
+        s_over = 1;
+      }
+      __syncthreads();  // rows and s_over visible
+      if (s_over) break;
+    }
+
+    // ---- force pass (force mapping): bounding-circle hits of this lane's
+    // entries as a bit mask, then the pair math over the set bits
+    const bool fp64 = !MIXED || S.prec == kFP64;
+    double Fx = 0.0, Fy = 0.0, Tq = 0.0;
+    if (own) {
+      const double xp = s_x[p], yp = s_y[p], nxp = s_nx[p], nyp = s_ny[p];
+      const double Rp = s_R[p], Ap = s_A[p], reach_p = Rp + Ap;
+      const int idp = s_id[p];
+      const unsigned short* row = s_row + p * kRowStride;
+      unsigned long long mask = 0;
+      for (int k = lane, j = 0; k < cnt; k += L, ++j) {
+        const int q = row[k];
+        double dx = min_image(xp - s_x[q], Lx);
+        const double dy = min_image(yp - s_y[q], Ly);
+        dx += gam * dy;
+        const double reach = reach_p + s_R[q] + s_A[q];
+        if (dx * dx + dy * dy < reach * reach * (1.0 + 1e-10)) mask |= 1ull << j;
+      }
+      while (mask) {
+        const int j = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int q = row[lane + j * L];
+        pair_into<false, TWO_POINT>(xp, yp, nxp, nyp, Rp, Ap, idp, s_x[q], s_y[q], s_nx[q],
+                                    s_ny[q], s_R[q], s_A[q], s_id[q], Lx, Ly, gam, fp64, false,
+                                    Fx, Fy, Tq, nullptr);
+      }
+    }
+    for (int off = L >> 1; off > 0; off >>= 1) {
+      Fx += __shfl_xor(Fx, off);
+      Fy += __shfl_xor(Fy, off);
+      Tq += __shfl_xor(Tq, off);
+    }
+    if (own && lane == 0) {
+      s_fx[p] = Fx;
+      s_fy[p] = Fy;
+      s_tq[p] = Tq;
+    }
+    __syncthreads();  // forces visible to the update mapping
+
+    // ---- FIRE sums (update mapping), per wavefront in lane order
+    double fx = 0.0, fy = 0.0, tq = 0.0, gp = 0.0;
+    if (wave < nuw) {
+      double acc[kFire] = {0.0, 0.0, 0.0, 0.0};
+      if (upd) {
+        fx = s_fx[tid];
+        fy = s_fy[tid];
+        tq = s_tq[tid];
+        gp = tq / sq;
+        acc[0] = fx * fx + fy * fy + tq * tq;
+        acc[1] = fx * vx + fy * vy + gp * vp;
+        acc[2] = fx * fx + fy * fy + gp * gp;
+        acc[3] = vx * vx + vy * vy + vp * vp;
+      }
+#pragma unroll
+      for (int i = 0; i < kFire; ++i) {
+        double v = acc[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        acc[i] = v;
+      }
+      if ((tid & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < kFire; ++i) s_part[wave * kFire + i] = acc[i];
+    }
+    __syncthreads();  // partials visible
+    if (tid == 0) {
+      double tot[kNSum];
+      for (int i = 0; i < kNSum; ++i) tot[i] = 0.0;
+      for (int w = 0; w < nuw; ++w) {
+        tot[kSumR2] += s_part[w * kFire + 0];
+        tot[kSumP] += s_part[w * kFire + 1];
+        tot[kSumG] += s_part[w * kFire + 2];
+        tot[kSumV] += s_part[w * kFire + 3];
+      }
+      ReduceArgs ra{};
+      ra.fire = fp;
+      ra.do_fire = 1;
+      s_S = decide(ra, P, S, tot);
+      s_rebuild = 0;  // every thread read it before the barriers above
+    }
+    __syncthreads();  // decision visible
+    S = s_S;
+    if (S.converged) break;  // converged (or max_iter) at the state just evaluated
+
+    // ---- update (update mapping; the FIRE branch of k_update)
+    if (upd) {
+      double ux, uy, up;
+      if (S.fire_mix) {
+        ux = S.fire_c_keep * vx + S.fire_c_force * fx;
+        uy = S.fire_c_keep * vy + S.fire_c_force * fy;
+        up = S.fire_c_keep * vp + S.fire_c_force * gp;
+      } else {
+        x -= 0.5 * S.fire_dt_old * vx;
+        y -= 0.5 * S.fire_dt_old * vy;
+        phi -= 0.5 * S.fire_dt_old * vp / sq;
+        ux = uy = up = 0.0;
+      }
+      const double dtn = S.fire_dt_new;
+      ux += dtn * fx;
+      uy += dtn * fy;
+      up += dtn * gp;
+      vx = ux;
+      vy = uy;
+      vp = up;
+      double mx = dtn * ux, my = dtn * uy;
+      const double m2 = mx * mx + my * my;
+      if (m2 > fp.max_move * fp.max_move) {
+        const double sc = fp.max_move / sqrt(m2);
+        mx *= sc;
+        my *= sc;
+      }
+      x += mx;
+      y += my;
+      phi += dtn * up / sq;
+      const bool okx = wrap_coord(x, Lx), oky = wrap_coord(y, Ly);
+      if (!okx || !oky || !isfinite(phi)) {
+        atomicOr(a.err, kErrNonFinite);
+        s_stop = 1;
+      }
+      double sn, cs;
+      sincos(phi, &sn, &cs);
+      s_x[tid] = x;
+      s_y[tid] = y;
+      s_nx[tid] = Au * cs;
+      s_ny[tid] = Au * sn;
+      if (fabs(min_image(x - xb, Lx)) >= hs || fabs(min_image(y - yb, Ly)) >= hs) s_rebuild = 1;
+    }
+    __syncthreads();  // new coordinates, s_rebuild and s_stop visible
+  }
+  if (upd) {
+    b.x[gu] = x;
+    b.y[gu] = y;
+    b.phi[gu] = phi;
+    b.nx[gu] = s_nx[tid];
+    b.ny[gu] = s_ny[tid];
+    b.vx[gu] = vx;
+    b.vy[gu] = vy;
+    b.vpsi[gu] = vp;
+    b.fx[gu] = s_fx[tid];
+    b.fy[gu] = s_fy[tid];
+    b.tau[gu] = s_tq[tid];
+  }
+  if (tid == 0) a.scal[pk] = S;
+}
+template __global__ void k_fire_small<false, false>(SmallFireArgs);
+template __global__ void k_fire_small<true, false>(SmallFireArgs);
+template __global__ void k_fire_small<false, true>(SmallFireArgs);
+template __global__ void k_fire_small<true, true>(SmallFireArgs);
+
 // Affine rescale (x, y) *= scale[pack] (quasi-static compression step).
 __global__ void k_affine(Bank b, const int* pack_of, const double* scale, int ntot, int y_only) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;

@@ -193,6 +193,22 @@ struct UpdateArgs {
   int fused;               // 1: reduce and decide here; 0: k_reduce already wrote red.scal_next
 };
 
+// FIRE of small packings in one launch: one workgroup of kSmallThreads per
+// packing of at most kSmallMax particles holds the packing in LDS and runs
+// every iteration (force pass, reduction, decision, update, list rebuild)
+// itself, so an iteration costs no launch and no global round trip.
+constexpr int kSmallMax = 256;
+constexpr int kSmallThreads = 512;
+constexpr int kSmallRow = kNbrCap;  // partners a particle's LDS row holds
+struct SmallFireArgs {
+  Bank b;
+  const PackInfo* packs;
+  PackScal* scal;          // read at the start, written at the end (in place)
+  const int* pack_idx;     // workgroup -> packing
+  int* err;
+  FireParams fire;
+};
+
 // Device view of a system's current state, for other device code of this
 // library (the message-passing warm start).  Valid until the next call that
 // moves particles or re-sorts the bank.

@@ -464,6 +464,57 @@ __device__ __forceinline__ double iso_rot_seg(double RC, double nx, double ny, d
   return 0.5 * (1 - RC * c2);
 }
 
+// Force of partner q on particle i (i's data in registers), accumulated into
+// (Fx, Fy, Tq) and, for the lower-id side of the pair, into the stress/energy
+// sums.  The pair is evaluated in the reference's orientation (lower particle
+// id first, src/cpp/spherocyl_box.cpp:514-520) so both sides agree bitwise.
+template <bool SUMS, bool TWO_POINT>
+__device__ __forceinline__ void pair_into(double xi, double yi, double nxi, double nyi, double Ri,
+                                          double Ai, int idi, double xq, double yq, double nxq,
+                                          double nyq, double Rq, double Aq, int idq, double Lx,
+                                          double Ly, double gam, bool fp64, bool do_se,
+                                          double& Fx, double& Fy, double& Tq, double* acc) {
+#pragma clang fp contract(off)
+  const bool iP = idi < idq;
+  double ddx = min_image(iP ? xi - xq : xq - xi, Lx);
+  double ddy = min_image(iP ? yi - yq : yq - yi, Ly);
+  ddx += gam * ddy;
+  const double sig = Ri + Rq;
+  const double nxA = iP ? nxi : nxq, nyA = iP ? nyi : nyq;
+  const double nxB = iP ? nxq : nxi, nyB = iP ? nyq : nyi;
+  const double Ap = iP ? Ai : Aq, Aqq = iP ? Aq : Ai;
+  PairOut<double> r0, r1;
+  bool h0, h1;
+  pair_contacts<TWO_POINT>(ddx, ddy, nxA, nyA, nxB, nyB, Ap, Aqq, sig, fp64, r0, h0, r1, h1);
+  if (h0) {
+    if (iP) {
+      Fx += r0.fx;
+      Fy += r0.fy;
+      Tq += r0.tp;
+      if (SUMS && do_se) {
+        add_contact_sums(r0, nxA, nyA, nxB, nyB, sig, acc);
+        acc[kSeNc] += 1.0;
+      }
+    } else {
+      Fx -= r0.fx;
+      Fy -= r0.fy;
+      Tq += r0.tq;
+    }
+  }
+  if (TWO_POINT && h1) {
+    if (iP) {
+      Fx += r1.fx;
+      Fy += r1.fy;
+      Tq += r1.tp;
+      if (SUMS && do_se) add_contact_sums(r1, nxA, nyA, nxB, nyB, sig, acc);
+    } else {
+      Fx -= r1.fx;
+      Fy -= r1.fy;
+      Tq += r1.tq;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // block reduction helpers (fixed order -> deterministic)
 // ---------------------------------------------------------------------------
