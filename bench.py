@@ -197,7 +197,7 @@ def quasistatic_compression(eps=1e-4, phi0=0.80, phi1=0.86, n=16384, samples=13)
             def progress(step, _s):
                 if step % 100 == 0:
                     print(f"quasi-static {prec}: step {step}", file=sys.stderr, flush=True)
-            c = quasistatic.compress(s, eps, phi1, params, on_step=progress)
+            c = quasistatic.compress(s, eps, phi1, params, on_step=progress, tune_fire=True)
         finally:
             s.close()
         curves[prec] = c
@@ -208,7 +208,8 @@ def quasistatic_compression(eps=1e-4, phi0=0.80, phi1=0.86, n=16384, samples=13)
                      "fire_iterations_steps": int(it[1:].sum()),
                      "mean_iterations_per_step": float(it[1:].mean()),
                      "steps_per_s": (len(it) - 1) / c.seconds,
-                     "particle_iterations_per_s": n * float(it.sum()) / c.seconds}
+                     "particle_iterations_per_s": n * float(it.sum()) / c.seconds,
+                     "launch_shape": c.tuned}
     f64, mix = curves["fp64"], curves["mixed"]
     k = min(len(f64.packing), len(mix.packing))
     idx = np.unique(np.linspace(0, len(f64.packing) - 1, samples).astype(int))

@@ -164,6 +164,15 @@ int sc_autotune_pad(sc_system* s, int ncand, const double* pads, int steps, doub
  * or the lanes per particle (list path).  Outputs may be NULL. */
 int sc_autotune(sc_system* s, int reps, int* path, int* shape, int* block_threads,
                 double* us_per_pass);
+/* Autotuner for minimisation workloads (no reference counterpart): for each
+ * candidate path and launch shape, `iterations` FIRE iterations (params as in
+ * sc_fire_minimize, max_iter replaced) from the current positions, restored
+ * after every trial and at the end; keeps the fastest.  A minimiser's force
+ * pass (two-point contacts, list rebuilds as particles move, the update
+ * kernel) ranks the shapes differently from the Heun predictor that
+ * sc_autotune times.  *ms gets the kept shape's wall time for the trial. */
+int sc_autotune_fire(sc_system* s, const double* params, double switch_frms, int iterations,
+                     int* path, int* shape, int* block_threads, double* ms);
 
 /* set_particle for every particle (src/cpp/spherocyl_box.cpp:188-221): new
    positions / angles, wrapped, then cells rebuilt. */

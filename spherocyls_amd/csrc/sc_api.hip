@@ -1944,6 +1944,67 @@ int sc_autotune(sc_system* s, int reps, int* path_out, int* tile_cols_out, int* 
   API_END
 }
 
+int sc_autotune_fire(sc_system* s, const double* params, double switch_frms, int iterations,
+                     int* path_out, int* shape_out, int* block_out, double* ms_out) {
+  if (!s || !params || iterations < 1) return SC_ERR_ARG;
+  API_BEGIN
+  struct Cand {
+    int path, tile, block, lanes;
+  };
+  std::vector<Cand> cands;
+  for (int lanes : {1, 2, 4, 8})
+    for (int blk : {64, 128, 256}) cands.push_back({SC_PATH_LIST, 0, blk, lanes});
+  for (Cand c : {Cand{SC_PATH_TILE, 8, 128, 0}, Cand{SC_PATH_TILE, 12, 128, 0},
+                 Cand{SC_PATH_TILE, 16, 256, 0}})
+    cands.push_back(c);
+  std::vector<double> x(s->ntot), y(s->ntot), phi(s->ntot);
+  int e = sc_get_positions(s, x.data(), y.data(), phi.data());
+  if (e != SC_OK) return e;
+  std::vector<double> p(params, params + 10);
+  p[9] = iterations;  // max_iter
+  // Two interleaved rounds, each candidate scored by its faster one.  Every
+  // trial starts from the saved positions (FIRE zeroes the velocities and
+  // leaves the box alone), so the candidates run the same minimisation.
+  std::vector<double> score(cands.size(), 1e300);
+  for (int round = 0; round < 2; ++round) {
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
+      const Cand& c = cands[ci];
+      if (c.path == SC_PATH_TILE) {
+        if (tile_lds_bytes(3 * (c.tile + 2) * s->cap, c.block / kLanes, c.block) > (size_t)s->lds_max)
+          continue;
+        if (sc_set_tiling(s, c.tile, c.block) != SC_OK) continue;
+      } else if (sc_set_list_block(s, c.block, c.lanes) != SC_OK) {
+        continue;
+      }
+      if ((e = sc_set_path(s, c.path)) != SC_OK) return e;
+      if ((e = sc_set_positions(s, x.data(), y.data(), phi.data())) != SC_OK) return e;
+      HIPCK(hipStreamSynchronize(s->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      e = sc_fire_minimize(s, p.data(), switch_frms, nullptr, nullptr);
+      HIPCK(hipStreamSynchronize(s->stream));
+      if (e != SC_OK && e != SC_ERR_NOT_CONVERGED) return e;
+      const double ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      score[ci] = std::min(score[ci], ms);
+    }
+  }
+  size_t bi = 0;
+  for (size_t ci = 1; ci < cands.size(); ++ci)
+    if (score[ci] < score[bi]) bi = ci;
+  const Cand pick = cands[bi];
+  if (pick.path == SC_PATH_TILE) sc_set_tiling(s, pick.tile, pick.block);
+  else sc_set_list_block(s, pick.block, pick.lanes);
+  if ((e = sc_set_path(s, pick.path)) != SC_OK) return e;
+  if ((e = sc_set_positions(s, x.data(), y.data(), phi.data())) != SC_OK) return e;
+  s->err.clear();
+  if (path_out) *path_out = pick.path;
+  if (shape_out) *shape_out = pick.path == SC_PATH_TILE ? pick.tile : pick.lanes;
+  if (block_out) *block_out = pick.block;
+  if (ms_out) *ms_out = score[bi];
+  return SC_OK;
+  API_END
+}
+
 }  // extern "C"
 
 namespace sc {

@@ -72,6 +72,7 @@ def load():
         "sc_get_path": ([sysp, _ip], C.c_int),
         "sc_set_list_block": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_autotune": ([sysp, C.c_int, _ip, _ip, _ip, _dp], C.c_int),
+        "sc_autotune_fire": ([sysp, _dp, C.c_double, C.c_int, _ip, _ip, _ip, _dp], C.c_int),
         "sc_set_pad": ([sysp, C.c_int, C.c_double], C.c_int),
         "sc_autotune_pad": ([sysp, C.c_int, _dp, C.c_int, _dp, _dp], C.c_int),
         "sc_get_tiling": ([sysp, _ip, _ip], C.c_int),
@@ -371,6 +372,17 @@ class System:
                                       C.byref(us)))
         path = ("tile", "list")[p.value]
         return dict(path=path, shape=t.value, block=b.value, us_per_pass=us.value)
+
+    def autotune_fire(self, params=None, iterations=60, switch_frms=1e-6):
+        """Launch-shape autotuner for minimisation: `iterations` FIRE
+        iterations per candidate path and shape from the current positions
+        (restored after every trial); keeps the fastest."""
+        p = fire_params() if params is None else _f64(params)
+        pa, t, b, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._ck(self.lib.sc_autotune_fire(self.h, _p(p), switch_frms, int(iterations),
+                                           C.byref(pa), C.byref(t), C.byref(b), C.byref(ms)))
+        return dict(path=("tile", "list")[pa.value], shape=t.value, block=b.value,
+                    ms_per_trial=ms.value)
 
     def set_pad(self, pad, k=-1):
         self._ck(self.lib.sc_set_pad(self.h, k, float(pad)))

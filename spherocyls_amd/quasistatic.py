@@ -30,6 +30,7 @@ class Curve:
     iterations: np.ndarray  # FIRE iterations of each minimisation
     frms: np.ndarray
     seconds: float
+    tuned: dict = None      # launch shape kept by the FIRE-timed tuner (tune_fire)
 
 
 def _record(s):
@@ -41,20 +42,26 @@ def _record(s):
 
 
 def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm=None,
-             on_step=None, after_step=None):
+             on_step=None, after_step=None, tune_fire=False):
     """Compress every packing of system s in steps of x, y, L *= 1 - eps until
     its packing fraction reaches phi_final (a scalar or one per packing),
     minimising after each step (mixed precision: FP32 pair math down to
     switch_frms).  warm: a native.Mpnn whose damped proposal precedes FIRE.
     on_step(step, s) runs after each affine and Heun step, before the warm
     start and FIRE, and after_step(step, s) after FIRE (used to record pre-
-    and post-minimisation states).  Leaves the system's dynamics at strain
-    rate 0 and step 0.01.  Returns a Curve."""
+    and post-minimisation states).  tune_fire: at the first compression step,
+    after its affine and Heun step, time FIRE iterations per launch shape from
+    that state (System.autotune_fire, which restores it) and keep the fastest
+    for the rest of the run: a quasi-static minimisation moves particles far
+    less than the first one from a random start, and prefers other shapes.
+    Leaves the system's dynamics at strain rate 0 and step 0.01.  Returns a
+    Curve (its `tuned` field: the shape kept, or None)."""
     from . import warmstart
 
     fin = np.broadcast_to(np.asarray(phi_final, dtype=np.float64), (s.npack,))
     sw = switch_frms if s.precision == "mixed" else 0.0
     rows = []
+    tuned = None
     s.set_dynamics(strain_rate=0.0, resize_rate=0.0, step=0.01)
     s.synchronize()
     t0 = time.perf_counter()
@@ -69,6 +76,8 @@ def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm
         s.strain_steps(1)
         if on_step is not None:
             on_step(step, s)
+        if tune_fire and step == 1:
+            tuned = s.autotune_fire(params, iterations=60, switch_frms=sw)
         if warm is not None:
             warmstart.damped_warm_start(s, s.sizes, warm)
         it, fr, _ = s.fire(params, switch_frms=sw, check=False)
@@ -79,4 +88,4 @@ def compress(s, eps, phi_final, params, switch_frms=1e-6, max_steps=100000, warm
     s.synchronize()
     sec = time.perf_counter() - t0
     cols = list(zip(*rows))
-    return Curve(*(np.array(c) for c in cols), seconds=sec)
+    return Curve(*(np.array(c) for c in cols), seconds=sec, tuned=tuned)

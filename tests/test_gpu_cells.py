@@ -75,3 +75,63 @@ def test_autotune_pad_restores_the_state_and_keeps_a_candidate():
             ref.close()
     finally:
         s.close()
+
+
+def test_autotune_fire_restores_positions_and_keeps_a_shape():
+    """The minimisation autotuner (FIRE iterations per candidate path and
+    launch shape) puts the positions back exactly, and the system then
+    minimises as one built from those positions with the shape it kept."""
+    packs = _packs()
+    params = native.fire_params(max_iter=20000)
+    s = native.System(packs, pads=0.1)
+    try:
+        s.set_contact_model("two_point")
+        before = s.positions()
+        print("autotune_fire 30 iterations per candidate", flush=True)
+        pick = s.autotune_fire(params, iterations=30)
+        print("kept", pick, flush=True)
+        assert pick["path"] in ("list", "tile") and pick["ms_per_trial"] > 0
+        after = s.positions()
+        for u, v in zip(before, after):
+            np.testing.assert_array_equal(u, v)
+        it1, fr1, _ = s.fire(params)
+        p1 = s.positions()
+    finally:
+        s.close()
+    ref = native.System(packs, pads=0.1)
+    try:
+        ref.set_contact_model("two_point")
+        ref.set_path(pick["path"])
+        if pick["path"] == "list":
+            ref.set_list_block(pick["block"], pick["shape"])
+        else:
+            ref.set_tiling(pick["shape"], pick["block"])
+        it2, fr2, _ = ref.fire(params)
+        p2 = ref.positions()
+    finally:
+        ref.close()
+    np.testing.assert_array_equal(it1, it2)
+    for u, v in zip(p1, p2):
+        np.testing.assert_array_equal(u, v)
+    assert np.all(fr1 < 1e-10)
+
+
+def test_compression_tunes_at_its_first_step():
+    """quasistatic.compress(tune_fire=True) times the launch shapes at its
+    first quasi-static step, keeps one, and every minimisation of the run
+    still reaches F_rms < 1e-10."""
+    from spherocyls_amd import quasistatic
+
+    R, A = configs.bidisperse(1200)
+    pk = configs.make_packing(R, A, 0.80, 73, "compress N=1200")
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_contact_model("two_point")
+        print("compress with tune_fire, 4 steps", flush=True)
+        c = quasistatic.compress(s, 1e-3, 0.8032, native.fire_params(max_iter=100000),
+                                 tune_fire=True)
+    finally:
+        s.close()
+    print("tuned", c.tuned, "steps", len(c.packing) - 1, flush=True)
+    assert c.tuned is not None and c.tuned["path"] in ("list", "tile")
+    assert len(c.packing) >= 3 and np.all(c.frms[:, 0] < 1e-10)
