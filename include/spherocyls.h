@@ -119,12 +119,13 @@ int sc_set_contact_model(sc_system* s, int model);
  * (sc_timing_enable) always launches eagerly. */
 int sc_set_graphs(sc_system* s, int enable);
 /* FIRE of small packings in one launch (no reference counterpart; a launch
- * choice like sc_set_graphs).  Once every packing that sc_fire_minimize still
- * advances holds at most max_particles particles (0 <= max_particles <= 256;
- * default 256, 0 = never), one workgroup per packing runs the remaining
- * iterations inside a single kernel: same iteration, decisions and stop rule,
- * sums in another order (results agree to rounding, not bit for bit).  With
- * event timing on, that kernel is counted under SC_KERNEL_FIRE_FORCE. */
+ * choice like sc_set_graphs).  On the list path, once every packing that
+ * sc_fire_minimize still advances holds at most max_particles particles
+ * (0 <= max_particles <= 256; default 256, 0 = never), one workgroup per
+ * packing runs the remaining iterations inside a single kernel: same
+ * iteration, decisions and stop rule, sums in another order (results agree
+ * to rounding, not bit for bit).  With event timing on, that kernel is
+ * counted under SC_KERNEL_FIRE_FORCE. */
 int sc_set_small_fire(sc_system* s, int max_particles);
 int sc_get_contact_model(const sc_system* s, int* model);
 /* Force-pass implementation (no reference counterpart; both give the same

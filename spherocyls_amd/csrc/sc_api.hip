@@ -1617,11 +1617,14 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
   };
   // Once every packing still running is small, one launch runs them to the
   // end (k_fire_small): the tail of a batch then costs no launches per
-  // iteration, and a small system never launches per iteration at all.
+  // iteration, and a small system never launches per iteration at all.  List
+  // path only: its rows hold as many partners as the list kernels' (64) and
+  // are chosen by a rule no wider than theirs, so the one-launch path stops
+  // with SC_ERR_LIST_OVERFLOW only where the two-launch path would.
   std::vector<int> active;
   for (int k = 0; k < s->npack; ++k) active.push_back(k);
   auto all_small = [&]() {
-    if (s->small_max <= 0 || active.empty()) return false;
+    if (s->small_max <= 0 || s->path != SC_PATH_LIST || active.empty()) return false;
     for (int k : active)
       if (s->packs[k].n > s->small_max) return false;
     return true;

@@ -131,6 +131,28 @@ def test_tail_of_a_batch_and_tiny_packings():
         assert ob.frms() < 1e-10, pk.name
 
 
+def test_tile_path_keeps_two_launches():
+    """The one-launch kernel runs on the list path only (its row capacity is
+    the list kernels'): on the tile path a small packing minimises the same,
+    bit for bit, whatever the small-packing setting."""
+    R, A = configs.bidisperse(120)
+    pk = configs.make_packing(R, A, 0.84, 9)
+    out = []
+    for small in (256, 0):
+        s = native.System(pk, pads=0.1)
+        try:
+            s.set_contact_model("two_point")
+            s.set_path("tile")
+            s.set_small_fire(small)
+            it, fr, _ = s.fire(native.fire_params(**FIRE, max_iter=50000))
+            out.append((it.copy(), s.positions()))
+        finally:
+            s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for u, v in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(u, v)
+
+
 def test_small_fire_argument_range():
     R, A = configs.bidisperse(64)
     pk = configs.make_packing(R, A, 0.80, 5)
