@@ -842,7 +842,9 @@ __global__ void __launch_bounds__(kSmallThreads) k_fire_small(SmallFireArgs a) {
   __syncthreads();
   PackScal S = s_S;
   const double Lx = S.Lx, Ly = S.Ly, gam = S.gamma;
-  const double skin = P.pad, sx = 1.5 * P.pad, hs = 0.5 * P.pad;
+  // the list kernels' selection rule and skin (the x margin covers the
+  // largest strain before the remap, Lx / (2 Ly))
+  const double skin = P.pad, sx = fmax(1.5, 1.0 + 0.5 * Lx / Ly) * P.pad, hs = 0.5 * P.pad;
   const FireParams& fp = a.fire;
   double xb = x, yb = y;
   int cnt = 0;  // row length of the force mapping's particle (same in its L lanes)

@@ -16,14 +16,17 @@
 // own row, before using it; no host decision and no extra launch are involved.
 //
 // Selection rule (skin s <= cell pad; S = R_p + A_p + R_q + A_q; dx_s, dy minimum
-// images in sheared coordinates; strain window [g_lo, g_hi]):
-//     |dy| < S + s  and  ( |dx_s + g_lo dy| < S + 1.5 s  or  |dx_s + g_hi dy| < S + 1.5 s )
+// images in sheared coordinates; strain window [g_lo, g_hi]; c = max(1.5,
+// 1 + Lx / (2 Ly))):
+//     |dy| < S + s  and  ( |dx_s + g_lo dy| < S + c s  or  |dx_s + g_hi dy| < S + c s )
 // A pair that touches at any strain in the window with every particle within
 // s/2 of its build position satisfies it: its lab x-separation at that strain
-// changed by less than (1 + |g|) s <= 1.5 s, and a linear function of the strain
-// is smallest in magnitude at an end of the window unless it crosses zero, when
-// one end is within |dy| (g_hi - g_lo) / 2.  Such a pair also lies in adjacent
-// cells at build time (|dx_s| < 1.118 S + s <= 2.24 (Rmax + Amax) + pad).
+// changed by less than (1 + |g|) s <= c s (the strain stays below the remap
+// threshold Lx / (2 Ly), which is 1/2 in a square box), and a linear function
+// of the strain is smallest in magnitude at an end of the window unless it
+// crosses zero, when one end is within |dy| (g_hi - g_lo) / 2.  Such a pair
+// also lies in adjacent cells at build time (the cell width bounds its
+// sheared x-separation).
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -141,7 +144,9 @@ __device__ int build_row(const ListArgs& a, const PackInfo& P, double Lx, double
     for (int k = 0; k < nc; ++k) dup |= cs[k] == c;
     if (!dup) cs[nc++] = c;
   }
-  const double s = P.pad, sx = 1.5 * P.pad;  // skin = the packing's cell padding
+  // skin = the packing's cell padding; the x margin covers the largest strain
+  // before the remap (selection rule above)
+  const double s = P.pad, sx = fmax(1.5, 1.0 + 0.5 * Lx / Ly) * P.pad;
   int n = 0;
   for (int i = 0; i < nr; ++i)
     for (int j = 0; j < nc; ++j) {
