@@ -479,6 +479,98 @@ __global__ void k_graph_fill(SystemView v, const int* __restrict__ off, int* __r
   }
 }
 
+// Exclusive prefix sums of the node degrees, off[g] = deg[0] + ... + deg[g-1]
+// and off[n] = the number of edges, in three launches: the sum of each tile
+// of kScanTile degrees, a one-workgroup scan of the tile sums, then each
+// tile's offsets.  Integer sums: the offsets are exact.
+constexpr int kScanTile = 1024;  // 256 threads x 4 degrees
+__global__ void __launch_bounds__(256) k_scan_tiles(const int* __restrict__ deg, int n,
+                                                    int* __restrict__ tsum) {
+  __shared__ int s[256];
+  const int t = threadIdx.x, base = blockIdx.x * kScanTile + 4 * t;
+  int v = 0;
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) v += deg[base + k];
+  s[t] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) s[t] += s[t + h];
+    __syncthreads();
+  }
+  if (t == 0) tsum[blockIdx.x] = s[0];
+}
+
+// One workgroup: exclusive scan of the nt tile sums in place, 1024 at a time.
+__global__ void __launch_bounds__(1024) k_scan_tile_sums(int* tsum, int nt) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nt; base += 1024) {
+    const int i = base + t;
+    const int v = i < nt ? tsum[i] : 0;
+    s[t] = v;
+    __syncthreads();
+    for (int h = 1; h < 1024; h <<= 1) {
+      const int add = t >= h ? s[t - h] : 0;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    if (i < nt) tsum[i] = carry + s[t] - v;
+    __syncthreads();  // every thread has read carry
+    if (t == 1023) carry += s[1023];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan_offsets(const int* __restrict__ deg, int n,
+                                                      const int* __restrict__ tsum,
+                                                      int* __restrict__ off) {
+  __shared__ int s[256];
+  const int t = threadIdx.x, base = blockIdx.x * kScanTile + 4 * t;
+  int v[4], sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? deg[base + k] : 0;
+    sum += v[k];
+  }
+  s[t] = sum;
+  __syncthreads();
+  for (int h = 1; h < 256; h <<= 1) {
+    const int add = t >= h ? s[t - h] : 0;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  int run = tsum[blockIdx.x] + s[t] - sum;
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) off[base + k] = run;
+    run += v[k];
+  }
+  // degrees past n count zero, so the last thread ends on the total
+  if (blockIdx.x == gridDim.x - 1 && t == 255) off[n] = run;
+}
+
+// Reverse edge of each edge k (receiver dst[k], sender src[k]): the first edge
+// of the sender's in-edges whose sender is that receiver.  A contact graph is
+// symmetric; a missing reverse edge raises *bad.
+__global__ void k_graph_rev(const int* __restrict__ off, const int* __restrict__ src,
+                            const int* __restrict__ dst, int e, int* __restrict__ rev,
+                            int* __restrict__ bad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e) return;
+  const int i = dst[k], j = src[k];
+  int r = -1;
+  for (int t = off[j]; t < off[j + 1]; ++t)
+    if (src[t] == i) {
+      r = t;
+      break;
+    }
+  rev[k] = r;
+  if (r < 0) atomicOr(bad, 1);
+}
+
 // x, y, phi of storage particle g += scale * y_g, each component clamped to
 // +-cap (a proposal, not a step the minimiser must trust) and non-finite
 // outputs dropped
@@ -538,6 +630,8 @@ struct sc_mpnn {
   float* part = nullptr;
   size_t pcap = 0;
   int* d_deg = nullptr;
+  int* d_scan = nullptr;  // per-tile sums of the degree scan
+  int* d_flag = nullptr;  // graph build: a contact without its reverse edge
 
   template <typename T>
   T* dalloc(size_t count) {
@@ -607,6 +701,8 @@ void ensure_capacity(sc_mpnn* m, int n, int e) {
   realloc_dev(m, m->dH2, (size_t)nc * kH);
   realloc_dev(m, m->dEin, (size_t)ec * kKe);
   realloc_dev(m, m->d_deg, nc + 1);
+  realloc_dev(m, m->d_scan, (size_t)nc / 1024 + 2);
+  if (!m->d_flag) m->d_flag = m->template dalloc<int>(1);
   const size_t chunks = (size_t)std::max(nc, ec) / kChunk + 1;
   m->pcap = chunks * kKe * kH;
   realloc_dev(m, m->P, m->pcap);
@@ -983,37 +1079,35 @@ int sc_mpnn_graph_from_system(sc_mpnn* m, sc_system* s, int* n_out, int* e_out) 
   if (v.device != m->device) return fail(m, SC_ERR_ARG, "system and network on different devices");
   MPCK(hipStreamSynchronize(v.stream));
   const int n = v.ntot;
+  const int nt = (n + kScanTile - 1) / kScanTile;
+  // degrees, their prefix sums and the edge count, all on the device
+  auto offsets = [&]() -> int {
+    hipLaunchKernelGGL(k_graph_count, dim3(blocks(n)), dim3(256), 0, m->stream, v, m->d_deg);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, m->stream, m->d_deg, n, m->d_scan);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, m->stream, m->d_scan, nt);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(nt), dim3(256), 0, m->stream, m->d_deg, n, m->d_scan,
+                       m->off);
+    int total = 0;
+    MPCK(hipMemcpyAsync(&total, m->off + n, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    MPCK(hipStreamSynchronize(m->stream));
+    return total;
+  };
   ensure_capacity(m, n, m->ecap);
-  hipLaunchKernelGGL(k_graph_count, dim3(blocks(n)), dim3(256), 0, m->stream, v, m->d_deg);
-  std::vector<int> deg(n), off(n + 1, 0);
-  MPCK(hipMemcpyAsync(deg.data(), m->d_deg, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
-  MPCK(hipStreamSynchronize(m->stream));
-  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + deg[i];
-  const int e = off[n];
-  ensure_capacity(m, n, e);
-  MPCK(hipMemcpyAsync(m->off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
-                      m->stream));
+  int e = offsets();
+  if (e > m->ecap) {  // growing the edge arrays reallocates the offsets too: count again
+    ensure_capacity(m, n, e);
+    e = offsets();
+  }
   hipLaunchKernelGGL(k_graph_fill, dim3(blocks(n)), dim3(256), 0, m->stream, v, m->off, m->src,
                      m->dst, m->nf, m->ef);
-  // reverse-edge index (host): senders within a group are ordered by caller id,
-  // so look them up through the storage -> caller id map
-  std::vector<int> src(e), dst(e), id(n);
-  MPCK(hipMemcpyAsync(src.data(), m->src, sizeof(int) * e, hipMemcpyDeviceToHost, m->stream));
-  MPCK(hipMemcpyAsync(id.data(), v.b.id, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
+  MPCK(hipMemsetAsync(m->d_flag, 0, sizeof(int), m->stream));
+  if (e > 0)
+    hipLaunchKernelGGL(k_graph_rev, dim3(blocks(e)), dim3(256), 0, m->stream, m->off, m->src,
+                       m->dst, e, m->rev, m->d_flag);
+  int bad = 0;
+  MPCK(hipMemcpyAsync(&bad, m->d_flag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
   MPCK(hipStreamSynchronize(m->stream));
-  std::vector<int> rev(e, -1);
-  for (int i = 0; i < n; ++i)
-    for (int k = off[i]; k < off[i + 1]; ++k) {
-      const int j = src[k];
-      for (int r = off[j]; r < off[j + 1]; ++r)
-        if (src[r] == i) {
-          rev[k] = r;
-          break;
-        }
-      if (rev[k] < 0) return fail(m, SC_ERR_STATE, "contact graph is not symmetric");
-    }
-  MPCK(hipMemcpyAsync(m->rev, rev.data(), sizeof(int) * e, hipMemcpyHostToDevice, m->stream));
-  MPCK(hipStreamSynchronize(m->stream));
+  if (bad) return fail(m, SC_ERR_STATE, "contact graph is not symmetric");
   m->n = n;
   m->e = e;
   m->has_forward = false;
