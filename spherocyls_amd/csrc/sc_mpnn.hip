@@ -227,45 +227,93 @@ __global__ void k_build_x(const float* __restrict__ nf, int n, bf16* __restrict_
   }
 }
 
-// Ein[e] = [h_dst, h_src, e_feat, 1, 0...]; one thread per (edge, column).
+// The input builders write 8 columns per thread: two 16-byte loads of FP32
+// features, one 16-byte store of 8 BF16 values (each column is converted, and
+// the node input summed over in-edges, in the same order as one thread per
+// column would).
+static_assert(kH % 8 == 0 && kKe % 8 == 0 && kKu % 8 == 0 && kKo % 8 == 0,
+              "builder rows are whole 8-column chunks");
+__device__ __forceinline__ void load8(const float* __restrict__ p, float (&v)[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  const float4 b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+  v[4] = b.x;
+  v[5] = b.y;
+  v[6] = b.z;
+  v[7] = b.w;
+}
+
+__device__ __forceinline__ void store8(bf16* __restrict__ p, const float (&v)[8]) {
+  uint4 w;
+  w.x = (unsigned)to_bf16(v[0]) | ((unsigned)to_bf16(v[1]) << 16);
+  w.y = (unsigned)to_bf16(v[2]) | ((unsigned)to_bf16(v[3]) << 16);
+  w.z = (unsigned)to_bf16(v[4]) | ((unsigned)to_bf16(v[5]) << 16);
+  w.w = (unsigned)to_bf16(v[6]) | ((unsigned)to_bf16(v[7]) << 16);
+  *reinterpret_cast<uint4*>(p) = w;
+}
+
+// Ein[e] = [h_dst, h_src, e_feat, 1, 0...]; one thread per (edge, 8 columns).
 __global__ void k_build_edge_in(const float* __restrict__ H, const int* __restrict__ src,
                                 const int* __restrict__ dst, const float* __restrict__ ef, int ne,
                                 bf16* __restrict__ Ein) {
+  constexpr int kCh = kKe / 8, kHc = kH / 8;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)ne * kKe) return;
-  const int e = (int)(t / kKe), c = (int)(t % kKe);
-  float v;
-  if (c < kH) v = H[(size_t)dst[e] * kH + c];
-  else if (c < 2 * kH) v = H[(size_t)src[e] * kH + (c - kH)];
-  else if (c < 2 * kH + kEdgeFeat) v = ef[(size_t)e * kEdgeFeat + (c - 2 * kH)];
-  else v = c == 2 * kH + kEdgeFeat ? 1.0f : 0.0f;
-  Ein[t] = to_bf16(v);
+  if (t >= (size_t)ne * kCh) return;
+  const int e = (int)(t / kCh), ch = (int)(t % kCh);
+  float v[8];
+  if (ch < kHc) {
+    load8(H + (size_t)dst[e] * kH + 8 * ch, v);
+  } else if (ch < 2 * kHc) {
+    load8(H + (size_t)src[e] * kH + 8 * (ch - kHc), v);
+  } else {
+    for (int k = 0; k < 8; ++k) {
+      const int c = 8 * ch + k;
+      v[k] = c < 2 * kH + kEdgeFeat ? ef[(size_t)e * kEdgeFeat + (c - 2 * kH)]
+                                    : (c == 2 * kH + kEdgeFeat ? 1.0f : 0.0f);
+    }
+  }
+  store8(Ein + (size_t)e * kKe + 8 * ch, v);
 }
 
-// Uin[i] = [h_i, sum over in-edges of silu(z), 1, 0...]; one thread per (node, column).
+// Uin[i] = [h_i, sum over in-edges of silu(z), 1, 0...]; one thread per (node, 8 columns).
 __global__ void k_build_node_in(const float* __restrict__ H, const float* __restrict__ Z,
                                 const int* __restrict__ off, int n, bf16* __restrict__ Uin) {
+  constexpr int kCh = kKu / 8, kHc = kH / 8;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n * kKu) return;
-  const int i = (int)(t / kKu), c = (int)(t % kKu);
-  float v;
-  if (c < kH) {
-    v = H[(size_t)i * kH + c];
-  } else if (c < 2 * kH) {
-    v = 0.0f;
-    for (int e = off[i]; e < off[i + 1]; ++e) v += silu(Z[(size_t)e * kH + (c - kH)]);
+  if (t >= (size_t)n * kCh) return;
+  const int i = (int)(t / kCh), ch = (int)(t % kCh);
+  float v[8];
+  if (ch < kHc) {
+    load8(H + (size_t)i * kH + 8 * ch, v);
+  } else if (ch < 2 * kHc) {
+    for (int k = 0; k < 8; ++k) v[k] = 0.0f;
+    for (int e = off[i]; e < off[i + 1]; ++e) {
+      float z[8];
+      load8(Z + (size_t)e * kH + 8 * (ch - kHc), z);
+      for (int k = 0; k < 8; ++k) v[k] += silu(z[k]);
+    }
   } else {
-    v = c == 2 * kH ? 1.0f : 0.0f;
+    for (int k = 0; k < 8; ++k) v[k] = 8 * ch + k == 2 * kH ? 1.0f : 0.0f;
   }
-  Uin[t] = to_bf16(v);
+  store8(Uin + (size_t)i * kKu + 8 * ch, v);
 }
 
+// Hin[i] = [h_i, 1, 0...]; one thread per (node, 8 columns).
 __global__ void k_build_out_in(const float* __restrict__ H, int n, bf16* __restrict__ Hin) {
+  constexpr int kCh = kKo / 8, kHc = kH / 8;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n * kKo) return;
-  const int i = (int)(t / kKo), c = (int)(t % kKo);
-  float v = c < kH ? H[(size_t)i * kH + c] : (c == kH ? 1.0f : 0.0f);
-  Hin[t] = to_bf16(v);
+  if (t >= (size_t)n * kCh) return;
+  const int i = (int)(t / kCh), ch = (int)(t % kCh);
+  float v[8];
+  if (ch < kHc) {
+    load8(H + (size_t)i * kH + 8 * ch, v);
+  } else {
+    for (int k = 0; k < 8; ++k) v[k] = 8 * ch + k == kH ? 1.0f : 0.0f;
+  }
+  store8(Hin + (size_t)i * kKo + 8 * ch, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -753,16 +801,16 @@ void forward(sc_mpnn* m) {
   gemm_nt<4, kEpiSiluH>(m, m->X, kKx, m->Wt + L.we, kKx, n, kKx, m->Ze, kH, nullptr, m->Hs[0], kH);
   for (int l = 0; l < kLayers; ++l) {
     if (e > 0) {
-      hipLaunchKernelGGL(k_build_edge_in, dim3(blocks((size_t)e * kKe)), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_build_edge_in, dim3(blocks((size_t)e * (kKe / 8))), dim3(256), 0, st,
                          m->Hs[l], m->src, m->dst, m->ef, e, m->Ein[l]);
       gemm_nt<4, kEpiStore>(m, m->Ein[l], kKe, m->Wt + L.wm[l], kKe, e, kKe, m->Z[l], kH);
     }
-    hipLaunchKernelGGL(k_build_node_in, dim3(blocks((size_t)n * kKu)), dim3(256), 0, st, m->Hs[l],
-                       m->Z[l], m->off, n, m->Uin[l]);
+    hipLaunchKernelGGL(k_build_node_in, dim3(blocks((size_t)n * (kKu / 8))), dim3(256), 0, st,
+                       m->Hs[l], m->Z[l], m->off, n, m->Uin[l]);
     gemm_nt<4, kEpiResid>(m, m->Uin[l], kKu, m->Wt + L.wu[l], kKu, n, kKu, m->U[l], kH, m->Hs[l],
                           m->Hs[l + 1], kH);
   }
-  hipLaunchKernelGGL(k_build_out_in, dim3(blocks((size_t)n * kKo)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_build_out_in, dim3(blocks((size_t)n * (kKo / 8))), dim3(256), 0, st,
                      m->Hs[kLayers], n, m->Hin);
   gemm_nt<1, kEpiStore>(m, m->Hin, kKo, m->Wt + L.wo, kKo, n, kKo, m->Y, kNo);
   m->has_forward = true;
