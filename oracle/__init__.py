@@ -1,12 +1,12 @@
 """ORACLE -- test infrastructure only.
 
-ctypes bindings for the two CPU checkers:
+ctypes bindings for the two CPU references:
   * ``OracleBox``  -- liboracle.so, the project's CPU restatement (sc_oracle.cpp)
   * ``RefBox``     -- _ref/libspherocyl_ref.so, the reference's own src/cpp
-                      compiled from /root/reference (present only where that
-                      tree exists and ``make ref`` ran)
+                      compiled from the reference's own sources (present
+                      only where those exist and ``make ref`` ran)
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
-this package, and only as the checker or the timed CPU baseline.
+this package, and only to test results against or as the timed CPU baseline.
 """
 from __future__ import annotations
 

@@ -2,9 +2,9 @@
 //
 // A CPU restatement of the spherocylinder dynamics of tmarschall/spherocyls
 // (src/cpp), written from the equations and operation order of that code so
-// that it agrees with the compiled reference to rounding.  It is the checker for
-// the HIP path and the timed CPU baseline ("port"); nothing on the product path
-// may link or call it.
+// that it agrees with the compiled reference to rounding.  The HIP path is
+// tested against it, and it is the timed CPU baseline ("port"); nothing on the
+// product path may link or call it.
 //
 // Pinning (what each part is checked against, see tests/):
 //   pair force / torque / E / stress .. golden vectors from the compiled
@@ -611,7 +611,7 @@ void heun_step(orc_box* b) {
 //   corrector  forces of the predicted state in the new box at the same
 //              strain gamma; x = x~ + dt (Fx~ - Fx) / 2, likewise y, and phi
 //              with torque / I
-// gamma_total advances by dt er.  The checker rebuilds cells and lists every
+// gamma_total advances by dt er.  The oracle rebuilds cells and lists every
 // step.
 void pure_step(orc_box* b) {
   const int n = b->n;

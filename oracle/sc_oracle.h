@@ -1,6 +1,7 @@
 // Test infrastructure only: the CPU oracle restatement.  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
-// as the checker / the timed CPU baseline -- never on the product path.
+// to test results against or as the timed CPU baseline -- never on the product
+// path.
 //
 // C ABI of liboracle.so.  Every function follows the reference CPU code
 // (tmarschall/spherocyls src/cpp) cited beside it; FIRE has no reference
