@@ -184,6 +184,8 @@ class OracleBox(_BoxBase):
         self.lib.orc_set_contact_model(self.h, int(model))
 
     def set_positions(self, x=None, y=None, phi=None):
+        """Overwrite coordinates; the neighbour list is kept (call
+        find_neighbors() before evaluating a state far from the old one)."""
         x, y, phi = (None if v is None else _f64(v) for v in (x, y, phi))
         self.lib.orc_set_positions(self.h, _ptr(x), _ptr(y), _ptr(phi))
 

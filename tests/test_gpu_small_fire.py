@@ -131,6 +131,42 @@ def test_tail_of_a_batch_and_tiny_packings():
         assert ob.frms() < 1e-10, pk.name
 
 
+@pytest.mark.parametrize("small", [256, 0])
+def test_fire_in_a_non_square_box(small):
+    """After rotate_by_gamma (Lx / Ly = 1 + gamma^2) and at a strain just below
+    the box's remap strain Lx / (2 Ly) > 1/2, where the rows' x margin must
+    exceed 1.5 skins: FIRE in one launch and in two launches reaches F_rms <
+    1e-10 as the oracle, in the same box at the same strain, judges it."""
+    R, A = configs.bidisperse(200)
+    gamma = 0.4
+    pk = configs.make_packing(R, A, 0.86, 31, "non-square", gamma=gamma)
+    params = native.fire_params(**FIRE, max_iter=50000)
+    s = native.System(pk, pads=0.1)
+    try:
+        s.set_contact_model("two_point")
+        s.set_gamma(gamma, 0.0)
+        s.rotate_by_gamma()
+        b = s.box()
+        g1 = 0.5 * b["Lx"] / b["Ly"] - 0.01
+        assert g1 > 0.55
+        s.set_gamma(g1, 0.0)
+        s.set_small_fire(small)
+        it, fr, st = s.fire(params)
+        x, y, phi = s.positions()
+        print("non-square FIRE, small", small, "iterations", int(it[0]), flush=True)
+    finally:
+        s.close()
+    ob = oracle.OracleBox(pk.L, pk.x, pk.y, pk.phi, pk.A, pk.R, 0.1)
+    ob.set_params(gamma, 0.0, 0.0, 0.0, 0.05)
+    ob.rotate_by_gamma()
+    ob.set_contact_model(1)
+    ob.set_params(g1, 0.0, 0.0, 0.0, 0.05)
+    ob.set_positions(x, y, phi)
+    ob.find_neighbors()
+    assert ob.scalars()["Lx"] == pytest.approx(b["Lx"], rel=1e-15)
+    assert ob.frms() < 1e-10
+
+
 def test_tile_path_keeps_two_launches():
     """The one-launch kernel runs on the list path only (its row capacity is
     the list kernels'): on the tile path a small packing minimises the same,
