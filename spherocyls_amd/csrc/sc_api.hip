@@ -1647,6 +1647,9 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
         }
       }
     }
+    // the scalars come back with the error word: one synchronisation per chunk
+    HIPCK(hipMemcpyAsync(s->scal.data(), s->d_scal[s->cur_scal], sizeof(PackScal) * s->npack,
+                         hipMemcpyDeviceToHost, s->stream));
     int e = check_errors(s);
     if (e != SC_OK) {
       s->velocity_live = false;
@@ -1654,7 +1657,6 @@ int sc_fire_minimize(sc_system* s, const double* params, double switch_frms, int
       shape_back();
       return e;
     }
-    download_scal(s);
     active.clear();
     for (int k = 0; k < s->npack; ++k)
       if (!s->scal[k].converged) active.push_back(k);
