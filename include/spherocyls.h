@@ -124,9 +124,11 @@ int sc_set_graphs(sc_system* s, int enable);
  * (0 <= max_particles <= 256; default 256, 0 = never), one workgroup per
  * packing runs the remaining iterations inside a single kernel: same
  * iteration, decisions and stop rule, sums in another order (results agree
- * to rounding, not bit for bit).  With event timing on, that kernel is
- * counted under SC_KERNEL_FIRE_FORCE. */
-int sc_set_small_fire(sc_system* s, int max_particles);
+ * to rounding, not bit for bit).  A launch runs at most steps_per_launch
+ * iterations (0: the default, 50000) and the packings still running are
+ * launched again, so no single kernel runs long.  With event timing on, that
+ * kernel is counted under SC_KERNEL_FIRE_FORCE. */
+int sc_set_small_fire(sc_system* s, int max_particles, int steps_per_launch);
 int sc_get_contact_model(const sc_system* s, int* model);
 /* Force-pass implementation (no reference counterpart; both give the same
  * forces to rounding):

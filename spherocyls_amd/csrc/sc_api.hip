@@ -118,6 +118,7 @@ struct sc_system {
   int contact_model = SC_CONTACT_CLOSEST;
   bool use_graphs = true;
   int small_max = kSmallMax;  // FIRE of packings up to this size in one launch (0: never)
+  int small_steps = 50000;    // ... for at most this many iterations per launch
   int* d_small_idx = nullptr;
   int ncells = 0, ntiles = 0, npblocks = 0;
   std::vector<int4> tiles;
@@ -813,9 +814,9 @@ void launch_update(sc_system* s, const PassSpec& ps, const FireParams& fp, int m
   s->cur_scal = 1 - s->cur_scal;
 }
 
-// FIRE of the packings in `active` to convergence in one launch of
-// k_fire_small (one workgroup per packing; every packing holds at most
-// s->small_max <= kSmallMax particles).  Positions, velocities, forces and the
+// FIRE of the packings in `active` in one launch of k_fire_small (one
+// workgroup per packing; every packing holds at most s->small_max <= kSmallMax
+// particles), to convergence or for at most s->small_steps iterations.  Positions, velocities, forces and the
 // per-packing scalars are written back in place; the rows of the list path no
 // longer match the moved particles, so every row is rebuilt by the next pass.
 void run_small_fire(sc_system* s, const std::vector<int>& active, const FireParams& fp) {
@@ -829,6 +830,11 @@ void run_small_fire(sc_system* s, const std::vector<int>& active, const FirePara
   a.pack_idx = s->d_small_idx;
   a.err = s->d_err;
   a.fire = fp;
+  // a launch stops after s->small_steps iterations (50000 by default, about
+  // half a second), so no single kernel runs long; the minimiser's loop
+  // launches again for packings still running, whose rows are rebuilt from
+  // the state reached
+  a.max_steps = s->small_steps;
   const bool tp = s->contact_model == SC_CONTACT_TWO_POINT;
   const bool mixed = s->prec == SC_PREC_MIXED;
   auto kern = tp ? (mixed ? &k_fire_small<true, true> : &k_fire_small<true, false>)
@@ -1129,9 +1135,11 @@ int sc_set_graphs(sc_system* s, int enable) {
   return SC_OK;
 }
 
-int sc_set_small_fire(sc_system* s, int max_particles) {
-  if (!s || max_particles < 0 || max_particles > kSmallMax) return SC_ERR_ARG;
+int sc_set_small_fire(sc_system* s, int max_particles, int steps_per_launch) {
+  if (!s || max_particles < 0 || max_particles > kSmallMax || steps_per_launch < 0)
+    return SC_ERR_ARG;
   s->small_max = max_particles;
+  s->small_steps = steps_per_launch > 0 ? steps_per_launch : 50000;
   return SC_OK;
 }
 

@@ -848,8 +848,8 @@ __global__ void __launch_bounds__(kSmallThreads) k_fire_small(SmallFireArgs a) {
   const FireParams& fp = a.fire;
   double xb = x, yb = y;
   int cnt = 0;  // row length of the force mapping's particle (same in its L lanes)
-  while (true) {
-    if (s_stop) break;
+  for (int steps = 0;; ++steps) {
+    if (s_stop || steps == a.max_steps) break;
     if (s_rebuild) {
       xb = x;
       yb = y;

@@ -207,6 +207,7 @@ struct SmallFireArgs {
   const int* pack_idx;     // workgroup -> packing
   int* err;
   FireParams fire;
+  int max_steps;           // iterations per launch: the host launches again for packings left
 };
 
 // Device view of a system's current state, for other device code of this

@@ -65,7 +65,7 @@ def load():
         "sc_set_precision": ([sysp, C.c_int], C.c_int),
         "sc_set_contact_model": ([sysp, C.c_int], C.c_int),
         "sc_set_graphs": ([sysp, C.c_int], C.c_int),
-        "sc_set_small_fire": ([sysp, C.c_int], C.c_int),
+        "sc_set_small_fire": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_get_contact_model": ([sysp, _ip], C.c_int),
         "sc_set_tiling": ([sysp, C.c_int, C.c_int], C.c_int),
         "sc_set_path": ([sysp, C.c_int], C.c_int),
@@ -342,10 +342,11 @@ class System:
     def set_graphs(self, enable):
         self._ck(self.lib.sc_set_graphs(self.h, int(bool(enable))))
 
-    def set_small_fire(self, max_particles):
+    def set_small_fire(self, max_particles, steps_per_launch=0):
         """FIRE of packings of at most max_particles (<= 256; 0: never) in one
-        launch once every running packing is that small."""
-        self._ck(self.lib.sc_set_small_fire(self.h, int(max_particles)))
+        launch once every running packing is that small, at most
+        steps_per_launch iterations per launch (0: the default, 50000)."""
+        self._ck(self.lib.sc_set_small_fire(self.h, int(max_particles), int(steps_per_launch)))
 
     def contact_model(self):
         m = C.c_int()

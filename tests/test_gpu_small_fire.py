@@ -189,6 +189,31 @@ def test_tile_path_keeps_two_launches():
         np.testing.assert_array_equal(u, v)
 
 
+def test_launches_of_a_few_iterations_each():
+    """With 7 iterations per launch the minimiser launches the one-launch
+    kernel again and again (a few hundred times here); every launch resumes
+    from the state, velocities and FIRE scalars the last one left, so the
+    packing still reaches a minimum the oracle accepts, in about as many
+    iterations as in one long launch."""
+    R, A = configs.bidisperse(160, aspect=2.0)
+    pk = configs.make_packing(R, A, 0.90, 3, "N=160 phi 0.90")
+    out = {}
+    for steps in (0, 7):
+        s = native.System(pk, pads=0.1)
+        try:
+            s.set_contact_model("two_point")
+            s.set_small_fire(256, steps)
+            it, fr, st = s.fire(native.fire_params(**FIRE, max_iter=50000))
+            x, y, phi = s.positions()
+        finally:
+            s.close()
+        print("launches of", steps or "50000", "iterations:", int(it[0]), "iterations", flush=True)
+        assert st == 0 and fr[0] < 1e-10
+        assert _oracle_box(pk, x, y, phi, "two_point").frms() < 1e-10
+        out[steps] = int(it[0])
+    assert 0.5 * out[0] < out[7] < 2.0 * out[0]
+
+
 def test_small_fire_argument_range():
     R, A = configs.bidisperse(64)
     pk = configs.make_packing(R, A, 0.80, 5)
@@ -197,6 +222,8 @@ def test_small_fire_argument_range():
         for bad in (-1, 257):
             with pytest.raises(native.SpherocylsError):
                 s.set_small_fire(bad)
+        with pytest.raises(native.SpherocylsError):
+            s.set_small_fire(256, -1)
         s.set_small_fire(0)
         s.set_small_fire(256)
     finally:
