@@ -1995,7 +1995,14 @@ int sc_autotune_fire(sc_system* s, const double* params, double switch_frms, int
       const auto t0 = std::chrono::steady_clock::now();
       e = sc_fire_minimize(s, p.data(), switch_frms, nullptr, nullptr);
       HIPCK(hipStreamSynchronize(s->stream));
-      if (e != SC_OK && e != SC_ERR_NOT_CONVERGED) return e;
+      if (e != SC_OK && e != SC_ERR_NOT_CONVERGED) {
+        // the trial failed (e.g. a non-finite coordinate): put the caller's
+        // positions back, keep the trial's message, report its status
+        const std::string msg = s->err;
+        (void)sc_set_positions(s, x.data(), y.data(), phi.data());
+        s->err = msg;
+        return e;
+      }
       const double ms =
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       score[ci] = std::min(score[ci], ms);

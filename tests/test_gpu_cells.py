@@ -135,3 +135,20 @@ def test_compression_tunes_at_its_first_step():
     print("tuned", c.tuned, "steps", len(c.packing) - 1, flush=True)
     assert c.tuned is not None and c.tuned["path"] in ("list", "tile")
     assert len(c.packing) >= 3 and np.all(c.frms[:, 0] < 1e-10)
+
+
+def test_autotune_fire_reports_a_failing_trial_and_keeps_the_state():
+    """A trial that fails (here FIRE parameters the minimiser rejects:
+    max_move must stay below pad / 2) is reported with the minimiser's
+    status, and the positions are those the tuner started from."""
+    packs = _packs()
+    s = native.System(packs, pads=0.1)
+    try:
+        before = s.positions()
+        with pytest.raises(native.SpherocylsError) as e:
+            s.autotune_fire(native.fire_params(max_move=0.06), iterations=10)
+        assert e.value.status == native.SC_ERR_ARG
+        for u, v in zip(before, s.positions()):
+            np.testing.assert_array_equal(u, v)
+    finally:
+        s.close()
