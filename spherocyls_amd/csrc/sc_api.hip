@@ -1861,7 +1861,13 @@ int sc_autotune_pad(sc_system* s, int ncand, const double* pads, int steps, doub
     if ((e = sc_set_pad(s, -1, pads[c])) != SC_OK || (e = restore()) != SC_OK) return e;
     HIPCK(hipStreamSynchronize(s->stream));
     const auto t0 = std::chrono::steady_clock::now();
-    if ((e = sc_strain_steps(s, steps, 0, nullptr)) != SC_OK) return e;
+    if ((e = sc_strain_steps(s, steps, 0, nullptr)) != SC_OK) {
+      // the trial failed: put the caller's state back, keep the trial's message
+      const std::string msg = s->err;
+      (void)restore();
+      s->err = msg;
+      return e;
+    }
     HIPCK(hipStreamSynchronize(s->stream));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
